@@ -1,0 +1,155 @@
+/*
+ * pyqcm_ed.h -- C-ABI of the MI355X exact-diagonalization impurity solver.
+ *
+ * Plain pointers and sizes only.  Every entry point returns a status code
+ * (ED_OK on success); ed_last_error() gives the message of the last failure
+ * on the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the pyqcm source tree).
+ *
+ * Conventions shared with the reference:
+ *   - spin-orbital indices are 0-based here (src_python/qcm_ED_wrap.hpp:200-204
+ *     converts the 1-based indices users type before calling ED::new_operator);
+ *   - a < L is spin up orbital a, a >= L spin down orbital a - L, L = n_sites + n_bath;
+ *   - sector strings "R<irrep>:N<N>:S<S>" (src_ed/sector.hpp:54-106);
+ *   - complex outputs are interleaved (re, im) doubles; matrices are column-major
+ *     (src_python/qcm_nb.hpp:27-41 copies matrix<T>::v, which is column-major).
+ */
+#ifndef PYQCM_ED_H
+#define PYQCM_ED_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference's qcm_ED_throw triggers, SURVEY of failure paths) */
+enum {
+  ED_OK = 0,
+  ED_ERR_INVALID = 1,        /* bad argument, unknown name, malformed elements */
+  ED_ERR_SECTOR = 2,         /* invalid sector string / sector (model_instance_base.cpp:99-124) */
+  ED_ERR_NOT_CONVERGED = 3,  /* Lanczos / Davidson iteration limit (Lanczos.hpp:92, Davidson.hpp:228) */
+  ED_ERR_QNORM = 4,          /* Q-matrix unitarity violated (Q_matrix.hpp:236-260) */
+  ED_ERR_OOM = 5,            /* device or host allocation failed */
+  ED_ERR_HIP = 6,            /* HIP runtime error */
+  ED_ERR_NOT_FOUND = 7,      /* unknown model / instance label */
+  ED_ERR_UNSUPPORTED = 8,    /* feature outside this build (point groups, complex operators) */
+  ED_ERR_INTERNAL = 9,
+  ED_ERR_DEGENERATE = 10     /* band Lanczos degenerate gap with no_degenerate_BL (Lanczos.hpp:368-370) */
+};
+
+const char *ed_last_error(void);
+int ed_version(char *buf, int cap);
+int ed_device_count(int *n);
+
+/* ---- global parameters: pyqcm.set_global_parameter (pyqcm/__init__.py:2259-2273,
+        src_python/qcm_wrap.hpp:292-324; registry src_util/global_parameter.cpp:44-90) ---- */
+int ed_set_global_bool(const char *name, int value);
+int ed_set_global_int(const char *name, long value);
+int ed_set_global_double(const char *name, double value);
+int ed_set_global_char(const char *name, char value);
+int ed_get_global_double(const char *name, double *value);
+int ed_get_global_int(const char *name, long *value);
+
+/* ---- model / operator / instance: namespace ED (src_ed/qcm_ED.hpp:34-339) ---- */
+/* ED::new_model (src_ed/qcm_ED.cpp:43-59); n_gen must be 0 (trivial point group). */
+int ed_new_model(const char *name, int n_sites, int n_bath, const int *generators, int n_gen, int bath_irrep);
+/* ED::new_operator, real elements (src_ed/qcm_ED.cpp:74-90); type "one-body" or "interaction". */
+int ed_new_operator(const char *model, const char *name, const char *type, int n,
+                    const int *r, const int *c, const double *v);
+/* ED::new_model_instance (src_ed/qcm_ED.cpp:125-158); sectors is slash-separated. */
+int ed_new_model_instance(const char *model, int n_values, const char *const *names, const double *values,
+                          const char *sectors, int label);
+/* ED::erase_model_instance (src_ed/qcm_ED.cpp:68-71) */
+int ed_erase_model_instance(int label);
+/* ED::ground_state_solve (src_ed/qcm_ED.cpp:179-188) -> (E0, "sector:...") */
+int ed_ground_state_solve(int label, double *E0, char *sector, int cap);
+/* ED::Green_function_solve (src_ed/model_instance.hpp:517-555) */
+int ed_green_function_solve(int label);
+/* ED::Green_function_dimension */
+int ed_green_function_dimension(int label, int *d);
+/* ED::Green_function (src_ed/qcm_ED.cpp:230-236): nz frequencies (re,im pairs);
+   out receives nz blocks of d x d complex, column-major. */
+int ed_green_function(int label, int nz, const double *z, int spin_down, int blocks, double *out);
+/* ED::self_energy (src_ed/model_instance.hpp:683-700) */
+int ed_self_energy(int label, int nz, const double *z, int spin_down, double *out);
+/* ED::hybridization_function (src_ed/model_instance.hpp:1004-1018) */
+int ed_hybridization_function(int label, int nz, const double *z, int spin_down, double *out);
+/* ED::Potthoff_functional contribution E0 (src_ed/qcm_ED.cpp:330-335, bath part excluded) */
+int ed_ground_state_energy(int label, double *E0);
+/* ED::cluster_averages (src_ed/qcm_ED.cpp:192-204): for each instance operator
+   (same order as ed_instance_values), the ground-state average and variance. */
+int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n);
+/* ED::qmatrix (src_python/qcm_ED_wrap.hpp:339-349): the Lehmann representation of the
+   first state's Green function (band-Lanczos method); e[M], v[L x M] column-major. */
+int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, int *M);
+/* number of low-energy states and their (energy, weight, sector) */
+int ed_states(int label, int cap, double *energy, double *weight, int *n);
+int ed_mixing(int label, int *mixing);
+
+/* ---- batched solve (north-star batching of sectors / parameter points) ----
+   Solves ground state + Green function of n instances, sharing the device. */
+int ed_solve_batch(const int *labels, int n, int flags);
+
+/* ---- inner plug-in: Hamiltonian<T> (src_ed/Hamiltonian/Hamiltonian_base.hpp:43-82),
+        factory create_hamiltonian (src_ed/model_instance.hpp:192-238) ----
+   format: 'G' (GPU factorized, CSR fallback), 'F' factorized, 'S'/'E' CSR, 'N' on-the-fly. */
+int ed_hamiltonian_create(int label, const char *sector, char format, int64_t *handle);
+int ed_hamiltonian_destroy(int64_t handle);
+int ed_hamiltonian_dim(int64_t handle, int64_t *dim);
+/* y += H x on host arrays (Hamiltonian::mult_add) */
+int ed_hamiltonian_mult_add(int64_t handle, const double *x, double *y);
+/* y = H x + beta y on device arrays, on the library stream (y not read when beta == 0) */
+int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, double beta);
+/* Hamiltonian::diag */
+int ed_hamiltonian_diag(int64_t handle, double *d);
+/* strict-lower CSR pattern of the 'S' format (Hamiltonian_CSR.hpp:60-81): Iptr[dim+1], J[nnz] */
+int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v);
+/* time nrep device applications y = H x (HIP events on the library stream) */
+int ed_hamiltonian_time(int64_t handle, int nrep, double *ms_per_apply, int variant);
+/* kernel selection knob for the factorized format: 0 auto, 1 fused row kernel,
+   2 two-pass (row up-part + column-slab down-part) */
+int ed_hamiltonian_set_variant(int64_t handle, int variant);
+
+/* ---- bases (src_ed/ED_basis.cpp:115-179) and sectors (src_ed/sector.hpp) ---- */
+int ed_half_basis(int L, int N, uint32_t *out, int64_t cap, int64_t *dim);
+int ed_sector_parse(const char *s, int *N, int *S, int *irrep);
+
+/* ---- Krylov drivers on a Hamiltonian handle (host vectors in/out) ---- */
+/* LanczosEigenvalue + LanczosEigenvector (src_ed/Hamiltonian/Lanczos.hpp:41-143) */
+int ed_lanczos_gs(int64_t h, const double *start, double *E0, double *gs, int *niter);
+/* LanczosGreen (src_ed/Hamiltonian/Lanczos.hpp:156-183): a[n], b[n+1] */
+int ed_lanczos_green(int64_t h, const double *psi, int cap, double *a, double *b, int *n);
+/* bandLanczos + build_Q_matrix (Lanczos.hpp:214-395, Hamiltonian_base.hpp:230-254) */
+int ed_band_lanczos(int64_t h, int p, const double *phi, int cap, double *evals, double *Q, int *M);
+/* Davidson (src_ed/Hamiltonian/Davidson.hpp:38-230) */
+int ed_davidson(int64_t h, int K, double *evals, double *evecs, int *niter);
+/* blockLanczos (src_ed/Hamiltonian/Lanczos.hpp:426-572), QR variant */
+int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double *A, double *B, int *nfl);
+/* c_a / c^dagger_a between factorized sectors (src_ed/model.cpp:155-182) */
+int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x, double z, double *y);
+
+/* ---- Green-function kernels on frequency grids ---- */
+/* continued_fraction::evaluate (src_ed/continued_fraction.cpp:52-57): ncf fractions with
+   offsets off[ncf+1] into a[], b[]; out[nz * ncf] complex, fraction-major per z. */
+int ed_cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out);
+/* Q_matrix::Green_function + VDVH (src_ed/Q_matrix.hpp:134-151, src_util/VDVH_kernel.cpp:28-48) */
+int ed_vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G);
+/* matrix_continued_fraction::evaluate (src_ed/matrix_continued_fraction.hpp:139-182) */
+int ed_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz,
+                const double *z, double *G);
+
+/* ---- device memory helpers for callers that keep vectors resident ---- */
+int ed_device_alloc(int64_t bytes, void **ptr);
+int ed_device_free(void *ptr, int64_t bytes);
+int ed_device_copy(void *dst, const void *src, int64_t bytes, int kind /* 0 H2D, 1 D2H, 2 D2D */);
+int ed_device_sync(void);
+int ed_release_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,520 @@
+// ed_capi.cpp -- the extern "C" boundary.  Every entry point converts library
+// exceptions into status codes; no exception crosses the C-ABI.
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/pyqcm_ed.h"
+#include "ed_common.hpp"
+#include "ed_instance.hpp"
+#include "ed_model.hpp"
+#include "ed_solvers.hpp"
+
+using namespace ed;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F &&f) {
+  try {
+    f();
+    return ED_OK;
+  } catch (const ed::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return ED_ERR_OOM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return ED_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return ED_ERR_INTERNAL;
+  }
+}
+
+std::mutex g_h_mu;
+std::map<int64_t, std::unique_ptr<Hamiltonian>> g_hams;
+std::atomic<int64_t> g_next_h{1};
+
+Hamiltonian &get_h(int64_t h) {
+  std::lock_guard<std::mutex> l(g_h_mu);
+  auto it = g_hams.find(h);
+  if (it == g_hams.end()) fail(ED_ERR_NOT_FOUND, "unknown Hamiltonian handle " + std::to_string(h));
+  return *it->second;
+}
+
+void need(bool c, const char *msg) {
+  if (!c) fail(ED_ERR_INVALID, msg);
+}
+}  // namespace
+
+extern "C" {
+
+const char *ed_last_error(void) { return g_last_error.c_str(); }
+
+int ed_version(char *buf, int cap) {
+  return guard([&] {
+    const char *v = "pyqcm_amd ED 0.1 (gfx950, FP64)";
+    need(buf && cap > (int)std::strlen(v), "buffer too small");
+    std::strcpy(buf, v);
+  });
+}
+
+int ed_device_count(int *n) {
+  return guard([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) { (void)hipGetLastError(); c = 0; }
+    *n = c;
+  });
+}
+
+// ---- global parameters ----
+int ed_set_global_bool(const char *name, int value) {
+  return guard([&] {
+    Params &P = params();
+    std::lock_guard<std::mutex> l(P.mu);
+    need(P.b.count(name), "unknown boolean global parameter");
+    P.b[name] = value != 0;
+  });
+}
+int ed_set_global_int(const char *name, long value) {
+  return guard([&] {
+    Params &P = params();
+    std::lock_guard<std::mutex> l(P.mu);
+    need(P.i.count(name), "unknown integer global parameter");
+    P.i[name] = value;
+  });
+}
+int ed_set_global_double(const char *name, double value) {
+  return guard([&] {
+    Params &P = params();
+    std::lock_guard<std::mutex> l(P.mu);
+    need(P.d.count(name), "unknown real global parameter");
+    P.d[name] = value;
+  });
+}
+int ed_set_global_char(const char *name, char value) {
+  return guard([&] {
+    Params &P = params();
+    std::lock_guard<std::mutex> l(P.mu);
+    need(P.c.count(name), "unknown char global parameter");
+    P.c[name] = value;
+  });
+}
+int ed_get_global_double(const char *name, double *value) {
+  return guard([&] { *value = params().getd(name); });
+}
+int ed_get_global_int(const char *name, long *value) {
+  return guard([&] { *value = params().geti(name); });
+}
+
+// ---- model / operator / instance ----
+int ed_new_model(const char *name, int n_sites, int n_bath, const int *generators, int n_gen, int bath_irrep) {
+  return guard([&] {
+    (void)generators;
+    (void)bath_irrep;
+    if (n_gen > 0) fail(ED_ERR_UNSUPPORTED, "cluster point-group generators are not supported (trivial group only)");
+    new_model(name, n_sites, n_bath);
+  });
+}
+
+int ed_new_operator(const char *model, const char *name, const char *type, int n, const int *r, const int *c,
+                    const double *v) {
+  return guard([&] {
+    need(n >= 0, "negative element count");
+    std::vector<Element> el(n);
+    for (int k = 0; k < n; ++k) el[k] = {r[k], c[k], v[k]};
+    new_operator(model, name, type, el);
+  });
+}
+
+int ed_new_model_instance(const char *model, int n_values, const char *const *names, const double *values,
+                          const char *sectors, int label) {
+  return guard([&] {
+    std::map<std::string, double> m;
+    for (int k = 0; k < n_values; ++k) m[names[k]] = values[k];
+    new_instance(model, m, sectors, label);
+  });
+}
+
+int ed_erase_model_instance(int label) {
+  return guard([&] { erase_instance(label); });
+}
+
+int ed_ground_state_solve(int label, double *E0, char *sector, int cap) {
+  return guard([&] {
+    auto r = get_instance(label).ground_state_solve();
+    if (E0) *E0 = r.first;
+    if (sector && cap > 0) {
+      std::strncpy(sector, r.second.c_str(), (size_t)cap - 1);
+      sector[cap - 1] = 0;
+    }
+  });
+}
+
+int ed_ground_state_energy(int label, double *E0) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    I.ground_state_solve();
+    *E0 = I.E0;
+  });
+}
+
+int ed_green_function_solve(int label) {
+  return guard([&] { get_instance(label).green_function_solve(); });
+}
+
+int ed_green_function_dimension(int label, int *d) {
+  return guard([&] { *d = get_instance(label).gf_dim(); });
+}
+
+int ed_green_function(int label, int nz, const double *z, int spin_down, int blocks, double *out) {
+  return guard([&] {
+    (void)blocks;  // trivial group: block form == site form
+    get_instance(label).green_function(nz, z, spin_down != 0, out);
+  });
+}
+
+int ed_self_energy(int label, int nz, const double *z, int spin_down, double *out) {
+  return guard([&] { get_instance(label).self_energy(nz, z, spin_down != 0, out); });
+}
+
+int ed_hybridization_function(int label, int nz, const double *z, int spin_down, double *out) {
+  return guard([&] { get_instance(label).hybridization(nz, z, spin_down != 0, out); });
+}
+
+int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    I.ground_state_solve();
+    *n = (int)I.averages.size();
+    for (int k = 0; k < *n && k < cap; ++k) {
+      ave[k] = std::get<1>(I.averages[k]);
+      var[k] = std::get<2>(I.averages[k]);
+    }
+  });
+}
+
+int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, int *M) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    I.green_function_solve();
+    need(!I.states.empty(), "no state");
+    const GFRep &G = I.states[0]->gf[(spin_down && (I.mixing & 4)) ? 1 : 0];
+    if (G.kind != 'L') fail(ED_ERR_INVALID, "qmatrix needs GF_method 'L'");
+    *L = G.n;
+    *M = (int)G.e.size();
+    if (*M <= cap) {
+      std::memcpy(e, G.e.data(), sizeof(double) * G.e.size());
+      std::memcpy(v, G.V.data(), sizeof(double) * G.V.size());
+    }
+  });
+}
+
+int ed_states(int label, int cap, double *energy, double *weight, int *n) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    I.ground_state_solve();
+    *n = (int)I.states.size();
+    for (int k = 0; k < *n && k < cap; ++k) {
+      energy[k] = I.states[k]->energy;
+      weight[k] = I.states[k]->weight;
+    }
+  });
+}
+
+int ed_mixing(int label, int *mixing) {
+  return guard([&] { *mixing = get_instance(label).mixing; });
+}
+
+int ed_solve_batch(const int *labels, int n, int flags) {
+  return guard([&] {
+    for (int k = 0; k < n; ++k) {
+      Instance &I = get_instance(labels[k]);
+      I.ground_state_solve();
+      if (flags & 1) I.green_function_solve();
+    }
+  });
+}
+
+// ---- Hamiltonian handles ----
+int ed_hamiltonian_create(int label, const char *sector, char format, int64_t *handle) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    Sector s = Sector::parse(sector);
+    if (!sector_valid(I.model->L, s)) fail(ED_ERR_SECTOR, std::string("sector ") + sector + " is not valid");
+    auto H = make_hamiltonian(I.terms(), s, format ? format : I.format);
+    const int64_t id = g_next_h++;
+    std::lock_guard<std::mutex> l(g_h_mu);
+    g_hams[id] = std::move(H);
+    *handle = id;
+  });
+}
+
+int ed_hamiltonian_destroy(int64_t handle) {
+  return guard([&] {
+    std::lock_guard<std::mutex> l(g_h_mu);
+    g_hams.erase(handle);
+  });
+}
+
+int ed_hamiltonian_dim(int64_t handle, int64_t *dim) {
+  return guard([&] { *dim = (int64_t)get_h(handle).dim; });
+}
+
+int ed_hamiltonian_mult_add(int64_t handle, const double *x, double *y) {
+  return guard([&] {
+    Hamiltonian &H = get_h(handle);
+    DevBuf<double> dx, dy;
+    dx.upload(x, H.dim);
+    dy.upload(y, H.dim);
+    H.apply(dx.p, dy.p, 1.0, 1.0, false);
+    dy.download(y, H.dim);
+  });
+}
+
+int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, double beta) {
+  return guard([&] {
+    Hamiltonian &H = get_h(handle);
+    H.apply(dx, dy, 1.0, beta, false);
+    sync();
+  });
+}
+
+int ed_hamiltonian_diag(int64_t handle, double *d) {
+  return guard([&] {
+    Hamiltonian &H = get_h(handle);
+    DevBuf<double> dd(H.dim);
+    H.diag(dd.p);
+    dd.download(d, H.dim);
+  });
+}
+
+int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v) {
+  return guard([&] {
+    Hamiltonian &H = get_h(handle);
+    std::vector<int32_t> ip, jj;
+    std::vector<double> vv, dg;
+    csr_lower(H.terms, H.Nup, H.Ndw, ip, jj, vv, dg);
+    const int64_t cap = *nnz;
+    *nnz = (int64_t)jj.size();
+    if (Iptr && J && cap >= (int64_t)jj.size()) {
+      std::memcpy(Iptr, ip.data(), sizeof(int32_t) * ip.size());
+      std::memcpy(J, jj.data(), sizeof(int32_t) * jj.size());
+      if (v) std::memcpy(v, vv.data(), sizeof(double) * vv.size());
+    }
+  });
+}
+
+int ed_hamiltonian_set_variant(int64_t handle, int variant) {
+  return guard([&] { get_h(handle).variant = variant; });
+}
+
+int ed_hamiltonian_time(int64_t handle, int nrep, double *ms, int variant) {
+  return guard([&] {
+    Hamiltonian &H = get_h(handle);
+    const int old = H.variant;
+    H.variant = variant;
+    DevBuf<double> x(H.dim), y(H.dim);
+    fill(H.dim, x.p, 1.0 / std::sqrt((double)H.dim), stream());
+    fill(H.dim, y.p, 0.0, stream());
+    for (int w = 0; w < 2; ++w) H.apply(x.p, y.p, 1.0, 0.0, false);
+    hipEvent_t e0, e1;
+    ED_HIP(hipEventCreate(&e0));
+    ED_HIP(hipEventCreate(&e1));
+    ED_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < nrep; ++r) H.apply(x.p, y.p, 1.0, 0.0, false);
+    ED_HIP(hipEventRecord(e1, stream()));
+    ED_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    ED_HIP(hipEventElapsedTime(&t, e0, e1));
+    ED_HIP(hipEventDestroy(e0));
+    ED_HIP(hipEventDestroy(e1));
+    *ms = (double)t / std::max(1, nrep);
+    H.variant = old;
+  });
+}
+
+// ---- bases and sectors ----
+int ed_half_basis(int L, int N, uint32_t *out, int64_t cap, int64_t *dim) {
+  return guard([&] {
+    auto B = half_basis(L, N);
+    *dim = B->dim;
+    if (out && cap >= (int64_t)B->dim) std::memcpy(out, B->bin.data(), sizeof(uint32_t) * B->dim);
+  });
+}
+
+int ed_sector_parse(const char *s, int *N, int *S, int *irrep) {
+  return guard([&] {
+    Sector x = Sector::parse(s);
+    *N = x.N;
+    *S = x.S;
+    *irrep = x.irrep;
+  });
+}
+
+// ---- Krylov drivers on handles (host vectors) ----
+int ed_lanczos_gs(int64_t h, const double *start, double *E0, double *gs, int *niter) {
+  return guard([&] {
+    Hamiltonian &H = get_h(h);
+    DevBuf<double> d0, dg(H.dim);
+    d0.upload(start, H.dim);
+    LanczosGS R = lanczos_gs(H, d0.p, dg.p);
+    *E0 = R.E0;
+    if (niter) *niter = R.niter;
+    if (gs) dg.download(gs, H.dim);
+  });
+}
+
+int ed_lanczos_green(int64_t h, const double *psi, int cap, double *a, double *b, int *n) {
+  return guard([&] {
+    Hamiltonian &H = get_h(h);
+    DevBuf<double> d0;
+    d0.upload(psi, H.dim);
+    std::vector<double> av, bv;
+    lanczos_green(H, d0.p, av, bv);
+    *n = (int)av.size();
+    if ((int)av.size() <= cap) {
+      std::memcpy(a, av.data(), sizeof(double) * av.size());
+      std::memcpy(b, bv.data(), sizeof(double) * bv.size());
+    }
+  });
+}
+
+int ed_band_lanczos(int64_t h, int p, const double *phi, int cap, double *evals, double *Q, int *M) {
+  return guard([&] {
+    Hamiltonian &H = get_h(h);
+    std::vector<DevBuf<double>> bufs;
+    std::vector<const double *> ptrs;
+    for (int k = 0; k < p; ++k) {
+      bufs.emplace_back();
+      bufs.back().upload(phi + (size_t)k * H.dim, H.dim);
+    }
+    for (auto &b : bufs) ptrs.push_back(b.p);
+    std::vector<double> e, q;
+    band_lanczos(H, ptrs, e, q);
+    *M = (int)e.size();
+    if (*M <= cap) {
+      std::memcpy(evals, e.data(), sizeof(double) * e.size());
+      std::memcpy(Q, q.data(), sizeof(double) * q.size());
+    }
+  });
+}
+
+int ed_davidson(int64_t h, int K, double *evals, double *evecs, int *niter) {
+  return guard([&] {
+    Hamiltonian &H = get_h(h);
+    std::vector<double> ev;
+    std::vector<DevBuf<double>> vecs;
+    const int it = davidson(H, K, params().getd("accur_Davidson"), ev, vecs);
+    for (int k = 0; k < (int)ev.size(); ++k) {
+      evals[k] = ev[k];
+      if (evecs) vecs[k].download(evecs + (size_t)k * H.dim, H.dim);
+    }
+    if (niter) *niter = it;
+  });
+}
+
+int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double *A, double *B, int *nfl) {
+  return guard([&] {
+    Hamiltonian &H = get_h(h);
+    std::vector<DevBuf<double>> bufs;
+    std::vector<const double *> ptrs;
+    for (int k = 0; k < p; ++k) {
+      bufs.emplace_back();
+      bufs.back().upload(q0 + (size_t)k * H.dim, H.dim);
+    }
+    for (auto &b : bufs) ptrs.push_back(b.p);
+    std::vector<double> Av, Bv;
+    const int n = block_lanczos(H, ptrs, M0, Av, Bv);
+    *nfl = n;
+    if (n <= cap) {
+      std::memcpy(A, Av.data(), sizeof(double) * Av.size());
+      std::memcpy(B, Bv.data(), sizeof(double) * Bv.size());
+    }
+  });
+}
+
+int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x, double z, double *y) {
+  return guard([&] {
+    need(pm == 1 || pm == -1, "pm must be +1 or -1");
+    need(a >= 0 && a < 2 * L, "orbital out of range");
+    Sector src(Nup + Ndw, Nup - Ndw, 0);
+    const Sector tgt = shift_sector(src, pm, a >= L);
+    const size_t ds = sector_dim(L, src), dt = sector_dim(L, tgt);
+    DevBuf<double> dx, dy;
+    dx.upload(x, ds);
+    dy.upload(y, dt);
+    if (dt && ds) cdag_apply(L, pm, a, src, dx.p, z, dy.p);
+    dy.download(y, dt);
+  });
+}
+
+// ---- GF kernels ----
+int ed_cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out) {
+  return guard([&] {
+    DevBuf<int> doff;
+    DevBuf<double> da, db, dz, dout((size_t)2 * ncf * nz + 1);
+    doff.upload(off, (size_t)ncf + 1);
+    const size_t na = off[ncf];
+    std::vector<double> av(a, a + na), bv(b, b + na);
+    da.upload(av.empty() ? std::vector<double>{0.0} : av);
+    db.upload(bv.empty() ? std::vector<double>{0.0} : bv);
+    dz.upload(z, (size_t)2 * nz);
+    cf_eval(ncf, doff.p, da.p, db.p, nz, dz.p, dout.p, stream());
+    dout.download(out, (size_t)2 * ncf * nz);
+  });
+}
+
+int ed_vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G) {
+  return guard([&] {
+    DevBuf<double> de, dV, dz, dG((size_t)2 * nz * L * L + 1);
+    de.upload(e, M);
+    dV.upload(V, (size_t)L * M);
+    dz.upload(z, (size_t)2 * nz);
+    vdvh(L, M, de.p, dV.p, nz, dz.p, dG.p, stream());
+    dG.download(G, (size_t)2 * nz * L * L);
+  });
+}
+
+int ed_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz, const double *z,
+                double *G) {
+  return guard([&] {
+    DevBuf<double> dA, dB, dW, dz, dG((size_t)2 * nz * q * q + 1);
+    dA.upload(A, (size_t)nfl * p * p);
+    dB.upload(B, (size_t)nfl * p * p);
+    dW.upload(W, (size_t)p * q);
+    dz.upload(z, (size_t)2 * nz);
+    mcf_eval(p, nfl, dA.p, dB.p, dW.p, q, nz, dz.p, dG.p, stream());
+    dG.download(G, (size_t)2 * nz * q * q);
+  });
+}
+
+// ---- device memory ----
+int ed_device_alloc(int64_t bytes, void **ptr) {
+  return guard([&] { *ptr = dev_alloc((size_t)bytes); });
+}
+int ed_device_free(void *ptr, int64_t bytes) {
+  return guard([&] { dev_free(ptr, (size_t)bytes); });
+}
+int ed_device_copy(void *dst, const void *src, int64_t bytes, int kind) {
+  return guard([&] {
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    ED_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, stream()));
+    sync();
+  });
+}
+int ed_device_sync(void) {
+  return guard([&] { sync(); });
+}
+int ed_release_cache(void) {
+  return guard([&] { dev_release_cache(); });
+}
+
+}  // extern "C"

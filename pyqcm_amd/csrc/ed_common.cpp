@@ -1,0 +1,148 @@
+// ed_common.cpp -- global parameters, caching device allocator, library stream.
+#include "ed_common.hpp"
+
+#include <unordered_map>
+
+namespace ed {
+
+Params::Params() {
+  // src_util/global_parameter.cpp:44-90 (the ED knobs, same names and defaults)
+  b["check_lanczos_residual"] = false;
+  b["no_degenerate_BL"] = false;
+  b["nosym"] = false;
+  b["continued_fraction"] = false;
+  b["verb_ED"] = false;
+  b["verb_Hilbert"] = false;
+  b["verb_warning"] = false;
+  b["parallel_sectors"] = false;
+  b["merge_states"] = false;
+  b["combine_mcf"] = false;
+  b["block_Lanczos_QR"] = false;
+  d["accur_band_lanczos"] = 1e-12;
+  d["accur_continued_fraction"] = 0.01;
+  d["accur_Davidson"] = 1.0e-5;
+  d["accur_deflation"] = 1e-7;
+  d["accur_lanczos"] = 1e-12;
+  d["accur_Q_matrix"] = 1.0e-5;
+  d["band_lanczos_minimum_gap"] = 1e-5;
+  d["minimum_weight"] = 0.01;
+  d["Qmatrix_wtol"] = 1.0e-5;
+  d["Qmatrix_vtol"] = 1.0e-10;
+  d["temperature"] = 0.0;
+  i["n_states"] = 1;
+  i["max_dim_full"] = 256;
+  i["max_iter_BL"] = 600;
+  i["max_iter_CF"] = 400;
+  i["max_iter_lanczos"] = 1000;
+  i["seed"] = 0;
+  i["GF_lookup_depth"] = 64;
+  // 'G' (added here): GPU factorized format when the model is factorizable,
+  // device CSR otherwise.  'F', 'S', 'E', 'N' keep their reference meaning.
+  c["Hamiltonian_format"] = 'G';
+  c["Ground_state_method"] = 'L';
+  c["GF_method"] = 'L';
+  // device-side knobs (no reference counterpart)
+  i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
+  b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
+  d["lanczos_basis_budget_GB"] = 96.0;
+}
+
+bool Params::getb(const std::string &n) {
+  std::lock_guard<std::mutex> l(mu);
+  auto it = b.find(n);
+  if (it == b.end()) fail(ED_ERR_INVALID, "unknown boolean global parameter " + n);
+  return it->second;
+}
+long Params::geti(const std::string &n) {
+  std::lock_guard<std::mutex> l(mu);
+  auto it = i.find(n);
+  if (it == i.end()) fail(ED_ERR_INVALID, "unknown integer global parameter " + n);
+  return it->second;
+}
+double Params::getd(const std::string &n) {
+  std::lock_guard<std::mutex> l(mu);
+  auto it = d.find(n);
+  if (it == d.end()) fail(ED_ERR_INVALID, "unknown real global parameter " + n);
+  return it->second;
+}
+char Params::getc(const std::string &n) {
+  std::lock_guard<std::mutex> l(mu);
+  auto it = c.find(n);
+  if (it == c.end()) fail(ED_ERR_INVALID, "unknown char global parameter " + n);
+  return it->second;
+}
+
+Params &params() {
+  static Params P;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// caching allocator: exact-size free lists (Krylov vectors of one sector all
+// have the same size, so reuse is near perfect); freed on ed_release_cache
+// or when an allocation fails.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_alloc_mu;
+std::unordered_multimap<size_t, void *> g_free;
+size_t g_cached_bytes = 0;
+
+size_t round_up(size_t b) {
+  const size_t a = 256;
+  return (b + a - 1) / a * a;
+}
+}  // namespace
+
+void dev_release_cache() {
+  std::lock_guard<std::mutex> l(g_alloc_mu);
+  for (auto &kv : g_free) (void)hipFree(kv.second);
+  g_free.clear();
+  g_cached_bytes = 0;
+}
+
+void *dev_alloc(size_t bytes) {
+  size_t b = round_up(bytes);
+  {
+    std::lock_guard<std::mutex> l(g_alloc_mu);
+    auto it = g_free.find(b);
+    if (it != g_free.end()) {
+      void *p = it->second;
+      g_free.erase(it);
+      g_cached_bytes -= b;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    dev_release_cache();
+    e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(ED_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  return p;
+}
+
+void dev_free(void *p, size_t bytes) {
+  if (!p) return;
+  size_t b = round_up(bytes);
+  std::lock_guard<std::mutex> l(g_alloc_mu);
+  g_free.emplace(b, p);
+  g_cached_bytes += b;
+}
+
+hipStream_t stream() {
+  static hipStream_t s = [] {
+    hipStream_t st;
+    ED_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    return st;
+  }();
+  return s;
+}
+
+void sync() { ED_HIP(hipStreamSynchronize(stream())); }
+
+}  // namespace ed

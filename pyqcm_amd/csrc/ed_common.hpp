@@ -1,0 +1,106 @@
+// ed_common.hpp -- status codes, errors, global parameters, device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pyqcm_ed.h"
+
+namespace ed {
+
+// Exception carried inside the library; converted to a status code at the
+// C-ABI boundary (no exception crosses it).  Replaces qcm_ED_throw
+// (src_python/qcm_lib.cpp:71-74).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define ED_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t _e = (call);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      ::ed::fail(ED_ERR_HIP, std::string("HIP error ") + hipGetErrorString(_e) + " at " + \
+                                 __FILE__ + ":" + std::to_string(__LINE__));               \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Global parameters: same names and defaults as the reference registry
+// (src_util/global_parameter.cpp:44-90), read once per call.
+// ---------------------------------------------------------------------------
+struct Params {
+  std::map<std::string, bool> b;
+  std::map<std::string, long> i;
+  std::map<std::string, double> d;
+  std::map<std::string, char> c;
+  std::mutex mu;
+  Params();
+  bool getb(const std::string &n);
+  long geti(const std::string &n);
+  double getd(const std::string &n);
+  char getc(const std::string &n);
+};
+Params &params();
+
+// ---------------------------------------------------------------------------
+// Device memory: a size-bucketed caching allocator so that repeated solves do
+// not pay hipMalloc/hipFree on every Krylov vector.
+// ---------------------------------------------------------------------------
+void *dev_alloc(size_t bytes);
+void dev_free(void *p, size_t bytes);
+void dev_release_cache();
+hipStream_t stream();  // the library's compute stream (default priority)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) p = static_cast<T *>(dev_alloc(count * sizeof(T)));
+  }
+  void release() {
+    if (p) dev_free(p, n * sizeof(T));
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count) {
+    if (count > n) alloc(count);
+    if (count) ED_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+  }
+  void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
+  void download(T *h, size_t count) const {
+    if (count) ED_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+    ED_HIP(hipStreamSynchronize(stream()));
+  }
+  std::vector<T> to_host() const {
+    std::vector<T> h(n);
+    download(h.data(), n);
+    return h;
+  }
+};
+
+void sync();
+
+}  // namespace ed

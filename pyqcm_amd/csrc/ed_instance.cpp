@@ -1,0 +1,688 @@
+// ed_instance.cpp -- low-energy states, Green-function build and evaluation.
+#include "ed_instance.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iomanip>
+#include <mutex>
+#include <random>
+#include <sstream>
+
+#include "ed_solvers.hpp"
+
+namespace ed {
+
+namespace {
+std::mutex g_inst_mu;
+std::map<int, std::unique_ptr<Instance>> g_instances;
+constexpr double kSmall = 1.0e-12;  // SMALL_VALUE (src_util/vector_num.hpp:23)
+constexpr double kMinGap = 1.0e-6;  // MIN_GAP (src_ed/model.hpp:19)
+}  // namespace
+
+Instance &get_instance(int label) {
+  std::lock_guard<std::mutex> l(g_inst_mu);
+  auto it = g_instances.find(label);
+  if (it == g_instances.end())
+    fail(ED_ERR_NOT_FOUND, "The cluster instance label " + std::to_string(label) + " is out of range.");
+  return *it->second;
+}
+
+void erase_instance(int label) {
+  std::lock_guard<std::mutex> l(g_inst_mu);
+  g_instances.erase(label);
+}
+
+// src_ed/symmetry_group.cpp:515-533 (g = 1)
+bool sector_valid(int L, const Sector &s) {
+  if (s.N < Sector::odd) {
+    if (s.N < 0 || s.N > 2 * L) return false;
+    if (s.S < Sector::odd) {
+      if (s.S > s.N || s.S < -s.N || ((s.N + s.S) % 2)) return false;
+    }
+  }
+  if (s.S < Sector::odd && (s.S > L || s.S < -L)) return false;
+  return s.irrep == 0;
+}
+
+// src_ed/symmetry_group.cpp:540-558 (g = 1)
+Sector shift_sector(const Sector &s0, int pm, int spin) {
+  Sector s = s0;
+  if (s.N == Sector::odd) s.N = Sector::even;
+  else if (s.N == Sector::even) s.N = Sector::odd;
+  else s.N += pm;
+  if (s.S == Sector::odd) s.S = Sector::even;
+  else if (s.S == Sector::even) s.S = Sector::odd;
+  else s.S += spin ? -pm : pm;
+  return s;
+}
+
+size_t sector_dim(int L, const Sector &s) {
+  const int nu = s.Nup(), nd = s.Ndw();
+  if (nu < 0 || nd < 0 || nu > L || nd > L) return 0;
+  return (size_t)half_basis(L, nu)->dim * half_basis(L, nd)->dim;
+}
+
+// ED::new_model_instance + model_instance_base ctor (src_ed/qcm_ED.cpp:125-158,
+// src_ed/model_instance_base.cpp:14-130), trivial point group only.
+void new_instance(const std::string &model, const std::map<std::string, double> &values, const std::string &sectors,
+                  int label) {
+  Model &M = get_model(model);
+  M.closed = true;
+  auto I = std::make_unique<Instance>();
+  I->label = label;
+  I->model = &M;
+  for (const auto &kv : values)
+    if (M.ops.count(kv.first)) I->value[kv.first] = kv.second;
+  if (I->value.empty())
+    fail(ED_ERR_INVALID, "ED model instance (label = " + std::to_string(label) + ") for model " + model +
+                             " has no parameters!");
+  Params &P = params();
+  char gf = P.getc("GF_method");
+  if (gf == 'L' && P.getb("continued_fraction")) gf = 'F';
+  I->gf_method = gf;
+  I->format = P.getc("Hamiltonian_format");
+  bool asym = false;
+  for (const auto &kv : I->value) {
+    const Operator &op = M.ops.at(kv.first);
+    if (op.spin_flip && kv.second != 0.0)
+      fail(ED_ERR_UNSUPPORTED, "operator " + op.name + " mixes spins: spin-flip mixing is not supported");
+    if (op.spin_asym && kv.second != 0.0) asym = true;
+  }
+  I->mixing = asym ? 4 : 0;
+  std::stringstream ss(sectors);
+  std::string item;
+  while (std::getline(ss, item, '/')) {
+    if (item.empty()) continue;
+    Sector s = Sector::parse(item);
+    if (!sector_valid(M.L, s)) fail(ED_ERR_SECTOR, "sector " + item + " is not valid");
+    if (s.N >= Sector::odd || s.S >= Sector::odd)
+      fail(ED_ERR_UNSUPPORTED, "sector string " + s.name() + " defines a non conserved quantity, not supported");
+    if (s.S && I->mixing == 0) I->mixing = 4;
+    I->targets.insert(s);
+  }
+  if (I->targets.empty()) fail(ED_ERR_SECTOR, "no target sectors were specified for model " + model);
+  std::lock_guard<std::mutex> l(g_inst_mu);
+  g_instances[label] = std::move(I);
+}
+
+TermList Instance::terms() const {
+  TermList T;
+  T.L = model->L;
+  for (const auto &kv : value) T.terms.push_back({&model->ops.at(kv.first), kv.second});
+  return T;
+}
+
+std::string Instance::gs_string() const {
+  std::map<Sector, double> ws;
+  for (const auto &s : states) ws[s->sec] += s->weight;
+  std::ostringstream o;
+  bool first = true;
+  for (const auto &x : ws) {
+    if (!first) o << '/';
+    first = false;
+    o << x.first.name() << ':' << std::setprecision(3) << x.second;
+  }
+  return o.str();
+}
+
+// ---------------------------------------------------------------------------
+// c_a / c^dagger_a on factorized vectors (src_ed/model.cpp:155-182,
+// src_ed/ED_basis.cpp:79-110): gather over the target sector.
+// ---------------------------------------------------------------------------
+Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, double z, double *y) {
+  const int spin = a >= L ? 1 : 0;
+  const int orb = a - spin * L;
+  Sector tgt = shift_sector(src, pm, spin);
+  auto BUs = half_basis(L, src.Nup()), BDs = half_basis(L, src.Ndw());
+  auto BUt = half_basis(L, tgt.Nup()), BDt = half_basis(L, tgt.Ndw());
+  const HalfBasis &Bt = spin ? *BDt : *BUt;   // changed spin, target
+  const HalfBasis &Bs = spin ? *BDs : *BUs;   // changed spin, source
+  std::vector<int32_t> map(Bt.dim, -1);
+  std::vector<int8_t> sgn(Bt.dim, 1), upar;
+  const uint32_t m = 1u << orb;
+  for (uint32_t t = 0; t < Bt.dim; ++t) {
+    const uint32_t st = Bt.bin[t];
+    uint32_t ss;
+    if (pm == -1) {  // target lacks orb, source has it
+      if (st & m) continue;
+      ss = st | m;
+    } else {  // target has orb, source lacks it
+      if (!(st & m)) continue;
+      ss = st & ~m;
+    }
+    const uint32_t k = Bs.index(ss);
+    if (k == Bs.dim) continue;
+    map[t] = (int32_t)k;
+    sgn[t] = (__builtin_popcount(st & (m - 1u)) & 1) ? -1 : 1;
+  }
+  if (spin) {
+    upar.resize(BUt->dim);
+    for (uint32_t u = 0; u < BUt->dim; ++u) upar[u] = (__builtin_popcount(BUt->bin[u]) & 1) ? -1 : 1;
+  }
+  DevBuf<int32_t> dmap;
+  DevBuf<int8_t> dsgn, dupar;
+  dmap.upload(map.empty() ? std::vector<int32_t>{-1} : map);
+  dsgn.upload(sgn.empty() ? std::vector<int8_t>{1} : sgn);
+  if (spin) dupar.upload(upar.empty() ? std::vector<int8_t>{1} : upar);
+  CdagArgs A;
+  A.spin = spin;
+  A.du_t = BUt->dim;
+  A.dd_t = BDt->dim;
+  A.du_s = BUs->dim;
+  A.map = dmap.p;
+  A.sgn = dsgn.p;
+  A.upar = spin ? dupar.p : nullptr;
+  A.z = z;
+  cdag_gather(A, x, y, stream());
+  sync();
+  return tgt;
+}
+
+// ---------------------------------------------------------------------------
+// ground state (model_instance<T>::low_energy_states, src_ed/model_instance.hpp:245-347;
+// Hamiltonian<T>::states, src_ed/Hamiltonian/Hamiltonian_base.hpp:115-221;
+// compute_weights, src_ed/model_instance.hpp:356-417)
+// ---------------------------------------------------------------------------
+static void random_start(size_t n, long seed, std::vector<double> &x) {
+  // random() of src_util/vector_num.cpp:114-119: default_random_engine(seed),
+  // N(0,1) draws, normalized
+  std::default_random_engine gen((unsigned)seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  x.resize(n);
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    x[i] = nd(gen);
+    s += x[i] * x[i];
+  }
+  const double f = 1.0 / std::sqrt(s);
+  for (auto &v : x) v *= f;
+}
+
+std::pair<double, std::string> Instance::ground_state_solve() {
+  if (gs_solved) return {GS_energy, gs_string()};
+  Params &P = params();
+  const double temperature = P.getd("temperature");
+  double max_gap = -std::log(P.getd("minimum_weight")) * temperature;
+  if (max_gap < kMinGap) max_gap = kMinGap;
+  const long seed = P.geti("seed");
+  long n_states = P.geti("n_states");
+  char method = P.getc("Ground_state_method");
+  const double acc_dav = P.getd("accur_Davidson");
+  TermList T = terms();
+  states.clear();
+  GS_energy = 1.0e12;
+  for (const auto &s : targets) {
+    const size_t dim = sector_dim(model->L, s);
+    if (dim == 0) continue;
+    auto H = make_hamiltonian(T, s, format);
+    long ns = std::max<long>(1, std::min<long>(n_states, (long)dim));
+    char meth = (method == 'L' && ns > 1) ? 'D' : method;
+    std::vector<double> ev;
+    std::vector<DevBuf<double>> vecs;
+    if (meth == 'D') {
+      davidson(*H, (int)ns, acc_dav, ev, vecs);
+    } else if (meth == 'L') {
+      std::vector<double> x0;
+      random_start(dim, seed, x0);
+      DevBuf<double> d0;
+      d0.upload(x0);
+      vecs.emplace_back(dim);
+      LanczosGS R = lanczos_gs(*H, d0.p, vecs[0].p);
+      ev.push_back(R.E0);
+    } else {
+      fail(ED_ERR_INVALID, std::string("Unknown method in Ground_state_method: ") + meth);
+    }
+    if (ev[0] < GS_energy) GS_energy = ev[0];
+    for (size_t i = 0; i < vecs.size(); ++i) {
+      if (ev[i] - GS_energy > max_gap) continue;
+      auto st = std::make_unique<State>();
+      st->sec = s;
+      st->energy = ev[i];
+      // normalize the state (the reference keeps the Lanczos Ritz vector as is)
+      const double nn = std::sqrt(ddot(dim, vecs[i].p, vecs[i].p));
+      scale(dim, vecs[i].p, 1.0 / nn, stream());
+      st->psi = std::move(vecs[i]);
+      states.push_back(std::move(st));
+    }
+  }
+  if (states.empty()) fail(ED_ERR_SECTOR, "instance " + std::to_string(label) + " has no low-energy state!");
+  // compute_weights
+  if (temperature < 1.0e-8) {
+    double e0 = 1e12;
+    for (auto &x : states) e0 = std::min(e0, x->energy);
+    std::vector<std::unique_ptr<State>> keep;
+    for (auto &x : states)
+      if (x->energy - e0 <= kSmall) keep.push_back(std::move(x));
+    states.swap(keep);
+    for (auto &x : states) x->weight = 1.0 / states.size();
+  } else {
+    std::vector<std::unique_ptr<State>> keep;
+    for (auto &x : states)
+      if (x->energy - GS_energy <= max_gap) keep.push_back(std::move(x));
+    states.swap(keep);
+    double Z = 0.0;
+    for (auto &x : states) {
+      x->weight = std::exp(-(x->energy - GS_energy) / temperature);
+      Z += x->weight;
+    }
+    for (auto &x : states) x->weight /= Z;
+  }
+  // minimum_weight filter and renormalization
+  const double minw = P.getd("minimum_weight");
+  {
+    std::vector<std::unique_ptr<State>> keep;
+    double tot = 0.0;
+    for (auto &x : states)
+      if (x->weight >= minw) { tot += x->weight; keep.push_back(std::move(x)); }
+    states.swap(keep);
+    for (auto &x : states) x->weight /= tot;
+  }
+  for (auto &x : states)
+    if (x->sec.S && mixing == 0) mixing = 4;
+  GS_energy = 1.0e18;
+  E0 = 0.0;
+  for (auto &x : states) {
+    E0 += x->weight * x->energy;
+    GS_energy = std::min(GS_energy, x->energy);
+  }
+  // averages and variances of every instance operator (Hermitian_operator.hpp:67-83)
+  averages.clear();
+  for (const auto &kv : value) {
+    double ave = 0.0, var = 0.0;
+    TermList one;
+    one.L = model->L;
+    one.terms.push_back({&model->ops.at(kv.first), 1.0});
+    for (auto &x : states) {
+      auto Ho = make_hamiltonian(one, x->sec, format);
+      DevBuf<double> y(Ho->dim);
+      const int nb = Ho->apply(x->psi.p, y.p, 1.0, 0.0, true);
+      const double e = Ho->dot_result(nb);
+      const double e2 = ddot(Ho->dim, y.p, y.p);
+      ave += x->weight * e;
+      var += x->weight * e2;
+    }
+    averages.emplace_back(kv.first, ave, var - ave * ave);
+  }
+  gs_solved = true;
+  return {GS_energy, gs_string()};
+}
+
+// ---------------------------------------------------------------------------
+// Green-function build
+// ---------------------------------------------------------------------------
+static void streamline(int L, std::vector<double> &e, std::vector<double> &V) {
+  // Q_matrix::streamline (src_ed/Q_matrix.hpp:181-223, :286-316)
+  Params &P = params();
+  const double wtol = P.getd("Qmatrix_wtol"), vtol = P.getd("Qmatrix_vtol");
+  if (vtol < 1e-12) return;
+  const int M = (int)e.size();
+  std::vector<double> e2, v2;
+  for (int i = 0; i < M; ++i) {
+    int j;
+    for (j = i + 1; j < M; ++j)
+      if (std::fabs(e[j] - e[i]) > wtol) break;
+    const int n = j - i;
+    if (n == 1) {
+      double s = 0.0;
+      for (int a = 0; a < L; ++a) s += V[a + (size_t)i * L] * V[a + (size_t)i * L];
+      if (!(s < vtol)) {
+        e2.push_back(e[i]);
+        v2.insert(v2.end(), V.begin() + (size_t)i * L, V.begin() + (size_t)(i + 1) * L);
+      }
+    } else {
+      std::vector<double> QQ((size_t)L * L, 0.0), d(L), U((size_t)L * L);
+      for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+          double s = 0.0;
+          for (int k = 0; k < n; ++k) s += V[a + (size_t)(i + k) * L] * V[b + (size_t)(i + k) * L];
+          QQ[a + (size_t)b * L] = s;
+        }
+      symeig(L, QQ.data(), d.data(), U.data());
+      for (int c = 0; c < L; ++c)
+        if (d[c] > vtol) {
+          e2.push_back(e[i]);
+          const double sq = std::sqrt(d[c]);
+          for (int a = 0; a < L; ++a) v2.push_back(U[a + (size_t)c * L] * sq);
+        }
+    }
+    i += n - 1;
+  }
+  e.swap(e2);
+  V.swap(v2);
+}
+
+static double qnorm_error(int L, const std::vector<double> &V) {
+  // Q_matrix::check_norm (src_ed/Q_matrix.hpp:236-260)
+  const size_t M = V.size() / std::max(L, 1);
+  double tot = 0.0;
+  for (int i = 0; i < L; ++i) {
+    double z = 0.0;
+    for (size_t k = 0; k < M; ++k) z += V[i + k * L] * V[i + k * L];
+    tot += (z - 1.0) * (z - 1.0);
+    for (int j = 0; j < i; ++j) {
+      double zz = 0.0;
+      for (size_t k = 0; k < M; ++k) zz += V[i + k * L] * V[j + k * L];
+      tot += 2.0 * zz * zz;
+    }
+  }
+  return tot;
+}
+
+static void build_gf(Instance &I, State &st, int spin) {
+  const int L = I.model->L, ns = I.model->n_sites;
+  const TermList T = I.terms();
+  const size_t dim0 = sector_dim(L, st.sec);
+  GFRep &G = st.gf[spin];
+  G = GFRep();
+  G.kind = I.gf_method;
+  G.n = ns;
+  const int np = ns * (ns + 1) / 2;
+  if (G.kind == 'F') { G.ea.assign(np, {}); G.eb.assign(np, {}); G.ha.assign(np, {}); G.hb.assign(np, {}); }
+  std::vector<double> eh, Vh, ee, Ve;  // Lehmann parts (hole, electron)
+  for (int pm = -1; pm <= 1; pm += 2) {
+    Sector tgt = shift_sector(st.sec, pm, spin);
+    if (!sector_valid(L, tgt)) continue;
+    const size_t dt = sector_dim(L, tgt);
+    if (dt == 0) continue;
+    auto H = make_hamiltonian(T, tgt, I.format);
+    // phi_i = c^{pm}_{i, spin} |Omega>
+    std::vector<DevBuf<double>> phi;
+    for (int i = 0; i < ns; ++i) {
+      phi.emplace_back(dt);
+      fill(dt, phi.back().p, 0.0, stream());
+      cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p, 1.0, phi.back().p);
+    }
+    (void)dim0;
+    if (G.kind == 'L') {
+      std::vector<const double *> ph;
+      for (auto &b : phi) ph.push_back(b.p);
+      std::vector<double> e, Q;
+      band_lanczos(*H, ph, e, Q);
+      for (auto &x : e) x -= st.energy;
+      if (pm == -1)
+        for (auto &x : e) x = -x;
+      streamline(ns, e, Q);
+      if (pm == -1) { eh = e; Vh = Q; }
+      else { ee = e; Ve = Q; }
+    } else if (G.kind == 'F') {
+      // build_cf (src_ed/model_instance.cpp:35-100)
+      DevBuf<double> psi(dt);
+      int idx = 0;
+      for (int o1 = 0; o1 < ns; ++o1)
+        for (int o2 = 0; o2 <= o1; ++o2, ++idx) {
+          std::vector<const double *> src{phi[o1].p};
+          std::vector<double> cf{1.0};
+          if (o2 != o1) { src.push_back(phi[o2].p); cf.push_back(1.0); }
+          // psi = phi_o1 (+ phi_o2), norm^2
+          const int nb = combine(dt, psi.p, 0.0, (int)src.size(), src.data(), cf.data(), work_partials(), nullptr,
+                                 nullptr, stream());
+          const double nrm = sum_partials(work_partials(), nb);
+          if (!(nrm > 1e-14)) continue;
+          scale(dt, psi.p, 1.0 / std::sqrt(nrm), stream());
+          std::vector<double> a, b;
+          lanczos_green(*H, psi.p, a, b);
+          for (auto &x : b) x *= x;
+          b[0] = nrm * st.weight;
+          if (pm == 1) for (auto &x : a) x -= st.energy;
+          else for (auto &x : a) x = st.energy - x;
+          if (pm == 1) { G.ea[idx] = a; G.eb[idx] = b; }
+          else { G.ha[idx] = a; G.hb[idx] = b; }
+        }
+    } else if (G.kind == 'M') {
+      // build_mcf (src_ed/model_instance.hpp:847-946), QR block Lanczos
+      const int p = ns;
+      std::vector<double> W((size_t)p * p, 0.0);
+      int pa = p;
+      for (int l = 0; l < p; ++l) {
+        for (int k = 0; k < l; ++k) {
+          const double z = ddot(dt, phi[k].p, phi[l].p);
+          W[k + (size_t)l * p] = z;
+          std::vector<const double *> s1{phi[k].p};
+          std::vector<double> c1{-z};
+          combine(dt, phi[l].p, 1.0, 1, s1.data(), c1.data(), nullptr, nullptr, nullptr, stream());
+        }
+        const double nrm = std::sqrt(ddot(dt, phi[l].p, phi[l].p));
+        if (nrm < params().getd("accur_deflation")) { pa = l; break; }
+        W[l + (size_t)l * p] = nrm;
+        scale(dt, phi[l].p, 1.0 / nrm, stream());
+      }
+      if (pa == 0) continue;
+      for (auto &w : W) w *= std::sqrt(st.weight);
+      GFRep::MCF mc;
+      mc.p = pa;
+      mc.W.assign((size_t)pa * p, 0.0);
+      for (int k = 0; k < pa; ++k)
+        for (int l = 0; l < p; ++l) mc.W[k + (size_t)l * pa] = W[k + (size_t)l * p];
+      std::vector<const double *> q;
+      for (int k = 0; k < pa; ++k) q.push_back(phi[k].p);
+      const int M0 = (int)(14 * pa * std::log(1.0 * dt));
+      mc.nfl = block_lanczos(*H, q, M0, mc.A, mc.B);
+      const int pp = pa * pa;
+      for (int j = 0; j < mc.nfl; ++j) {
+        double *Aj = mc.A.data() + (size_t)j * pp;
+        if (pm == 1) {
+          for (int k = 0; k < pa; ++k) Aj[k + k * pa] -= st.energy;
+        } else {
+          for (int k = 0; k < pp; ++k) Aj[k] = -Aj[k];
+          for (int k = 0; k < pa; ++k) Aj[k + k * pa] += st.energy;
+        }
+      }
+      if (pm == 1) G.me = mc;
+      else G.mh = mc;
+    } else {
+      fail(ED_ERR_INVALID, std::string("GF_method must be L, F or M (got ") + G.kind + ")");
+    }
+  }
+  if (G.kind == 'L') {
+    G.e = eh;
+    G.e.insert(G.e.end(), ee.begin(), ee.end());
+    G.V = Vh;
+    G.V.insert(G.V.end(), Ve.begin(), Ve.end());
+    const double err = qnorm_error(ns, G.V);
+    if (err > params().getd("accur_Q_matrix"))
+      fail(ED_ERR_QNORM, "Q-matrix does not satisfy unitary condition by " + std::to_string(err));
+    const double sw = std::sqrt(st.weight);
+    for (auto &v : G.V) v *= sw;
+  }
+}
+
+void Instance::green_function_solve() {
+  if (gf_solved) return;
+  if (!gs_solved) ground_state_solve();
+  for (auto &s : states) {
+    build_gf(*this, *s, 0);
+    if (mixing & 4) build_gf(*this, *s, 1);
+  }
+  gf_solved = true;
+}
+
+// ---------------------------------------------------------------------------
+// Green-function evaluation on frequency grids (device kernels)
+// ---------------------------------------------------------------------------
+void Instance::green_function(int nz, const double *z, bool spin_down, double *out) {
+  if (!gf_solved) green_function_solve();
+  if (spin_down && !(mixing & 4) && mixing != 0)
+    fail(ED_ERR_INVALID, "spin_down=True impossible with Hilbert space mixing " + std::to_string(mixing));
+  const int spin = (spin_down && (mixing & 4)) ? 1 : 0;
+  const int n = gf_dim();
+  std::vector<double> acc((size_t)nz * n * n * 2, 0.0);
+  DevBuf<double> dz;
+  dz.upload(z, (size_t)2 * nz);
+  for (auto &s : states) {
+    const GFRep &G = s->gf[spin];
+    if (G.kind == 'L') {
+      const int M = (int)G.e.size();
+      if (!M) continue;
+      DevBuf<double> de, dV, dG((size_t)nz * n * n * 2);
+      de.upload(G.e);
+      dV.upload(G.V);
+      vdvh(n, M, de.p, dV.p, nz, dz.p, dG.p, stream());
+      std::vector<double> h = dG.to_host();
+      for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
+    } else if (G.kind == 'F') {
+      // pack every fraction (electron, hole per pair) into one launch
+      std::vector<int> off{0};
+      std::vector<double> A, B;
+      std::vector<int> which;  // fraction index -> (pair * 2 + eh)
+      const int np = n * (n + 1) / 2;
+      for (int q = 0; q < np; ++q)
+        for (int eh = 0; eh < 2; ++eh) {
+          const auto &a = eh ? G.ha[q] : G.ea[q];
+          const auto &b = eh ? G.hb[q] : G.eb[q];
+          A.insert(A.end(), a.begin(), a.end());
+          B.insert(B.end(), b.begin(), b.begin() + a.size());
+          off.push_back((int)A.size());
+          which.push_back(q * 2 + eh);
+        }
+      const int ncf = (int)which.size();
+      DevBuf<int> doff;
+      DevBuf<double> dA, dB, dO((size_t)nz * ncf * 2);
+      doff.upload(off);
+      dA.upload(A.empty() ? std::vector<double>{0.0} : A);
+      dB.upload(B.empty() ? std::vector<double>{0.0} : B);
+      cf_eval(ncf, doff.p, dA.p, dB.p, nz, dz.p, dO.p, stream());
+      std::vector<double> h = dO.to_host();
+      // continued_fraction_set::Green_function (src_ed/continued_fraction_set.cpp:63-98)
+      for (int iz = 0; iz < nz; ++iz) {
+        auto F = [&](int q) {
+          const double *p0 = &h[2 * ((size_t)iz * ncf + 2 * q)];
+          return cplx(p0[0] + p0[2], p0[1] + p0[3]);
+        };
+        std::vector<cplx> Gd(n);
+        int q = 0;
+        for (int o1 = 0; o1 < n; ++o1)
+          for (int o2 = 0; o2 <= o1; ++o2, ++q)
+            if (o1 == o2) Gd[o1] = F(q);
+        double *g = &acc[(size_t)iz * n * n * 2];
+        q = 0;
+        for (int o1 = 0; o1 < n; ++o1)
+          for (int o2 = 0; o2 <= o1; ++o2, ++q) {
+            cplx v = (o1 == o2) ? Gd[o1] : 0.5 * (F(q) - Gd[o1] - Gd[o2]);
+            g[2 * (o1 + (size_t)o2 * n)] += v.real();
+            g[2 * (o1 + (size_t)o2 * n) + 1] += v.imag();
+            if (o1 != o2) {
+              g[2 * (o2 + (size_t)o1 * n)] += v.real();
+              g[2 * (o2 + (size_t)o1 * n) + 1] += v.imag();
+            }
+          }
+      }
+    } else if (G.kind == 'M') {
+      for (int eh = 0; eh < 2; ++eh) {
+        const GFRep::MCF &mc = eh ? G.mh : G.me;
+        if (mc.nfl == 0) continue;
+        DevBuf<double> dA, dB, dW, dG((size_t)nz * n * n * 2);
+        dA.upload(mc.A);
+        dB.upload(mc.B);
+        dW.upload(mc.W);
+        mcf_eval(mc.p, mc.nfl, dA.p, dB.p, dW.p, n, nz, dz.p, dG.p, stream());
+        std::vector<double> h = dG.to_host();
+        for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
+      }
+    }
+  }
+  std::memcpy(out, acc.data(), acc.size() * sizeof(double));
+}
+
+std::vector<double> Instance::one_body_matrix(bool spin_down) const {
+  const int L = model->L, off = spin_down ? L : 0;
+  std::vector<double> h((size_t)L * L, 0.0);
+  for (const auto &kv : value) {
+    const Operator &op = model->ops.at(kv.first);
+    if (op.kind != OP_ONE_BODY) continue;
+    for (const auto &x : op.el) {
+      if ((x.r >= L) != spin_down || (x.c >= L) != spin_down) continue;
+      h[(size_t)(x.r - off) * L + (x.c - off)] += kv.second * x.v;
+    }
+  }
+  return h;
+}
+
+// hybridization_function (src_ed/model_instance.hpp:1004-1018): bath block
+// diagonalized, Gamma_ab = sum_i tcb(a,i) tcb(b,i) / (w - eps_i)
+void Instance::hybridization(int nz, const double *z, bool spin_down, double *out) {
+  const int L = model->L, ns = model->n_sites, nb = model->n_bath;
+  const bool sd = spin_down && (mixing & 4);
+  std::vector<double> h = one_body_matrix(sd);
+  std::memset(out, 0, sizeof(double) * 2 * (size_t)nz * ns * ns);
+  if (!nb) return;
+  std::vector<double> tb((size_t)nb * nb), eb(nb), Sb((size_t)nb * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+  symeig(nb, tb.data(), eb.data(), Sb.data());
+  std::vector<double> tcb((size_t)ns * nb, 0.0);  // tcb' = tcb Sb
+  for (int a = 0; a < ns; ++a)
+    for (int i = 0; i < nb; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nb; ++k) s += h[(size_t)a * L + ns + k] * Sb[k + (size_t)i * nb];
+      tcb[a + (size_t)i * ns] = s;
+    }
+  for (int iz = 0; iz < nz; ++iz) {
+    const cplx w(z[2 * iz], z[2 * iz + 1]);
+    double *g = out + (size_t)iz * ns * ns * 2;
+    for (int i = 0; i < nb; ++i) {
+      const cplx f = 1.0 / (w - eb[i]);
+      for (int a = 0; a < ns; ++a)
+        for (int b = 0; b < ns; ++b) {
+          const cplx v = tcb[a + (size_t)i * ns] * tcb[b + (size_t)i * ns] * f;
+          g[2 * (a + (size_t)b * ns)] += v.real();
+          g[2 * (a + (size_t)b * ns) + 1] += v.imag();
+        }
+    }
+  }
+}
+
+static void cinv_host(int n, std::vector<cplx> &a) {
+  std::vector<cplx> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[i + (size_t)i * n] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int pr = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[r + (size_t)c * n]) > std::abs(a[pr + (size_t)c * n])) pr = r;
+    if (std::abs(a[pr + (size_t)c * n]) == 0.0) fail(ED_ERR_INTERNAL, "singular Green function matrix");
+    if (pr != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[c + (size_t)k * n], a[pr + (size_t)k * n]);
+        std::swap(inv[c + (size_t)k * n], inv[pr + (size_t)k * n]);
+      }
+    const cplx d = 1.0 / a[c + (size_t)c * n];
+    for (int k = 0; k < n; ++k) { a[c + (size_t)k * n] *= d; inv[c + (size_t)k * n] *= d; }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const cplx f = a[r + (size_t)c * n];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[r + (size_t)k * n] -= f * a[c + (size_t)k * n];
+        inv[r + (size_t)k * n] -= f * inv[c + (size_t)k * n];
+      }
+    }
+  }
+  a.swap(inv);
+}
+
+// self_energy (src_ed/model_instance.hpp:683-700): Sigma = z - tc - Gamma - G^{-1}
+void Instance::self_energy(int nz, const double *z, bool spin_down, double *out) {
+  const int n = gf_dim(), L = model->L;
+  std::vector<double> G((size_t)nz * n * n * 2), Gam((size_t)nz * n * n * 2);
+  green_function(nz, z, spin_down, G.data());
+  hybridization(nz, z, spin_down, Gam.data());
+  const bool sd = spin_down && (mixing & 4);
+  std::vector<double> h = one_body_matrix(sd);
+  for (int iz = 0; iz < nz; ++iz) {
+    std::vector<cplx> g((size_t)n * n);
+    for (size_t k = 0; k < (size_t)n * n; ++k)
+      g[k] = cplx(G[2 * ((size_t)iz * n * n + k)], G[2 * ((size_t)iz * n * n + k) + 1]);
+    cinv_host(n, g);
+    const cplx w(z[2 * iz], z[2 * iz + 1]);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        const size_t k = a + (size_t)b * n;
+        cplx s = -g[k] - h[(size_t)a * L + b] - cplx(Gam[2 * ((size_t)iz * n * n + k)], Gam[2 * ((size_t)iz * n * n + k) + 1]);
+        if (a == b) s += w;
+        out[2 * ((size_t)iz * n * n + k)] = s.real();
+        out[2 * ((size_t)iz * n * n + k) + 1] = s.imag();
+      }
+  }
+}
+
+}  // namespace ed

@@ -1,0 +1,75 @@
+// ed_instance.hpp -- model instances: sectors, low-energy states, Green-function
+// representations (model_instance<T>, src_ed/model_instance.hpp).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ed_model.hpp"
+
+namespace ed {
+
+// Green-function representation of one state and one spin
+struct GFRep {
+  char kind = 0;  // 'L' Lehmann (Q-matrix), 'F' continued fractions, 'M' matrix continued fractions
+  int n = 0;      // GF dimension (cluster orbitals)
+  // Lehmann: poles e[M], residues V (n x M, column-major), weight folded in
+  std::vector<double> e, V;
+  // continued fractions per ordered pair (o1 >= o2) for electrons / holes; a, b
+  // already transformed (continued_fraction ctor, src_ed/continued_fraction.cpp:33-42)
+  std::vector<std::vector<double>> ea, eb, ha, hb;
+  // matrix continued fractions: floors A, B (p x p), weight W (p x n)
+  struct MCF { int p = 0, nfl = 0; std::vector<double> A, B, W; };
+  MCF me, mh;
+};
+
+struct State {
+  Sector sec;
+  double energy = 0.0, weight = 0.0;
+  DevBuf<double> psi;
+  GFRep gf[2];
+};
+
+struct Instance {
+  int label = 0;
+  Model *model = nullptr;
+  std::map<std::string, double> value;
+  std::set<Sector> targets;
+  int mixing = 0;  // 0 normal, 4 up_down
+  char format = 'G';
+  char gf_method = 'L';
+  bool gs_solved = false, gf_solved = false;
+  double GS_energy = 0.0, E0 = 0.0;
+  std::vector<std::unique_ptr<State>> states;
+  std::vector<std::tuple<std::string, double, double>> averages;
+
+  TermList terms() const;
+  std::pair<double, std::string> ground_state_solve();
+  void green_function_solve();
+  // nz frequencies -> G (nz blocks n x n complex column-major, interleaved)
+  void green_function(int nz, const double *z, bool spin_down, double *out);
+  void self_energy(int nz, const double *z, bool spin_down, double *out);
+  void hybridization(int nz, const double *z, bool spin_down, double *out);
+  std::string gs_string() const;
+  int gf_dim() const { return model->n_sites; }
+  // one-body matrix of the spin block (L x L, row-major) from the instance's terms
+  std::vector<double> one_body_matrix(bool spin_down) const;
+};
+
+Instance &get_instance(int label);
+void new_instance(const std::string &model, const std::map<std::string, double> &values, const std::string &sectors,
+                  int label);
+void erase_instance(int label);
+
+// c_a / c^dagger_a (pm = -1 / +1) of spin-orbital a on a factorized sector vector
+// (device); returns the target sector.  y must hold the target dimension.
+Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, double z, double *y);
+size_t sector_dim(int L, const Sector &s);
+bool sector_valid(int L, const Sector &s);
+Sector shift_sector(const Sector &s, int pm, int spin);
+
+}  // namespace ed

@@ -1,0 +1,807 @@
+// ed_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the ED path.
+//
+// All arithmetic is FP64.  H.v is HBM/cache-bandwidth bound (~3 flop/B), so no
+// MFMA is used: the kernels stream FP64 vectors with coalesced accesses, stage
+// rows of the factorized vector in LDS, and reduce dot products with wave64
+// shuffles + LDS, writing deterministic per-block partial sums.
+#include <algorithm>
+
+#include "ed_common.hpp"
+#include "ed_kernels.hpp"
+
+namespace ed {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kPartialsCap = 8192;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Block-wide sum; result valid in thread 0.  sh must hold BS/64 doubles.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x < kWave) {
+    r = (threadIdx.x < BS / kWave) ? sh[threadIdx.x] : 0.0;
+    r = wave_sum(r);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double half_entry(const DevHalf &h, const double *vtab, uint32_t e, uint32_t &col) {
+  const uint32_t w = h.ent[e];
+  if (h.packed) {
+    col = w & ((1u << h.colbits) - 1u);
+    return vtab[w >> h.colbits];
+  }
+  col = w;
+  return h.val[e];
+}
+
+__device__ __forceinline__ double diag_value(const FactArgs &a, const double *Ush, size_t I, uint32_t u,
+                                             uint32_t d, uint32_t sd) {
+  if (a.diag_mode == 0) return a.dg[I];
+  const uint32_t su = a.up.states[u];
+  double s = a.up.diag[u] + a.dn.diag[d];
+  uint32_t m = su & sd;
+  if (a.diag_mode == 2) return s + Ush[0] * (double)__popc(m);
+  while (m) {
+    const int i = __ffs(m) - 1;
+    s += Ush[i];
+    m &= m - 1u;
+  }
+  return s;
+}
+
+struct FusedParams {
+  FactArgs a;
+  const double *x;
+  double *y;
+  double ax;        // y = ax * H x + beta * y
+  double beta;
+  double *part;
+  uint32_t R;       // rows of X per block iteration
+  int ntab_up, ntab_dn;  // value-table sizes staged in LDS (packed mode)
+};
+
+constexpr int kVtabMax = 256;
+
+// Row kernel: each block owns R consecutive rows of X[i_dn][i_up]; the rows are
+// staged in LDS so the up-spin part (sparse K_up applied inside a row) gathers
+// from LDS; the down-spin part (K_dn couples whole rows) reads coalesced
+// segments of the neighbour rows through L2/MALL.  DOWN = false gives the
+// first pass of the two-pass variant (diagonal + up part only).
+template <int BS, bool ONE_ROW, bool DOWN, bool STAGE>
+__global__ __launch_bounds__(BS) void k_hv_rows(FusedParams P) {
+  extern __shared__ double smem[];
+  __shared__ double Ush[32];
+  __shared__ double vtu[kVtabMax];
+  __shared__ double vtd[kVtabMax];
+  __shared__ double red[BS / kWave];
+  const FactArgs &a = P.a;
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  const uint32_t R = ONE_ROW ? 1u : P.R;
+  if (threadIdx.x < 32) Ush[threadIdx.x] = a.Uc[threadIdx.x];
+  for (int i = threadIdx.x; i < P.ntab_up; i += BS) vtu[i] = a.up.val[i];
+  for (int i = threadIdx.x; i < P.ntab_dn; i += BS) vtd[i] = a.dn.val[i];
+  const double *vtab_u = P.ntab_up ? vtu : a.up.val;
+  const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
+  double dot = 0.0;
+  for (uint32_t d0 = blockIdx.x * R; d0 < dd; d0 += gridDim.x * R) {
+    const uint32_t nr = min(R, dd - d0);
+    const size_t base = (size_t)d0 * du;
+    const uint32_t n = nr * du;
+    if (STAGE) {
+      __syncthreads();
+      for (uint32_t f = threadIdx.x; f < n; f += BS) smem[f] = P.x[base + f];
+      __syncthreads();
+    } else {
+      __syncthreads();  // value tables staged
+    }
+    for (uint32_t f = threadIdx.x; f < n; f += BS) {
+      const uint32_t r = ONE_ROW ? 0u : f / du;
+      const uint32_t u = f - r * du;
+      const uint32_t d = d0 + r;
+      const uint32_t sd = (a.diag_mode != 0) ? a.dn.states[d] : 0u;
+      const double xv = STAGE ? smem[f] : P.x[base + f];
+      const size_t I = base + f;
+      double acc = diag_value(a, Ush, I, u, d, sd) * xv;
+      const double *xr = STAGE ? smem + (size_t)r * du : P.x + base + (size_t)r * du;
+      const uint32_t e1 = a.up.ptr[u + 1];
+      for (uint32_t e = a.up.ptr[u]; e < e1; ++e) {
+        uint32_t c;
+        const double v = half_entry(a.up, vtab_u, e, c);
+        acc += v * xr[c];
+      }
+      if (DOWN) {
+        const uint32_t g1 = a.dn.ptr[d + 1];
+        for (uint32_t e = a.dn.ptr[d]; e < g1; ++e) {
+          uint32_t c;
+          const double v = half_entry(a.dn, vtab_d, e, c);
+          acc += v * P.x[(size_t)c * du + u];
+        }
+      }
+      const double out = (P.beta != 0.0) ? P.ax * acc + P.beta * P.y[I] : P.ax * acc;
+      P.y[I] = out;
+      if (DOWN) dot += out * xv;
+    }
+  }
+  if (DOWN && P.part) {
+    const double s = block_sum<BS>(dot, red);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = s;
+  }
+}
+
+struct SlabParams {
+  FactArgs a;
+  const double *x;
+  double *y;
+  double ax;
+  double *part;
+  uint32_t nslab, nchunk, chunk_rows;
+  int nxcd;
+  int ntab_dn;
+};
+
+// Second pass of the two-pass variant: y += (K_dn x 1) x over column slabs.
+// A slab of SW columns of every row (dim_dn * SW * 8 bytes) is what the down
+// part re-reads ~nnz(K_dn)/row times; blocks are mapped so that all chunks of a
+// slab run on the same XCD (round-robin dispatch: block b -> XCD b % nxcd), which
+// keeps the slab resident in that XCD's L2.
+template <int SW, int RL>
+__global__ __launch_bounds__(SW *RL) void k_hv_down_slab(SlabParams P) {
+  __shared__ double vtd[kVtabMax];
+  __shared__ double red[(SW * RL) / kWave];
+  const FactArgs &a = P.a;
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  for (int i = threadIdx.x; i < P.ntab_dn; i += SW * RL) vtd[i] = a.dn.val[i];
+  __syncthreads();
+  const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
+  const uint32_t b = blockIdx.x;
+  const uint32_t xcd = b % P.nxcd, k = b / P.nxcd;
+  const uint32_t slab = xcd + P.nxcd * (k / P.nchunk);
+  const uint32_t chunk = k % P.nchunk;
+  double dot = 0.0;
+  if (slab < P.nslab) {
+    const uint32_t u = slab * SW + (threadIdx.x % SW);
+    const uint32_t rl = threadIdx.x / SW;
+    const uint32_t r0 = chunk * P.chunk_rows;
+    const uint32_t r1 = min(dd, r0 + P.chunk_rows);
+    if (u < du) {
+      for (uint32_t d = r0 + rl; d < r1; d += RL) {
+        double acc = 0.0;
+        const uint32_t g1 = a.dn.ptr[d + 1];
+        for (uint32_t e = a.dn.ptr[d]; e < g1; ++e) {
+          uint32_t c;
+          const double v = half_entry(a.dn, vtab_d, e, c);
+          acc += v * P.x[(size_t)c * du + u];
+        }
+        const size_t I = (size_t)d * du + u;
+        const double out = P.y[I] + P.ax * acc;
+        P.y[I] = out;
+        dot += out * P.x[I];
+      }
+    }
+  }
+  if (P.part) {
+    const double s = block_sum<SW * RL>(dot, red);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = s;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fact_diag(FactArgs a, double *d) {
+  __shared__ double Ush[32];
+  if (threadIdx.x < 32) Ush[threadIdx.x] = a.Uc[threadIdx.x];
+  __syncthreads();
+  const uint32_t du = a.up.dim;
+  const size_t n = (size_t)du * a.dn.dim;
+  for (size_t I = (size_t)blockIdx.x * BS + threadIdx.x; I < n; I += (size_t)gridDim.x * BS) {
+    const uint32_t dn = (uint32_t)(I / du), u = (uint32_t)(I - (size_t)dn * du);
+    d[I] = diag_value(a, Ush, I, u, dn, a.diag_mode ? a.dn.states[dn] : 0u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV (full symmetric matrix), G lanes per row.
+// ---------------------------------------------------------------------------
+template <int BS, int G>
+__global__ __launch_bounds__(BS) void k_hv_csr(CsrArgs A, const double *x, double *y, double ax, double beta,
+                                               double *part) {
+  __shared__ double red[BS / kWave];
+  const uint64_t rows_per_block = BS / G;
+  double dot = 0.0;
+  const int g = threadIdx.x % G;
+  for (uint64_t row = (uint64_t)blockIdx.x * rows_per_block + threadIdx.x / G; row < A.n;
+       row += (uint64_t)gridDim.x * rows_per_block) {
+    double acc = 0.0;
+    const int64_t e1 = A.ptr[row + 1];
+    for (int64_t e = A.ptr[row] + g; e < e1; e += G) acc += A.val[e] * x[A.col[e]];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (g == 0) {
+      acc += A.diag[row] * x[row];
+      const double out = (beta != 0.0) ? ax * acc + beta * y[row] : ax * acc;
+      y[row] = out;
+      dot += out * x[row];
+    }
+  }
+  if (part) {
+    const double s = block_sum<BS>(dot, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// On-the-fly H.x (pull form): matrix elements regenerated from the bit
+// patterns; target index by combinatorial ranking (binomials in LDS).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rank_half(uint32_t s, const uint32_t *binom, int L) {
+  uint32_t r = 0;
+  int k = 1;
+  while (s) {
+    const int p = __ffs(s) - 1;
+    r += binom[p * (L + 1) + k];
+    ++k;
+    s &= s - 1u;
+  }
+  return r;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_hv_otf(OtfArgs A, const double *x, double *y, double ax, double beta,
+                                               double *part) {
+  __shared__ uint32_t bn[33 * 33];
+  __shared__ double red[BS / kWave];
+  const int L = A.L;
+  for (int i = threadIdx.x; i < (L + 1) * (L + 1); i += BS) bn[i] = A.binom[i];
+  __syncthreads();
+  const size_t n = (size_t)A.du * A.dd;
+  double dot = 0.0;
+  for (size_t I = (size_t)blockIdx.x * BS + threadIdx.x; I < n; I += (size_t)gridDim.x * BS) {
+    const uint32_t id = (uint32_t)(I / A.du), iu = (uint32_t)(I - (size_t)id * A.du);
+    const uint32_t su = A.up_states[iu], sd = A.dn_states[id];
+    double acc = A.dg[I] * x[I];
+    for (int h = 0; h < A.nhop; ++h) {
+      const OtfHop hp = A.hops[h];
+      const bool down = hp.r >= L;
+      const uint32_t s = down ? sd : su;
+      const int r = down ? hp.r - L : hp.r, c = down ? hp.c - L : hp.c;
+      if (!((s >> r) & 1u) || ((s >> c) & 1u)) continue;
+      const uint32_t s2 = s ^ (1u << r) ^ (1u << c);
+      const int lo = min(r, c), hi = max(r, c);
+      const uint32_t inside = (hi - lo > 1) ? (((1u << hi) - 1u) & ~((2u << lo) - 1u)) : 0u;
+      const double sg = (__popc(s & inside) & 1) ? -1.0 : 1.0;
+      const uint32_t rk = rank_half(s2, bn, L);
+      const size_t J = down ? (size_t)rk * A.du + iu : (size_t)id * A.du + rk;
+      acc += hp.v * sg * x[J];
+    }
+    const double out = (beta != 0.0) ? ax * acc + beta * y[I] : ax * acc;
+    y[I] = out;
+    dot += out * x[I];
+  }
+  if (part) {
+    const double s = block_sum<BS>(dot, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+constexpr int kMaxVec = 16;
+struct PtrPack { const double *p[kMaxVec]; };
+struct MPtrPack { double *p[kMaxVec]; };
+struct CoefPack { double c[kMaxVec]; };
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, int m, PtrPack b, double *part) {
+  __shared__ double red[BS / kWave];
+  double acc[kMaxVec];
+#pragma unroll
+  for (int k = 0; k < kMaxVec; ++k) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    const double av = a[i];
+#pragma unroll
+    for (int k = 0; k < kMaxVec; ++k)
+      if (k < m) acc[k] += av * b.p[k][i];
+  }
+  for (int k = 0; k < m; ++k) {
+    const double s = block_sum<BS>(acc[k], red);
+    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, const double *x, int m, MPtrPack y, CoefPack c,
+                                                       double *part) {
+  __shared__ double red[BS / kWave];
+  double nn = 0.0;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    const double xv = x[i];
+#pragma unroll
+    for (int k = 0; k < kMaxVec; ++k)
+      if (k < m) {
+        const double v = y.p[k][i] + c.c[k] * xv;
+        y.p[k][i] = v;
+        if (k == 0) nn += v * v;
+      }
+  }
+  if (part) {
+    const double s = block_sum<BS>(nn, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, int m, PtrPack b, CoefPack c,
+                                                double *npart, const double *w, double *dpart) {
+  __shared__ double red[BS / kWave];
+  double nn = 0.0, dd = 0.0;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    double v = (st == 0.0) ? 0.0 : st * t[i];
+#pragma unroll
+    for (int k = 0; k < kMaxVec; ++k)
+      if (k < m) v += c.c[k] * b.p[k][i];
+    t[i] = v;
+    nn += v * v;
+    if (w) dd += v * w[i];
+  }
+  if (npart) {
+    const double s = block_sum<BS>(nn, red);
+    if (threadIdx.x == 0) npart[blockIdx.x] = s;
+  }
+  if (dpart) {
+    const double s = block_sum<BS>(dd, red);
+    if (threadIdx.x == 0) dpart[blockIdx.x] = s;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_scale(size_t n, double *x, double a) {
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] *= a;
+}
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fill(size_t n, double *x, double v) {
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] = v;
+}
+__global__ void k_set1(double *x, size_t i) { x[i] = 1.0; }
+
+// Davidson correction (src_ed/Hamiltonian/Davidson.hpp:181-188): r_i /= (mu - d_i + tiny)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_precond(size_t n, double *r, const double *d, double mu, double tiny) {
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS)
+    r[i] /= (-d[i] + mu + tiny);
+}
+
+// Lanczos eigenvector replay step: p = p + c1 r ; g = g + c2 p (new p)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_replay(size_t n, double *p, const double *r, double *g, double c1, double c2) {
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    const double v = p[i] + c1 * r[i];
+    p[i] = v;
+    g[i] += c2 * v;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cdag(CdagArgs A, const double *x, double *y) {
+  const size_t n = (size_t)A.du_t * A.dd_t;
+  for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < n; t += (size_t)gridDim.x * BS) {
+    const uint32_t id = (uint32_t)(t / A.du_t), iu = (uint32_t)(t - (size_t)id * A.du_t);
+    if (A.spin == 0) {
+      const int32_t s = A.map[iu];
+      if (s < 0) continue;
+      y[t] += A.z * (double)A.sgn[iu] * x[(size_t)s + (size_t)A.du_s * id];
+    } else {
+      const int32_t s = A.map[id];
+      if (s < 0) continue;
+      y[t] += A.z * (double)A.sgn[id] * (double)A.upar[iu] * x[(size_t)iu + (size_t)A.du_s * s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Green-function kernels (complex FP64, interleaved)
+// ---------------------------------------------------------------------------
+struct cplx { double re, im; };
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm for a / b
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, den = b.re + b.im * r;
+    return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  }
+  const double r = b.re / b.im, den = b.re * r + b.im;
+  return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+}
+
+__global__ void k_cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z,
+                          double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncf * nz) return;
+  const int f = t % ncf, iz = t / ncf;
+  const cplx zz{z[2 * iz], z[2 * iz + 1]};
+  cplx G{0.0, 0.0};
+  for (int i = off[f + 1] - 1; i >= off[f]; --i) {
+    const cplx den{zz.re - a[i] - G.re, zz.im - G.im};
+    G = cdiv(cplx{b[i], 0.0}, den);
+  }
+  out[2 * ((size_t)iz * ncf + f)] = G.re;
+  out[2 * ((size_t)iz * ncf + f) + 1] = G.im;
+}
+
+__global__ void k_vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int LL = L * L;
+  if (t >= nz * LL) return;
+  const int iz = t / LL, ab = t % LL;
+  const int bb = ab % L, aa = ab / L;  // G[b + a L]
+  const double zr = z[2 * iz], zi = z[2 * iz + 1];
+  double sr = 0.0, si = 0.0;
+  for (int i = 0; i < M; ++i) {
+    const double tr = zr - e[i], ti = zi;
+    const double inv = 1.0 / (tr * tr + ti * ti);
+    const double w = V[aa + (size_t)i * L] * V[bb + (size_t)i * L];
+    sr += w * tr * inv;
+    si -= w * ti * inv;
+  }
+  G[2 * ((size_t)iz * LL + ab)] = sr;
+  G[2 * ((size_t)iz * LL + ab) + 1] = si;
+}
+
+constexpr int kMcfMax = 8;
+__device__ bool cinv_small(int n, cplx *a, cplx *inv) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) inv[i + j * n] = {i == j ? 1.0 : 0.0, 0.0};
+  for (int c = 0; c < n; ++c) {
+    int pr = c;
+    double best = hypot(a[c + c * n].re, a[c + c * n].im);
+    for (int r = c + 1; r < n; ++r) {
+      const double m = hypot(a[r + c * n].re, a[r + c * n].im);
+      if (m > best) { best = m; pr = r; }
+    }
+    if (best == 0.0) return false;
+    if (pr != c)
+      for (int k = 0; k < n; ++k) {
+        cplx t = a[c + k * n]; a[c + k * n] = a[pr + k * n]; a[pr + k * n] = t;
+        t = inv[c + k * n]; inv[c + k * n] = inv[pr + k * n]; inv[pr + k * n] = t;
+      }
+    const cplx dinv = cdiv(cplx{1.0, 0.0}, a[c + c * n]);
+    for (int k = 0; k < n; ++k) {
+      cplx v = a[c + k * n];
+      a[c + k * n] = {v.re * dinv.re - v.im * dinv.im, v.re * dinv.im + v.im * dinv.re};
+      v = inv[c + k * n];
+      inv[c + k * n] = {v.re * dinv.re - v.im * dinv.im, v.re * dinv.im + v.im * dinv.re};
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const cplx f = a[r + c * n];
+      if (f.re == 0.0 && f.im == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        const cplx u = a[c + k * n], w = inv[c + k * n];
+        a[r + k * n].re -= f.re * u.re - f.im * u.im;
+        a[r + k * n].im -= f.re * u.im + f.im * u.re;
+        inv[r + k * n].re -= f.re * w.re - f.im * w.im;
+        inv[r + k * n].im -= f.re * w.im + f.im * w.re;
+      }
+    }
+  }
+  return true;
+}
+
+__global__ void k_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz,
+                           const double *z, double *G) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iz >= nz) return;
+  cplx F[kMcfMax * kMcfMax], D[kMcfMax * kMcfMax], T[kMcfMax * kMcfMax];
+  const int pp = p * p;
+  for (int k = 0; k < pp; ++k) F[k] = {0.0, 0.0};
+  const double zr = z[2 * iz], zi = z[2 * iz + 1];
+  for (int j = nfl - 1; j >= 0; --j) {
+    const double *Aj = A + (size_t)j * pp, *Bj = B + (size_t)j * pp;
+    for (int k = 0; k < pp; ++k) D[k] = {-Aj[k], 0.0};
+    for (int k = 0; k < p; ++k) { D[k + k * p].re += zr; D[k + k * p].im += zi; }
+    if (j < nfl - 1) {
+      for (int c = 0; c < p; ++c)
+        for (int r = 0; r < p; ++r) {
+          cplx s{0.0, 0.0};
+          for (int k = 0; k < p; ++k) { s.re += F[r + k * p].re * Bj[k + c * p]; s.im += F[r + k * p].im * Bj[k + c * p]; }
+          T[r + c * p] = s;
+        }
+      for (int c = 0; c < p; ++c)
+        for (int r = 0; r < p; ++r) {
+          cplx s{0.0, 0.0};
+          for (int k = 0; k < p; ++k) { s.re += Bj[k + r * p] * T[k + c * p].re; s.im += Bj[k + r * p] * T[k + c * p].im; }
+          D[r + c * p].re -= s.re;
+          D[r + c * p].im -= s.im;
+        }
+    }
+    if (!cinv_small(p, D, F)) {
+      for (int k = 0; k < q * q; ++k) { G[2 * ((size_t)iz * q * q + k)] = NAN; G[2 * ((size_t)iz * q * q + k) + 1] = NAN; }
+      return;
+    }
+  }
+  for (int c = 0; c < q; ++c)
+    for (int r = 0; r < q; ++r) {
+      double sr = 0.0, si = 0.0;
+      for (int k = 0; k < p; ++k)
+        for (int l = 0; l < p; ++l) {
+          const double w = W[k + r * p] * W[l + c * p];
+          sr += w * F[k + l * p].re;
+          si += w * F[k + l * p].im;
+        }
+      G[2 * ((size_t)iz * q * q + r + c * q)] = sr;
+      G[2 * ((size_t)iz * q * q + r + c * q) + 1] = si;
+    }
+}
+
+int g_ncu = 0;
+int num_cu() {
+  if (!g_ncu) {
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    ED_HIP(hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (g_ncu <= 0) g_ncu = 256;
+  }
+  return g_ncu;
+}
+
+int grid_for(size_t n, int bs, int per_cu = 8) {
+  size_t g = (n + bs - 1) / bs;
+  size_t cap = (size_t)num_cu() * per_cu;
+  if (g > cap) g = cap;
+  if (g > (size_t)kPartialsCap) g = kPartialsCap;
+  return g < 1 ? 1 : (int)g;
+}
+
+size_t g_lds_max = 0;
+// largest dynamic LDS a block may request (160 KB per CU on gfx950, minus the
+// static tables of the row kernel)
+size_t lds_row_budget() {
+  if (!g_lds_max) {
+    int dev = 0, v = 0;
+    ED_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+      (void)hipGetLastError();
+      v = 64 * 1024;
+    }
+    g_lds_max = (size_t)v;
+  }
+  const size_t reserve = 8 * 1024;
+  return g_lds_max > reserve ? g_lds_max - reserve : 0;
+}
+
+uint32_t rows_per_block(uint32_t du) {
+  if ((size_t)du * 8 > lds_row_budget()) return 0;  // row does not fit: no LDS staging
+  uint32_t R = 1;
+  while (R * du < 512 && (size_t)(2 * R) * du * 8 <= 32 * 1024) R *= 2;
+  return R;
+}
+
+template <typename K>
+void allow_lds(K kern, size_t bytes) {
+  if (bytes > 64 * 1024)
+    ED_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+}  // namespace
+
+int partials_cap() { return kPartialsCap; }
+
+static FusedParams make_fused(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part,
+                              uint32_t R) {
+  FusedParams P;
+  P.a = a;
+  P.x = x;
+  P.y = y;
+  P.ax = ax;
+  P.beta = beta;
+  P.part = part;
+  P.R = R;
+  // value tables up to kVtabMax entries are staged in LDS, larger ones are read from global
+  P.ntab_up = (a.up.packed && a.up.nval <= (uint32_t)kVtabMax) ? (int)a.up.nval : 0;
+  P.ntab_dn = (a.dn.packed && a.dn.nval <= (uint32_t)kVtabMax) ? (int)a.dn.nval : 0;
+  return P;
+}
+
+static int fused_grid(const FactArgs &a, uint32_t R) {
+  const uint32_t nb = (a.dn.dim + R - 1) / R;
+  return (int)std::min<uint32_t>(nb, (uint32_t)kPartialsCap);
+}
+
+int hv_factorized_max_partials(const FactArgs &a, int variant) { return kPartialsCap; }
+
+int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
+                  hipStream_t s) {
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  if ((size_t)du * dd == 0) return 0;
+  uint32_t R = rows_per_block(du);
+  const bool stage = R > 0;
+  if (!stage) R = 1;
+  if (variant == HV_AUTO) variant = ((size_t)du * dd >= ((size_t)1 << 26)) ? HV_TWO_PASS : HV_FUSED;
+  const size_t lds = stage ? (size_t)R * du * sizeof(double) : 0;
+  constexpr int BS = 1024;
+  const bool down_here = variant == HV_FUSED;
+  {
+    FusedParams P = make_fused(a, x, y, ax, beta, down_here ? part : nullptr, R);
+    const int g = fused_grid(a, R);
+#define ED_LAUNCH_ROWS(BSZ, ONE, DOWN, STG)                                   \
+  do {                                                                          \
+    auto kern = k_hv_rows<BSZ, ONE, DOWN, STG>;                                 \
+    if (STG) allow_lds(kern, lds);                                              \
+    hipLaunchKernelGGL(kern, dim3(g), dim3(BSZ), STG ? lds : 0, s, P);          \
+  } while (0)
+    if (down_here) {
+      if (!stage) ED_LAUNCH_ROWS(BS, true, true, false);
+      else if (R == 1) ED_LAUNCH_ROWS(BS, true, true, true);
+      else ED_LAUNCH_ROWS(256, false, true, true);
+    } else {
+      if (!stage) ED_LAUNCH_ROWS(BS, true, false, false);
+      else if (R == 1) ED_LAUNCH_ROWS(BS, true, false, true);
+      else ED_LAUNCH_ROWS(256, false, false, true);
+    }
+#undef ED_LAUNCH_ROWS
+    ED_HIP(hipGetLastError());
+    if (down_here) return g;
+  }
+  // second pass of the two-pass variant: column slabs (down part)
+  constexpr int SW = 32, RL = 8;
+  SlabParams Q;
+  Q.a = a;
+  Q.x = x;
+  Q.y = y;
+  Q.ax = ax;
+  Q.nxcd = 8;
+  Q.nslab = (du + SW - 1) / SW;
+  uint32_t ch = 64;
+  auto nblocks = [&](uint32_t c) {
+    const uint32_t nch = (dd + c - 1) / c;
+    const uint32_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
+    return (size_t)Q.nxcd * slabs_per_xcd * nch;
+  };
+  while (nblocks(ch) > (size_t)kPartialsCap && ch < dd) ch *= 2;
+  if (nblocks(ch) > (size_t)kPartialsCap)
+    fail(ED_ERR_UNSUPPORTED, "two-pass H.v: too many column slabs for the partial-sum buffer");
+  Q.chunk_rows = ch;
+  Q.nchunk = (dd + ch - 1) / ch;
+  Q.part = part;
+  Q.ntab_dn = (a.dn.packed && a.dn.nval <= (uint32_t)kVtabMax) ? (int)a.dn.nval : 0;
+  const int g = (int)nblocks(ch);
+  hipLaunchKernelGGL((k_hv_down_slab<SW, RL>), dim3(g), dim3(SW * RL), 0, s, Q);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+void fact_full_diag(const FactArgs &a, double *d, hipStream_t s) {
+  const size_t n = (size_t)a.up.dim * a.dn.dim;
+  if (!n) return;
+  hipLaunchKernelGGL((k_fact_diag<256>), dim3(grid_for(n, 256, 16)), dim3(256), 0, s, a, d);
+  ED_HIP(hipGetLastError());
+}
+
+int hv_csr_max_partials(const CsrArgs &) { return kPartialsCap; }
+int hv_csr(const CsrArgs &A, const double *x, double *y, double ax, double beta, double *part, hipStream_t s) {
+  if (!A.n) return 0;
+  constexpr int BS = 256, G = 8;
+  const int g = grid_for(A.n * G, BS, 16);
+  hipLaunchKernelGGL((k_hv_csr<BS, G>), dim3(g), dim3(BS), 0, s, A, x, y, ax, beta, part);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+int hv_otf_max_partials(const OtfArgs &) { return kPartialsCap; }
+int hv_otf(const OtfArgs &A, const double *x, double *y, double ax, double beta, double *part, hipStream_t s) {
+  const size_t n = (size_t)A.du * A.dd;
+  if (!n) return 0;
+  constexpr int BS = 256;
+  const int g = grid_for(n, BS, 16);
+  hipLaunchKernelGGL((k_hv_otf<BS>), dim3(g), dim3(BS), 0, s, A, x, y, ax, beta, part);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+int multi_dot(size_t n, const double *a, int m, const double *const *b, double *part, hipStream_t s) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_dot: too many vectors");
+  PtrPack pk{};
+  for (int k = 0; k < m; ++k) pk.p[k] = b[k];
+  constexpr int BS = 256;
+  const int g = grid_for(n, BS, 4);
+  hipLaunchKernelGGL((k_multi_dot<BS>), dim3(g), dim3(BS), 0, s, n, a, m, pk, part);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+int multi_axpy_src(size_t n, const double *x, int m, double *const *y, const double *c, double *part,
+                   hipStream_t s) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_axpy: too many vectors");
+  MPtrPack pk{};
+  CoefPack cp{};
+  for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
+  constexpr int BS = 256;
+  const int g = grid_for(n, BS, 4);
+  hipLaunchKernelGGL((k_multi_axpy_src<BS>), dim3(g), dim3(BS), 0, s, n, x, m, pk, cp, part);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *npart,
+            const double *w, double *dpart, hipStream_t s) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "combine: too many vectors");
+  PtrPack pk{};
+  CoefPack cp{};
+  for (int k = 0; k < m; ++k) { pk.p[k] = b[k]; cp.c[k] = c[k]; }
+  constexpr int BS = 256;
+  const int g = grid_for(n, BS, 4);
+  hipLaunchKernelGGL((k_combine<BS>), dim3(g), dim3(BS), 0, s, n, t, st, m, pk, cp, npart, w, dpart);
+  ED_HIP(hipGetLastError());
+  return g;
+}
+
+void scale(size_t n, double *x, double a, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_scale<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, x, a);
+  ED_HIP(hipGetLastError());
+}
+void fill(size_t n, double *x, double v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_fill<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, x, v);
+  ED_HIP(hipGetLastError());
+}
+void replay_step(size_t n, double *p, const double *r, double *g, double c1, double c2, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_replay<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, p, r, g, c1, c2);
+  ED_HIP(hipGetLastError());
+}
+
+void precond(size_t n, double *r, const double *d, double mu, double tiny, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_precond<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, r, d, mu, tiny);
+  ED_HIP(hipGetLastError());
+}
+
+void set_unit(double *x, size_t i, hipStream_t s) {
+  hipLaunchKernelGGL(k_set1, dim3(1), dim3(1), 0, s, x, i);
+  ED_HIP(hipGetLastError());
+}
+
+void cdag_gather(const CdagArgs &A, const double *x, double *y, hipStream_t s) {
+  const size_t n = (size_t)A.du_t * A.dd_t;
+  if (!n) return;
+  hipLaunchKernelGGL((k_cdag<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, A, x, y);
+  ED_HIP(hipGetLastError());
+}
+
+void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out,
+             hipStream_t s) {
+  const int n = ncf * nz;
+  if (!n) return;
+  hipLaunchKernelGGL(k_cf_eval, dim3((n + 255) / 256), dim3(256), 0, s, ncf, off, a, b, nz, z, out);
+  ED_HIP(hipGetLastError());
+}
+
+void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s) {
+  const int n = nz * L * L;
+  if (!n) return;
+  hipLaunchKernelGGL(k_vdvh, dim3((n + 255) / 256), dim3(256), 0, s, L, M, e, V, nz, z, G);
+  ED_HIP(hipGetLastError());
+}
+
+void mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz, const double *z,
+              double *G, hipStream_t s) {
+  if (p > kMcfMax) fail(ED_ERR_UNSUPPORTED, "matrix continued fraction with block size > 8 on device");
+  if (!nz) return;
+  hipLaunchKernelGGL(k_mcf_eval, dim3((nz + 63) / 64), dim3(64), 0, s, p, nfl, A, B, W, q, nz, z, G);
+  ED_HIP(hipGetLastError());
+}
+
+}  // namespace ed

@@ -1,0 +1,111 @@
+// ed_kernels.hpp -- host-side launchers of the HIP kernels (ed_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ed {
+
+// One spin's merged, symmetric one-body operator on a half basis (both
+// triangles of HS_half_operator, src_ed/Operators/HS_half_operator.hpp:17-70,
+// summed over terms with their coefficients).  Entries are packed as
+// col | (value_id << colbits) when the value table is small enough,
+// otherwise val[] holds one double per entry.
+struct DevHalf {
+  uint32_t dim = 0;
+  const uint32_t *ptr = nullptr;   // dim + 1
+  const uint32_t *ent = nullptr;   // packed entries
+  const double *val = nullptr;     // explicit values (packed == 0) or value table (packed == 1)
+  const double *diag = nullptr;    // one-body + same-spin interaction diagonal, per half state
+  const uint32_t *states = nullptr;  // half-basis bit patterns
+  int packed = 1;
+  int colbits = 0;
+  uint32_t nval = 0;               // size of the value table (packed == 1)
+};
+
+// Factorized H on X[i_dn][i_up]: H = D + (1 x K_up) + (K_dn x 1).
+struct FactArgs {
+  DevHalf up, dn;
+  int diag_mode = 0;      // 0 streamed full diagonal, 1 per-site cross U_i, 2 uniform cross U
+  const double *dg = nullptr;  // full diagonal (diag_mode 0)
+  double Uc[32];          // cross (up-down same-site) coefficients (diag_mode 1, 2)
+};
+
+enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
+
+// y = ax H x + beta y (y not read when beta == 0); when dot_partials != nullptr,
+// per-block partial sums of <y_new, x> are written there (nblocks returned).
+int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
+                  int variant, hipStream_t s);
+int hv_factorized_max_partials(const FactArgs &a, int variant);
+void fact_full_diag(const FactArgs &a, double *d, hipStream_t s);
+
+// full symmetric CSR (row pointers 64-bit safe)
+struct CsrArgs {
+  uint64_t n = 0;
+  const int64_t *ptr = nullptr;
+  const uint32_t *col = nullptr;
+  const double *val = nullptr;
+  const double *diag = nullptr;
+};
+int hv_csr(const CsrArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
+           hipStream_t s);
+int hv_csr_max_partials(const CsrArgs &a);
+
+// on-the-fly (no stored matrix): one-body hop list + interaction diagonal
+struct OtfHop { int r, c; double v; };   // y[I] += v * sign * x[J], J = I with electron moved r -> c
+struct OtfArgs {
+  int L = 0;
+  uint32_t du = 0, dd = 0;
+  const uint32_t *up_states = nullptr, *dn_states = nullptr;
+  int nhop = 0;
+  const OtfHop *hops = nullptr;    // device array
+  const double *dg = nullptr;      // full diagonal
+  const uint32_t *binom = nullptr; // (L+1) x (L+1) binomials for ranking
+};
+int hv_otf(const OtfArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
+           hipStream_t s);
+int hv_otf_max_partials(const OtfArgs &a);
+
+// ---- vector kernels (FP64, wave64 reductions, per-block partials) ----
+int partials_cap();
+// out[k * nb + block] = partial of <a, b_k>, k < m (m <= 16). returns nb
+int multi_dot(size_t n, const double *a, int m, const double *const *b, double *partials, hipStream_t s);
+// y_k += c_k x for k < m; if norm_partials, partials of ||y_0||^2 after the update. returns nb
+int multi_axpy_src(size_t n, const double *x, int m, double *const *y, const double *c, double *norm_partials,
+                   hipStream_t s);
+// t = st * t + sum_k c_k b_k ; optional partials of ||t||^2 and <t, w>. returns nb
+int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *norm_partials,
+            const double *w, double *dot_partials, hipStream_t s);
+void scale(size_t n, double *x, double a, hipStream_t s);
+void fill(size_t n, double *x, double v, hipStream_t s);
+void set_unit(double *x, size_t i, hipStream_t s);
+// r_i /= (mu - d_i + tiny)
+void precond(size_t n, double *r, const double *d, double mu, double tiny, hipStream_t s);
+// p += c1 r ; g += c2 p
+void replay_step(size_t n, double *p, const double *r, double *g, double c1, double c2, hipStream_t s);
+
+// c_a / c^dagger_a gather between factorized sectors: y[t] += z * sgn[t_half] * x[src(t)]
+struct CdagArgs {
+  int spin;                 // 0: up orbital changes, 1: down orbital
+  uint32_t du_t, dd_t;      // target half dims
+  uint32_t du_s;            // source up dim
+  const int32_t *map;       // target half index -> source half index or -1 (changed spin)
+  const int8_t *sgn;        // +-1 per target half index of the changed spin (same-spin bits below a)
+  const int8_t *upar;       // +-1 per up index (parity of up electrons) when spin == 1, else null
+  double z;
+};
+void cdag_gather(const CdagArgs &a, const double *x, double *y, hipStream_t s);
+
+// ---- Green-function kernels ----
+// continued fractions: out[iz * ncf + f] = G_f(z_iz)
+void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out,
+             hipStream_t s);
+// Lehmann sum per z: G[iz][b + a L] = sum_i V[a,i] V[b,i] / (z - e_i)
+void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s);
+// matrix continued fraction per z (p <= 16)
+void mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz, const double *z,
+              double *G, hipStream_t s);
+
+}  // namespace ed

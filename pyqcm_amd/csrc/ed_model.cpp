@@ -1,0 +1,800 @@
+// ed_model.cpp -- sectors, bases, model registry, Hamiltonian plans, host dense helpers.
+#include "ed_model.hpp"
+#include "ed_dense.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <unordered_map>
+
+namespace ed {
+
+// ---------------------------------------------------------------------------
+// Sector (src_ed/sector.hpp:54-106, ordering :150-163)
+// ---------------------------------------------------------------------------
+Sector Sector::parse(const std::string &str) {
+  Sector s;
+  bool valid = true;
+  s.N = even;
+  s.S = even;
+  s.irrep = 0;
+  auto p = str.find('N');
+  if (p != std::string::npos && sscanf(str.c_str() + p, "N%d", &s.N) != 1) valid = false;
+  p = str.find('S');
+  if (p != std::string::npos && sscanf(str.c_str() + p, "S%d", &s.S) != 1) valid = false;
+  p = str.find('R');
+  if (p != std::string::npos && sscanf(str.c_str() + p, "R%d", &s.irrep) != 1) valid = false;
+  if (!valid) fail(ED_ERR_SECTOR, "sector string " + str + " does not conform to standard!");
+  if (str.find('O') != std::string::npos) s.N = s.S = odd;
+  if (s.S == even && s.N != even && (s.N % 2)) s.S = odd;
+  if (s.N == even && s.S != even && (s.S % 2)) s.N = odd;
+  return s;
+}
+
+std::string Sector::name() const {
+  std::string r;
+  if (S == odd && N == odd) r += "O";
+  r += "R" + std::to_string(irrep);
+  if (N != even && N != odd) r += ":N" + std::to_string(N);
+  if (S != even && S != odd) r += ":S" + std::to_string(S);
+  return r;
+}
+
+int Sector::Nup() const {
+  if (N == even || S == even) return even;
+  if (N == odd || S == odd) return odd;
+  return (N + S) / 2;
+}
+int Sector::Ndw() const {
+  if (N == even || S == even) return even;
+  if (N == odd || S == odd) return odd;
+  return (N - S) / 2;
+}
+bool Sector::operator<(const Sector &y) const {
+  if (S != y.S) return S < y.S;
+  if (N != y.N) return N < y.N;
+  return irrep < y.irrep;
+}
+
+// ---------------------------------------------------------------------------
+// half basis
+// ---------------------------------------------------------------------------
+uint32_t HalfBasis::index(uint32_t b) const {
+  auto it = std::lower_bound(bin.begin(), bin.end(), b);
+  uint32_t I = (uint32_t)(it - bin.begin());
+  if (I < dim && bin[I] == b) return I;
+  return dim;
+}
+
+std::shared_ptr<const HalfBasis> half_basis(int L, int N) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, std::shared_ptr<const HalfBasis>> cache;
+  std::lock_guard<std::mutex> l(mu);
+  auto key = std::make_pair(L, N);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  auto B = std::make_shared<HalfBasis>();
+  B->L = L;
+  B->N = N;
+  if (L >= 0 && L <= 32 && N >= 0 && N <= L) {
+    // ascending integers with popcount N: Gosper's hack enumerates them in order
+    if (N == 0) {
+      B->bin.push_back(0u);
+    } else {
+      uint64_t v = (N == 64) ? ~0ull : ((1ull << N) - 1ull);
+      const uint64_t lim = 1ull << L;
+      while (v < lim) {
+        B->bin.push_back((uint32_t)v);
+        const uint64_t t = v | (v - 1);
+        v = (t + 1) | (((~t & -~t) - 1) >> (__builtin_ctzll(v) + 1));
+      }
+    }
+  }
+  B->dim = (uint32_t)B->bin.size();
+  cache[key] = B;
+  return B;
+}
+
+// ---------------------------------------------------------------------------
+// models and operators
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_model_mu;
+std::map<std::string, std::unique_ptr<Model>> g_models;
+}  // namespace
+
+Model &get_model(const std::string &name) {
+  std::lock_guard<std::mutex> l(g_model_mu);
+  auto it = g_models.find(name);
+  if (it == g_models.end()) fail(ED_ERR_NOT_FOUND, "The model " + name + " is not defined. Check spelling.");
+  return *it->second;
+}
+
+// ED::new_model (src_ed/qcm_ED.cpp:43-59)
+void new_model(const std::string &name, int n_sites, int n_bath) {
+  std::lock_guard<std::mutex> l(g_model_mu);
+  if (g_models.count(name)) return;  // reference: warning, keeps the existing model
+  if (n_bath < 0 || n_bath > 32) fail(ED_ERR_INVALID, "number of bath orbitals exceeds limits or negative!");
+  if (n_sites < 1 || n_sites > 1024) fail(ED_ERR_INVALID, "number of cluster sites exceeds limits or negative!");
+  if (n_sites + n_bath > 32) fail(ED_ERR_INVALID, "more than 32 orbitals per spin (ED_basis.hpp:23)");
+  auto M = std::make_unique<Model>();
+  M->name = name;
+  M->n_sites = n_sites;
+  M->n_bath = n_bath;
+  M->L = n_sites + n_bath;
+  g_models[name] = std::move(M);
+}
+
+// ED::new_operator (src_ed/qcm_ED.cpp:74-90) with the element handling of the
+// one_body_operator (one_body_operator.hpp:59-76) and interaction_operator
+// (interaction_operator.hpp:37-57) constructors.
+void new_operator(const std::string &model, const std::string &name, const std::string &type,
+                  const std::vector<Element> &el) {
+  if (el.empty()) return;
+  Model &M = get_model(model);
+  if (M.closed) return;  // reference: warning, model closed for modifications
+  const int L = M.L;
+  Operator op;
+  op.name = name;
+  for (const auto &x : el)
+    if (x.r < 0 || x.c < 0 || x.r >= 2 * L || x.c >= 2 * L)
+      fail(ED_ERR_INVALID, "element (" + std::to_string(x.r) + "," + std::to_string(x.c) + ") of operator " + name +
+                               " is out of range for " + std::to_string(2 * L) + " spin-orbitals");
+  if (type == "one-body") {
+    op.kind = OP_ONE_BODY;
+    for (const auto &x : el) {
+      if (x.r < x.c) {
+        op.el.push_back({x.r, x.c, x.v});
+        op.el.push_back({x.c, x.r, x.v});
+      } else if (x.r == x.c) {
+        op.el.push_back({x.r, x.r, x.v});
+      }
+    }
+    for (const auto &x : op.el)
+      if ((x.r < L) != (x.c < L)) op.spin_flip = true;
+  } else if (type == "interaction") {
+    op.kind = OP_INTERACTION;
+    for (const auto &x : el) op.el.push_back({std::min(x.r, x.c), std::max(x.r, x.c), x.v});
+  } else {
+    fail(ED_ERR_UNSUPPORTED, "type of operator " + name + " (" + type + ") is not supported by this solver");
+  }
+  // spin symmetry (one_body_operator.hpp:187-205): up and down parts equal?
+  {
+    std::map<std::pair<int, int>, double> m;
+    for (const auto &x : op.el) m[{x.r, x.c}] += x.v;
+    for (const auto &kv : m) {
+      int r = kv.first.first, c = kv.first.second;
+      r = (r < L) ? r + L : r - L;
+      c = (c < L) ? c + L : c - L;
+      auto f = m.find({std::min(r, c), std::max(r, c)});
+      auto g = m.find({r, c});
+      const bool ok = (op.kind == OP_INTERACTION) ? (f != m.end() && f->second == kv.second)
+                                                   : (g != m.end() && g->second == kv.second);
+      if (!ok) { op.spin_asym = true; break; }
+    }
+  }
+  std::lock_guard<std::mutex> l(g_model_mu);
+  if (!M.ops.count(name)) M.order.push_back(name);
+  M.ops[name] = std::move(op);
+}
+
+// ---------------------------------------------------------------------------
+// helpers on bit states (src_ed/binary_state.cpp:23-28, .hpp:42-47)
+// ---------------------------------------------------------------------------
+static inline int hop_sign(uint32_t s, int from, int to) {
+  // s must contain 'from' and not 'to'; sign of moving the electron
+  const int lo = std::min(from, to), hi = std::max(from, to);
+  if (hi - lo <= 1) return 1;
+  const uint32_t inside = (uint32_t)(((1ull << hi) - 1ull) & ~((2ull << lo) - 1ull));
+  return (__builtin_popcount(s & inside) & 1) ? -1 : 1;
+}
+
+double sum_partials(const double *dpart, int nb, int m, std::vector<double> *out) {
+  static thread_local std::vector<double> h;
+  h.resize((size_t)nb * m);
+  if (nb * m) {
+    ED_HIP(hipMemcpyAsync(h.data(), dpart, sizeof(double) * nb * m, hipMemcpyDeviceToHost, stream()));
+    ED_HIP(hipStreamSynchronize(stream()));
+  }
+  if (out) out->assign(m, 0.0);
+  double first = 0.0;
+  for (int k = 0; k < m; ++k) {
+    // pairwise summation in a fixed order: deterministic and accurate
+    std::vector<double> w(h.begin() + (size_t)k * nb, h.begin() + (size_t)(k + 1) * nb);
+    size_t n = w.size();
+    while (n > 1) {
+      size_t half = (n + 1) / 2;
+      for (size_t i = 0; i + half < n; ++i) w[i] += w[i + half];
+      n = half;
+    }
+    const double s = n ? w[0] : 0.0;
+    if (out) (*out)[k] = s;
+    if (k == 0) first = s;
+  }
+  return first;
+}
+
+double Hamiltonian::dot_result(int nb) { return sum_partials(partials.p, nb); }
+
+// ---------------------------------------------------------------------------
+// per-spin merged symmetric one-body table + diagonal (host build)
+// ---------------------------------------------------------------------------
+struct HalfHost {
+  uint32_t dim = 0;
+  std::vector<uint32_t> ptr, ent;
+  std::vector<double> val;
+  std::vector<double> diag;
+  int packed = 1, colbits = 1;
+  uint32_t nval = 0;
+};
+
+static HalfHost build_half(const TermList &T, int spin, const HalfBasis &B) {
+  const int L = T.L, off = spin ? L : 0;
+  HalfHost H;
+  H.dim = B.dim;
+  H.diag.assign(B.dim, 0.0);
+  std::vector<std::pair<uint64_t, double>> ent;  // key (row << 32 | col)
+  for (const auto &tc : T.terms) {
+    const Operator &op = *tc.first;
+    const double coef = tc.second;
+    if (op.kind == OP_ONE_BODY) {
+      for (const auto &x : op.el) {
+        if ((x.r >= L) != (spin == 1) || (x.c >= L) != (spin == 1)) continue;
+        const int r = x.r - off, c = x.c - off;
+        if (r == c) {
+          const uint32_t m = 1u << r;
+          for (uint32_t I = 0; I < B.dim; ++I)
+            if (B.bin[I] & m) H.diag[I] += coef * x.v;
+          continue;
+        }
+        // element v c+_r c_c: from state I (c occupied, r empty) to J = I with c -> r
+        const uint32_t mc = 1u << c, mr = 1u << r;
+        for (uint32_t I = 0; I < B.dim; ++I) {
+          const uint32_t s = B.bin[I];
+          if (!(s & mc) || (s & mr)) continue;
+          const uint32_t J = B.index(s ^ mc ^ mr);
+          if (J == B.dim) continue;
+          const double X = coef * x.v * hop_sign(s, c, r);
+          ent.push_back({((uint64_t)J << 32) | I, X});
+        }
+      }
+    } else {
+      for (const auto &x : op.el) {
+        if ((x.r >= L) != (spin == 1) || (x.c >= L) != (spin == 1)) continue;
+        const uint32_t m = (1u << (x.r - off)) | (1u << (x.c - off));
+        for (uint32_t I = 0; I < B.dim; ++I)
+          if ((B.bin[I] & m) == m) H.diag[I] += coef * x.v;
+      }
+    }
+  }
+  std::sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  std::vector<std::pair<uint64_t, double>> merged;
+  for (const auto &e : ent) {
+    if (!merged.empty() && merged.back().first == e.first) merged.back().second += e.second;
+    else merged.push_back(e);
+  }
+  H.ptr.assign((size_t)B.dim + 1, 0);
+  std::vector<uint32_t> cols;
+  std::vector<double> vals;
+  for (const auto &e : merged) {
+    if (e.second == 0.0) continue;
+    H.ptr[(e.first >> 32) + 1]++;
+    cols.push_back((uint32_t)e.first);
+    vals.push_back(e.second);
+  }
+  for (uint32_t r = 0; r < B.dim; ++r) H.ptr[r + 1] += H.ptr[r];
+  // value table
+  H.colbits = 1;
+  while (H.colbits < 32 && (1ull << H.colbits) < (uint64_t)std::max<uint32_t>(B.dim, 2)) ++H.colbits;
+  std::vector<double> table;
+  std::unordered_map<uint64_t, uint32_t> ids;
+  bool ok = H.colbits < 32;
+  const uint64_t cap = ok ? (1ull << (32 - H.colbits)) : 0;
+  std::vector<uint32_t> vid(vals.size());
+  for (size_t k = 0; ok && k < vals.size(); ++k) {
+    uint64_t bits;
+    std::memcpy(&bits, &vals[k], 8);
+    auto it = ids.find(bits);
+    if (it == ids.end()) {
+      if (table.size() >= cap) { ok = false; break; }
+      it = ids.emplace(bits, (uint32_t)table.size()).first;
+      table.push_back(vals[k]);
+    }
+    vid[k] = it->second;
+  }
+  if (ok) {
+    H.packed = 1;
+    H.val = table;
+    H.nval = (uint32_t)table.size();
+    H.ent.resize(cols.size());
+    for (size_t k = 0; k < cols.size(); ++k) H.ent[k] = cols[k] | (vid[k] << H.colbits);
+  } else {
+    H.packed = 0;
+    H.ent = cols;
+    H.val = vals;
+    H.nval = 0;
+  }
+  return H;
+}
+
+struct DevHalfStore {
+  DevBuf<uint32_t> ptr, ent, states;
+  DevBuf<double> val, diag;
+  DevHalf view;
+  HalfHost host;
+  void upload(const HalfHost &H, const HalfBasis &B) {
+    host = H;
+    ptr.upload(H.ptr);
+    ent.upload(H.ent.empty() ? std::vector<uint32_t>{0u} : H.ent);
+    val.upload(H.val.empty() ? std::vector<double>{0.0} : H.val);
+    diag.upload(H.diag.empty() ? std::vector<double>{0.0} : H.diag);
+    states.upload(B.bin.empty() ? std::vector<uint32_t>{0u} : B.bin);
+    view.dim = H.dim;
+    view.ptr = ptr.p;
+    view.ent = ent.p;
+    view.val = val.p;
+    view.diag = diag.p;
+    view.states = states.p;
+    view.packed = H.packed;
+    view.colbits = H.colbits;
+    view.nval = H.nval;
+  }
+};
+
+// cross-spin density-density coefficients W[i][j] (i up orbital, j down orbital)
+static std::vector<double> cross_terms(const TermList &T) {
+  const int L = T.L;
+  std::vector<double> W((size_t)L * L, 0.0);
+  for (const auto &tc : T.terms) {
+    if (tc.first->kind != OP_INTERACTION) continue;
+    for (const auto &x : tc.first->el) {
+      const bool ru = x.r < L, cu = x.c < L;
+      if (ru == cu) continue;
+      const int i = ru ? x.r : x.c, j = ru ? x.c - L : x.r - L;
+      W[(size_t)i * L + j] += tc.second * x.v;
+    }
+  }
+  return W;
+}
+
+// Host full diagonal for general cross terms (diag_mode 0)
+static std::vector<double> host_full_diag(const HalfHost &U, const HalfHost &D, const HalfBasis &BU,
+                                          const HalfBasis &BD, const std::vector<double> &W, int L) {
+  std::vector<double> dg((size_t)U.dim * D.dim);
+#pragma omp parallel for schedule(static)
+  for (long long d = 0; d < (long long)D.dim; ++d) {
+    std::vector<double> wd(L, 0.0);
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j)
+        if ((BD.bin[d] >> j) & 1u) wd[i] += W[(size_t)i * L + j];
+    for (uint32_t u = 0; u < U.dim; ++u) {
+      double s = U.diag[u] + D.diag[d];
+      uint32_t m = BU.bin[u];
+      while (m) {
+        const int i = __builtin_ctz(m);
+        s += wd[i];
+        m &= m - 1;
+      }
+      dg[(size_t)d * U.dim + u] = s;
+    }
+  }
+  return dg;
+}
+
+// ---------------------------------------------------------------------------
+// factorized plan ('F' / 'G')
+// ---------------------------------------------------------------------------
+struct HamFactorized : Hamiltonian {
+  DevHalfStore up, dn;
+  DevBuf<double> dg;
+  FactArgs args;
+  std::shared_ptr<const HalfBasis> BU, BD;
+
+  HamFactorized(const TermList &T, const Sector &s) {
+    sec = s;
+    L = T.L;
+    Nup = s.Nup();
+    Ndw = s.Ndw();
+    BU = half_basis(L, Nup);
+    BD = half_basis(L, Ndw);
+    dim = (size_t)BU->dim * BD->dim;
+    format = 'F';
+    HalfHost hu = build_half(T, 0, *BU), hd = build_half(T, 1, *BD);
+    up.upload(hu, *BU);
+    dn.upload(hd, *BD);
+    args.up = up.view;
+    args.dn = dn.view;
+    std::memset(args.Uc, 0, sizeof(args.Uc));
+    std::vector<double> W = cross_terms(T);
+    bool diagonal = true;
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j)
+        if (i != j && W[(size_t)i * L + j] != 0.0) diagonal = false;
+    if (diagonal) {
+      bool uniform = true;
+      for (int i = 0; i < L; ++i) {
+        args.Uc[i] = W[(size_t)i * L + i];
+        if (args.Uc[i] != args.Uc[0]) uniform = false;
+      }
+      args.diag_mode = uniform ? 2 : 1;
+    } else {
+      args.diag_mode = 0;
+      dg.upload(host_full_diag(hu, hd, *BU, *BD, W, L));
+      args.dg = dg.p;
+    }
+    partials.alloc(partials_cap());
+    variant = (int)params().geti("hv_variant");
+  }
+  int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
+    return hv_factorized(args, x, y, ax, beta, want_dot ? partials.p : nullptr, variant, stream());
+  }
+  void diag(double *d) override { fact_full_diag(args, d, stream()); }
+};
+
+// ---------------------------------------------------------------------------
+// CSR plan ('S' / 'E'): bit-exact strict-lower pattern of Hamiltonian_CSR,
+// stored on device as the full symmetric matrix for a row-parallel SpMV.
+// ---------------------------------------------------------------------------
+void csr_lower(const TermList &T, int Nup, int Ndw, std::vector<int32_t> &Iptr, std::vector<int32_t> &J,
+               std::vector<double> &v, std::vector<double> &diag) {
+  const int L = T.L;
+  auto BU = half_basis(L, Nup), BD = half_basis(L, Ndw);
+  const size_t du = BU->dim, dd = BD->dim, dim = du * dd;
+  if (dim >= 0x7fffffffull) fail(ED_ERR_UNSUPPORTED, "CSR format limited to dim < 2^31 (int32 indices)");
+  diag.assign(dim, 0.0);
+  std::vector<std::pair<uint64_t, double>> E;
+  for (const auto &tc : T.terms) {
+    const Operator &op = *tc.first;
+    const double z = tc.second;
+    for (size_t I = 0; I < dim; ++I) {
+      const uint32_t iu = (uint32_t)(I % du), id = (uint32_t)(I / du);
+      const uint32_t su = BU->bin[iu], sd = BD->bin[id];
+      if (op.kind == OP_INTERACTION) {
+        for (const auto &x : op.el) {
+          const bool r_ok = (x.r < L) ? ((su >> x.r) & 1u) : ((sd >> (x.r - L)) & 1u);
+          const bool c_ok = (x.c < L) ? ((su >> x.c) & 1u) : ((sd >> (x.c - L)) & 1u);
+          if (r_ok && c_ok) diag[I] += z * x.v;
+        }
+        continue;
+      }
+      for (const auto &x : op.el) {
+        if (x.r == x.c) {
+          const bool occ = (x.r < L) ? ((su >> x.r) & 1u) : ((sd >> (x.r - L)) & 1u);
+          if (occ && std::fabs(x.v) >= 1e-12) diag[I] += z * x.v;
+          continue;
+        }
+        if ((x.r < L) != (x.c < L)) continue;  // spin flip leaves the (Nup, Ndw) sector
+        const bool upspin = x.r < L;
+        const int r = upspin ? x.r : x.r - L, c = upspin ? x.c : x.c - L;
+        const uint32_t s = upspin ? su : sd;
+        if (!((s >> c) & 1u) || ((s >> r) & 1u)) continue;
+        const uint32_t s2 = s ^ (1u << c) ^ (1u << r);
+        size_t Jo;
+        if (upspin) {
+          const uint32_t k = BU->index(s2);
+          if (k == BU->dim) continue;
+          Jo = (size_t)k + du * id;
+        } else {
+          const uint32_t k = BD->index(s2);
+          if (k == BD->dim) continue;
+          Jo = (size_t)iu + du * k;
+        }
+        const double X = x.v * hop_sign(s, c, r);
+        if (std::fabs(X) < 1e-12) continue;
+        if (!(I < Jo)) continue;  // HS_nondiagonal_operator::insert(J_out, I_in): keep I_in < J_out
+        E.push_back({((uint64_t)Jo << 32) | I, X * z});
+      }
+    }
+  }
+  std::sort(E.begin(), E.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  Iptr.assign(dim + 1, 0);
+  J.clear();
+  v.clear();
+  for (size_t k = 0; k < E.size(); ++k) {
+    if (!J.empty() && k > 0 && E[k - 1].first == E[k].first) {
+      v.back() += E[k].second;
+      continue;
+    }
+    Iptr[(E[k].first >> 32) + 1]++;
+    J.push_back((int32_t)(E[k].first & 0xffffffffu));
+    v.push_back(E[k].second);
+  }
+  for (size_t r = 0; r < dim; ++r) Iptr[r + 1] += Iptr[r];
+}
+
+struct HamCSR : Hamiltonian {
+  DevBuf<int64_t> ptr;
+  DevBuf<uint32_t> col;
+  DevBuf<double> val, dgv;
+  CsrArgs args;
+  std::vector<int32_t> Iptr, J;
+  std::vector<double> v, dgh;
+  HamCSR(const TermList &T, const Sector &s, char fmt) {
+    sec = s;
+    L = T.L;
+    Nup = s.Nup();
+    Ndw = s.Ndw();
+    format = fmt;
+    csr_lower(T, Nup, Ndw, Iptr, J, v, dgh);
+    dim = dgh.size();
+    // full symmetric rows
+    std::vector<int64_t> fp(dim + 1, 0);
+    for (size_t i = 0; i < dim; ++i)
+      for (int32_t e = Iptr[i]; e < Iptr[i + 1]; ++e) { fp[i + 1]++; fp[(size_t)J[e] + 1]++; }
+    for (size_t i = 0; i < dim; ++i) fp[i + 1] += fp[i];
+    std::vector<uint32_t> fc(fp[dim]);
+    std::vector<double> fv(fp[dim]);
+    std::vector<int64_t> pos(fp.begin(), fp.end() - 1);
+    for (size_t i = 0; i < dim; ++i)
+      for (int32_t e = Iptr[i]; e < Iptr[i + 1]; ++e) {
+        fc[pos[i]] = (uint32_t)J[e];
+        fv[pos[i]++] = v[e];
+        fc[pos[J[e]]] = (uint32_t)i;
+        fv[pos[J[e]]++] = v[e];
+      }
+    ptr.upload(fp);
+    col.upload(fc.empty() ? std::vector<uint32_t>{0u} : fc);
+    val.upload(fv.empty() ? std::vector<double>{0.0} : fv);
+    dgv.upload(dgh.empty() ? std::vector<double>{0.0} : dgh);
+    args.n = dim;
+    args.ptr = ptr.p;
+    args.col = col.p;
+    args.val = val.p;
+    args.diag = dgv.p;
+    partials.alloc(partials_cap());
+  }
+  int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
+    return hv_csr(args, x, y, ax, beta, want_dot ? partials.p : nullptr, stream());
+  }
+  void diag(double *d) override {
+    if (dim) ED_HIP(hipMemcpyAsync(d, dgv.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// on-the-fly plan ('N')
+// ---------------------------------------------------------------------------
+struct HamOTF : Hamiltonian {
+  DevHalfStore up, dn;   // used for the diagonal and the state tables
+  DevBuf<double> dg;
+  DevBuf<OtfHop> hops;
+  DevBuf<uint32_t> binom;
+  OtfArgs args;
+  HamOTF(const TermList &T, const Sector &s) {
+    sec = s;
+    L = T.L;
+    Nup = s.Nup();
+    Ndw = s.Ndw();
+    format = 'N';
+    auto BU = half_basis(L, Nup), BD = half_basis(L, Ndw);
+    dim = (size_t)BU->dim * BD->dim;
+    HalfHost hu = build_half(T, 0, *BU), hd = build_half(T, 1, *BD);
+    up.upload(hu, *BU);
+    dn.upload(hd, *BD);
+    std::vector<double> W = cross_terms(T);
+    dg.upload(host_full_diag(hu, hd, *BU, *BD, W, L));
+    std::map<std::pair<int, int>, double> hm;
+    for (const auto &tc : T.terms) {
+      if (tc.first->kind != OP_ONE_BODY) continue;
+      for (const auto &x : tc.first->el) {
+        if (x.r == x.c) continue;
+        if ((x.r < L) != (x.c < L)) continue;
+        // element v c+_r c_c: pull row I from J = I with r -> c
+        hm[{x.r, x.c}] += tc.second * x.v;
+      }
+    }
+    std::vector<OtfHop> hv;
+    for (const auto &kv : hm)
+      if (kv.second != 0.0) hv.push_back({kv.first.first, kv.first.second, kv.second});
+    hops.upload(hv.empty() ? std::vector<OtfHop>{{0, 0, 0.0}} : hv);
+    std::vector<uint32_t> bn((size_t)(L + 1) * (L + 1), 0);
+    for (int n = 0; n <= L; ++n) {
+      bn[(size_t)n * (L + 1)] = 1;
+      for (int k = 1; k <= n; ++k)
+        bn[(size_t)n * (L + 1) + k] = bn[(size_t)(n - 1) * (L + 1) + k - 1] + (k <= n - 1 ? bn[(size_t)(n - 1) * (L + 1) + k] : 0);
+    }
+    binom.upload(bn);
+    args.L = L;
+    args.du = BU->dim;
+    args.dd = BD->dim;
+    args.up_states = up.states.p;
+    args.dn_states = dn.states.p;
+    args.nhop = (int)hv.size();
+    args.hops = hops.p;
+    args.dg = dg.p;
+    args.binom = binom.p;
+    partials.alloc(partials_cap());
+  }
+  int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
+    return hv_otf(args, x, y, ax, beta, want_dot ? partials.p : nullptr, stream());
+  }
+  void diag(double *d) override {
+    if (dim) ED_HIP(hipMemcpyAsync(d, dg.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  }
+};
+
+static bool factorizable(const TermList &T) {
+  for (const auto &tc : T.terms)
+    if (tc.first->kind == OP_ONE_BODY && tc.first->spin_flip && tc.second != 0.0) return false;
+  return true;
+}
+
+std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format) {
+  if (sec.Nup() >= Sector::odd || sec.Ndw() >= Sector::odd)
+    fail(ED_ERR_UNSUPPORTED, "sector " + sec.name() + ": only sectors with conserved N and S are supported");
+  if (sec.Nup() < 0 || sec.Ndw() < 0 || sec.Nup() > T.L || sec.Ndw() > T.L)
+    fail(ED_ERR_SECTOR, "sector " + sec.name() + " is empty for " + std::to_string(T.L) + " orbitals");
+  for (const auto &tc : T.terms)
+    if (tc.first->spin_flip && tc.second != 0.0)
+      fail(ED_ERR_UNSUPPORTED, "operator " + tc.first->name + " flips spin: spin-nonconserving sectors are not supported");
+  std::unique_ptr<Hamiltonian> H;
+  if (format == 'G' || format == 'F') {
+    if (factorizable(T)) H.reset(new HamFactorized(T, sec));
+    else H.reset(new HamCSR(T, sec, 'S'));
+  } else if (format == 'S' || format == 'E' || format == 'O') {
+    H.reset(new HamCSR(T, sec, format));
+  } else if (format == 'N') {
+    H.reset(new HamOTF(T, sec));
+  } else {
+    fail(ED_ERR_INVALID, std::string("Hamiltonian_format is not one of : G, N, S, O, F, E (got '") + format + "')");
+  }
+  H->sec = sec;
+  H->terms = T;
+  return H;
+}
+
+// ---------------------------------------------------------------------------
+// dense host helpers
+// ---------------------------------------------------------------------------
+// implicit QL on a symmetric tridiagonal matrix; rotations accumulated into the
+// rows of Z given by zrows (n_z rows of length n, row-major) -- the full matrix
+// (zrows = n, identity start) or only the last row (for the Ritz residual).
+void ql_rows(int n, double *d, double *e, double *Z, int nz) {
+  if (n <= 0) return;
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (m != l) {
+        if (iter++ == 60) fail(ED_ERR_NOT_CONVERGED, "tridiagonal eigensolver did not converge");
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i], b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) { d[i + 1] -= p; e[m] = 0.0; break; }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - b;
+          for (int k = 0; k < nz; ++k) {
+            double *zr = Z + (size_t)k * n;
+            const double fz = zr[i + 1];
+            zr[i + 1] = s * zr[i] + c * fz;
+            zr[i] = c * zr[i] - s * fz;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+}
+
+void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0) {
+  std::vector<double> dd(d, d + n), ee(n, 0.0), z(n, 0.0);
+  for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
+  z[n - 1] = 1.0;  // last row of the identity
+  ql_rows(n, dd.data(), ee.data(), z.data(), 1);
+  int k = 0;
+  for (int i = 1; i < n; ++i)
+    if (dd[i] < dd[k]) k = i;
+  ev0 = dd[k];
+  zlast0 = z[k];
+}
+
+void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0) {
+  std::vector<double> dd(d, d + n), ee(n, 0.0), Z((size_t)n * n, 0.0);
+  for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
+  for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+  ql_rows(n, dd.data(), ee.data(), Z.data(), n);
+  int k = 0;
+  for (int i = 1; i < n; ++i)
+    if (dd[i] < dd[k]) k = i;
+  ev0 = dd[k];
+  z0.resize(n);
+  for (int i = 0; i < n; ++i) z0[i] = Z[(size_t)i * n + k];
+}
+
+// Householder tridiagonalisation (row-major working copy) + implicit QL
+void symeig(int n, const double *Ain, double *w, double *V) {
+  if (n <= 0) return;
+  std::vector<double> a((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[(size_t)i * n + j] = Ain[i + (size_t)j * n];
+  std::vector<double> d(n), e(n);
+  const bool vec = V != nullptr;
+#define A_(i, j) a[(size_t)(i) * n + (j)]
+  for (int i = n - 1; i > 0; --i) {
+    const int l = i - 1;
+    double h = 0.0, scale = 0.0;
+    if (l > 0) {
+      for (int k = 0; k <= l; ++k) scale += std::fabs(A_(i, k));
+      if (scale == 0.0) {
+        e[i] = A_(i, l);
+      } else {
+        for (int k = 0; k <= l; ++k) { A_(i, k) /= scale; h += A_(i, k) * A_(i, k); }
+        double f = A_(i, l);
+        double g = (f >= 0.0) ? -std::sqrt(h) : std::sqrt(h);
+        e[i] = scale * g;
+        h -= f * g;
+        A_(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; ++j) {
+          if (vec) A_(j, i) = A_(i, j) / h;
+          g = 0.0;
+          for (int k = 0; k <= j; ++k) g += A_(j, k) * A_(i, k);
+          for (int k = j + 1; k <= l; ++k) g += A_(k, j) * A_(i, k);
+          e[j] = g / h;
+          f += e[j] * A_(i, j);
+        }
+        const double hh = f / (h + h);
+        for (int j = 0; j <= l; ++j) {
+          f = A_(i, j);
+          e[j] = g = e[j] - hh * f;
+          for (int k = 0; k <= j; ++k) A_(j, k) -= (f * e[k] + g * A_(i, k));
+        }
+      }
+    } else {
+      e[i] = A_(i, l);
+    }
+    d[i] = h;
+  }
+  d[0] = 0.0;
+  e[0] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (vec) {
+      const int l = i - 1;
+      if (d[i] != 0.0) {
+        for (int j = 0; j <= l; ++j) {
+          double g = 0.0;
+          for (int k = 0; k <= l; ++k) g += A_(i, k) * A_(k, j);
+          for (int k = 0; k <= l; ++k) A_(k, j) -= g * A_(k, i);
+        }
+      }
+      d[i] = A_(i, i);
+      A_(i, i) = 1.0;
+      for (int j = 0; j <= l; ++j) A_(j, i) = A_(i, j) = 0.0;
+    } else {
+      d[i] = A_(i, i);
+    }
+  }
+#undef A_
+  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+  e[n - 1] = 0.0;
+  ql_rows(n, d.data(), e.data(), vec ? a.data() : nullptr, vec ? n : 0);
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  for (int k = 0; k < n; ++k) w[k] = d[idx[k]];
+  if (vec)
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < n; ++i) V[i + (size_t)k * n] = a[(size_t)i * n + idx[k]];
+}
+
+}  // namespace ed

@@ -1,0 +1,124 @@
+// ed_model.hpp -- sectors, bases, models, operators, Hamiltonian plans (host side).
+#pragma once
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "ed_common.hpp"
+#include "ed_kernels.hpp"
+
+namespace ed {
+
+using cplx = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// sector (src_ed/sector.hpp:23-147)
+// ---------------------------------------------------------------------------
+struct Sector {
+  static constexpr int odd = 9999;
+  static constexpr int even = 10000;
+  int N = 0, S = 0, irrep = 0;
+  Sector() = default;
+  Sector(int n, int s, int r) : N(n), S(s), irrep(r) {}
+  static Sector parse(const std::string &str);
+  std::string name() const;
+  int Nup() const;
+  int Ndw() const;
+  bool operator<(const Sector &o) const;  // sector.hpp:150-163 ordering
+  bool operator==(const Sector &o) const { return N == o.N && S == o.S && irrep == o.irrep; }
+};
+
+// ---------------------------------------------------------------------------
+// half basis (src_ed/ED_basis.cpp:115-135), cached per (L, N)
+// ---------------------------------------------------------------------------
+struct HalfBasis {
+  int L = 0, N = 0;
+  std::vector<uint32_t> bin;
+  uint32_t dim = 0;
+  uint32_t index(uint32_t b) const;  // lower_bound; returns dim if absent
+};
+std::shared_ptr<const HalfBasis> half_basis(int L, int N);
+
+// ---------------------------------------------------------------------------
+// model and operators (src_ed/model.hpp, src_ed/Operators/*)
+// ---------------------------------------------------------------------------
+struct Element {
+  int r, c;
+  double v;
+};
+
+enum OpKind { OP_ONE_BODY = 0, OP_INTERACTION = 1 };
+
+struct Operator {
+  std::string name;
+  int kind = OP_ONE_BODY;
+  std::vector<Element> el;   // Hermitian-completed (one-body) / ordered (interaction)
+  bool spin_flip = false;    // one-body elements coupling up and down
+  bool spin_asym = false;    // up and down parts differ (mixing up_down)
+};
+
+struct Model {
+  std::string name;
+  int n_sites = 0, n_bath = 0, L = 0;
+  std::vector<std::string> order;            // definition order
+  std::map<std::string, Operator> ops;
+  bool closed = false;
+};
+
+Model &get_model(const std::string &name);
+void new_model(const std::string &name, int n_sites, int n_bath);
+void new_operator(const std::string &model, const std::string &name, const std::string &type,
+                  const std::vector<Element> &el);
+
+// (operator, coefficient) pairs of an instance
+struct TermList {
+  int L = 0;
+  std::vector<std::pair<const Operator *, double>> terms;
+};
+
+// ---------------------------------------------------------------------------
+// Hamiltonian plans: Hamiltonian<T>::mult_add replacement on device
+// (src_ed/Hamiltonian/Hamiltonian_base.hpp:43-82)
+// ---------------------------------------------------------------------------
+struct Hamiltonian {
+  Sector sec;
+  int L = 0;
+  int Nup = 0, Ndw = 0;
+  size_t dim = 0;
+  char format = 'G';
+  int variant = 0;
+  TermList terms;           // the instance terms this plan realizes
+  DevBuf<double> partials;  // per-block dot partials
+  virtual ~Hamiltonian() = default;
+  // y = ax H x + beta y on device; returns the number of per-block partials of
+  // <y_new, x> written to `partials` when want_dot
+  virtual int apply(const double *x, double *y, double ax, double beta, bool want_dot) = 0;
+  // diagonal of H on device
+  virtual void diag(double *d) = 0;
+  double dot_result(int nb);  // sum of the partials (deterministic order)
+};
+
+std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format);
+
+// host strict-lower CSR pattern (Hamiltonian_CSR.hpp:60-81) for parity checks
+void csr_lower(const TermList &T, int Nup, int Ndw, std::vector<int32_t> &Iptr, std::vector<int32_t> &J,
+               std::vector<double> &v, std::vector<double> &diag);
+
+// ---------------------------------------------------------------------------
+// dense host helpers
+// ---------------------------------------------------------------------------
+// symmetric tridiagonal: lowest eigenvalue and the last components of all
+// eigenvectors (for the Ritz residual); full eigenvector of the lowest state.
+void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0);
+void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0);
+// dense symmetric eigensystem, A column-major n x n, eigenvalues ascending
+void symeig(int n, const double *A, double *w, double *V /* may be null */);
+
+// sum of per-block partials read back from the device
+double sum_partials(const double *dpart, int nb, int m = 1, std::vector<double> *out = nullptr);
+
+}  // namespace ed

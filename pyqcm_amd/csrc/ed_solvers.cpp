@@ -1,0 +1,625 @@
+// ed_solvers.cpp -- Krylov drivers: host control flow, device vectors.
+//
+// Every vector operation runs on the library stream; the host only sees the
+// scalars of the recurrences (alpha, beta, Gram coefficients), obtained from
+// deterministic per-block partial sums.  All recurrences are FP64.
+#include "ed_solvers.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ed_dense.hpp"
+
+namespace ed {
+
+namespace {
+
+struct Work {
+  DevBuf<double> part;
+  Work() { part.alloc((size_t)partials_cap() * 16); }
+};
+Work &work() {
+  static thread_local Work w;
+  return w;
+}
+
+void d2d(double *dst, const double *src, size_t n) {
+  if (n) ED_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+}
+
+// dots of a with each b (chunks of 16 vectors per kernel)
+std::vector<double> dots(size_t n, const double *a, const std::vector<const double *> &b) {
+  std::vector<double> out(b.size(), 0.0);
+  for (size_t s = 0; s < b.size(); s += 16) {
+    const int m = (int)std::min<size_t>(16, b.size() - s);
+    const int nb = multi_dot(n, a, m, b.data() + s, work().part.p, stream());
+    std::vector<double> r;
+    sum_partials(work().part.p, nb, m, &r);
+    for (int k = 0; k < m; ++k) out[s + k] = r[k];
+  }
+  return out;
+}
+
+// t = st t + sum c_k b_k (chunks); returns ||t||^2 if want_norm
+double comb(size_t n, double *t, double st, const std::vector<const double *> &b, const std::vector<double> &c,
+            bool want_norm, const double *w = nullptr, double *dot_w = nullptr) {
+  if (b.empty()) {
+    if (st != 1.0) scale(n, t, st, stream());
+    if (!want_norm && !dot_w) return 0.0;
+    const int nb = combine(n, t, 1.0, 0, nullptr, nullptr, work().part.p, w, w ? work().part.p + partials_cap() : nullptr,
+                           stream());
+    const double nn = sum_partials(work().part.p, nb);
+    if (dot_w) *dot_w = sum_partials(work().part.p + partials_cap(), nb);
+    return nn;
+  }
+  double nn = 0.0;
+  for (size_t s = 0; s < b.size(); s += 16) {
+    const int m = (int)std::min<size_t>(16, b.size() - s);
+    const bool last = s + 16 >= b.size();
+    const bool wn = last && want_norm, wd = last && dot_w;
+    const int nb = combine(n, t, s == 0 ? st : 1.0, m, b.data() + s, c.data() + s, wn ? work().part.p : nullptr,
+                           wd ? w : nullptr, wd ? work().part.p + partials_cap() : nullptr, stream());
+    if (wn) nn = sum_partials(work().part.p, nb);
+    if (wd) *dot_w = sum_partials(work().part.p + partials_cap(), nb);
+  }
+  return nn;
+}
+
+}  // namespace
+
+double *work_partials() { return work().part.p; }
+
+double ddot(size_t n, const double *a, const double *b) {
+  const double *bb[1] = {b};
+  const int nb = multi_dot(n, a, 1, bb, work().part.p, stream());
+  return sum_partials(work().part.p, nb);
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos ground state.  Vectors are kept unnormalized: buffer A holds
+// beta_{k-1} v_k (norm bA), buffer B holds the previous one; the H.v kernel
+// folds the scalings (y = ax H x + b y) so no separate scaling pass runs.
+// With store_lanczos_basis the Krylov vectors stay resident in HBM and the
+// eigenvector is one combination pass; otherwise (or when the HBM budget is
+// exceeded) the recurrence is replayed as in the reference.
+// ---------------------------------------------------------------------------
+LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
+  Params &P = params();
+  const int max_iter = (int)P.geti("max_iter_lanczos");
+  const double accur = P.getd("accur_lanczos");
+  const size_t n = H.dim;
+  const double budget = P.getd("lanczos_basis_budget_GB") * 1e9;
+  bool store = P.getb("store_lanczos_basis");
+  LanczosGS R;
+  std::vector<DevBuf<double>> basis;  // basis[k] = s_k v_k
+  std::vector<double> sc;             // s_k
+  DevBuf<double> A(n), B(n);
+  d2d(A.p, d_start, n);
+  double bA = 1.0, bB = 1.0;
+  if (store) {
+    basis.emplace_back(n);
+    d2d(basis[0].p, d_start, n);
+    sc.push_back(1.0);
+  }
+  double ritz = 2.0 * accur;
+  while (ritz > accur) {
+    const int k = (int)R.alpha.size();
+    double alpha, beta;
+    if (store) {
+      // w = H v_k - beta_{k-1} v_{k-1} - alpha v_k into a fresh buffer
+      if ((double)(basis.size() + 1) * n * 8.0 > budget) {
+        // HBM budget reached: continue with two buffers and replay at the end
+        store = false;
+        d2d(A.p, basis[k].p, n);
+        bA = sc[k];
+        if (k > 0) { d2d(B.p, basis[k - 1].p, n); bB = sc[k - 1]; }
+        basis.clear();
+        sc.clear();
+      }
+    }
+    if (store) {
+      DevBuf<double> W(n);
+      const int nb = H.apply(basis[k].p, W.p, 1.0 / sc[k], 0.0, true);
+      alpha = H.dot_result(nb) / sc[k];
+      std::vector<const double *> src;
+      std::vector<double> cf;
+      src.push_back(basis[k].p);
+      cf.push_back(-alpha / sc[k]);
+      if (k > 0) { src.push_back(basis[k - 1].p); cf.push_back(-R.beta[k - 1] / sc[k - 1]); }
+      beta = std::sqrt(comb(n, W.p, 1.0, src, cf, true));
+      basis.push_back(std::move(W));
+      sc.push_back(beta);
+    } else {
+      const double cb = (k > 0) ? -R.beta[k - 1] / bB : 0.0;
+      const int nb = H.apply(A.p, B.p, 1.0 / bA, cb, true);
+      alpha = H.dot_result(nb) / bA;
+      beta = std::sqrt(comb(n, B.p, 1.0, {A.p}, {-alpha / bA}, true));
+      std::swap(A.p, B.p);
+      bB = bA;
+      bA = beta;
+    }
+    R.alpha.push_back(alpha);
+    R.beta.push_back(beta);
+    double e0, zl;
+    tridiag_lowest((int)R.alpha.size(), R.alpha.data(), R.beta.data(), e0, zl);
+    R.E0 = e0;
+    ritz = std::fabs(zl * beta);
+    if ((int)R.alpha.size() > max_iter)
+      fail(ED_ERR_NOT_CONVERGED, "Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+  }
+  const int kk = (int)R.alpha.size();
+  R.niter = kk;
+  std::vector<double> z;
+  double e0;
+  tridiag_eigvec0(kk, R.alpha.data(), R.beta.data(), e0, z);
+  if (store) {
+    std::vector<const double *> src;
+    std::vector<double> cf;
+    for (int j = 0; j < kk; ++j) {
+      src.push_back(basis[j].p);
+      cf.push_back(z[j] / sc[j]);
+    }
+    comb(n, d_gs, 0.0, src, cf, false);
+  } else {
+    // LanczosEigenvector replay (Lanczos.hpp:109-143)
+    d2d(A.p, d_start, n);
+    bA = 1.0;
+    bB = 1.0;
+    d2d(d_gs, d_start, n);
+    scale(n, d_gs, z[0], stream());
+    for (int j = 1; j < kk; ++j) {
+      const double cb = (j > 1) ? -R.beta[j - 2] / bB : 0.0;
+      H.apply(A.p, B.p, 1.0 / bA, cb, false);
+      replay_step(n, B.p, A.p, d_gs, -R.alpha[j - 1] / bA, z[j] / R.beta[j - 1], stream());
+      std::swap(A.p, B.p);
+      bB = bA;
+      bA = R.beta[j - 1];
+    }
+  }
+  sync();
+  return R;
+}
+
+// ---------------------------------------------------------------------------
+// LanczosGreen: same recurrence, stops on the products of beta.
+// ---------------------------------------------------------------------------
+void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, std::vector<double> &b) {
+  Params &P = params();
+  const double acc = P.getd("accur_continued_fraction");
+  const size_t max_cf = (size_t)P.geti("max_iter_CF");
+  const size_t n = H.dim;
+  a.clear();
+  b.assign(1, 1.0);
+  DevBuf<double> A(n), B(n);
+  d2d(A.p, d_psi, n);
+  double bA = 1.0, bB = 1.0, prod = 1.0;
+  while (prod > acc && std::fabs(b.back()) > acc && a.size() < max_cf) {
+    const size_t k = a.size();
+    const double cb = k ? -b.back() / bB : 0.0;
+    const int nb = H.apply(A.p, B.p, 1.0 / bA, cb, true);
+    const double alpha = H.dot_result(nb) / bA;
+    const double beta = std::sqrt(comb(n, B.p, 1.0, {A.p}, {-alpha / bA}, true));
+    a.push_back(alpha);
+    b.push_back(beta);
+    prod *= std::fabs(beta);
+    std::swap(A.p, B.p);
+    bB = bA;
+    bA = beta;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// band Lanczos (Freund), restated from src_ed/Hamiltonian/Lanczos.hpp:214-395
+// ---------------------------------------------------------------------------
+void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::vector<double> &evals,
+                  std::vector<double> &Q, int M_in) {
+  Params &Pm = params();
+  const size_t dim = H.dim;
+  const int p = (int)phi.size();
+  const double accur_deflation = Pm.getd("accur_deflation");
+  const double accur_bl = std::max(0.0, Pm.getd("accur_band_lanczos"));
+  const int max_iter_bl = (int)Pm.geti("max_iter_BL");
+  const double min_gap = Pm.getd("band_lanczos_minimum_gap");
+  const bool no_deg = Pm.getb("no_degenerate_BL");
+  evals.clear();
+  Q.clear();
+  if (dim == 0 || p == 0) return;
+  int M0 = (M_in < 0) ? (int)(14 * p * std::log(1.0 * dim)) : M_in;
+  int pc = p;
+  if (M0 > max_iter_bl) M0 = max_iter_bl;
+  const int M0p = (M0 / p + 1) * p;
+  M0 = M0p - pc;
+  std::vector<std::unique_ptr<DevBuf<double>>> v((size_t)M0p + 1);
+  std::vector<int> Idef((size_t)M0p + 1, 0), def((size_t)M0 + 1, 0);
+  std::vector<double> t((size_t)M0 * M0, 0.0), Pmat((size_t)p * M0, 0.0);
+  auto T = [&](int i, int j) -> double & { return t[(size_t)i + (size_t)j * M0]; };
+  for (int i = 0; i < p; ++i) {
+    v[i].reset(new DevBuf<double>(dim));
+    d2d(v[i]->p, phi[i], dim);
+  }
+  int nd = 0, j;
+  double ev_old = 1e12;
+  double znext = -1.0;  // ||v[j]||^2 carried from the previous step when known
+  for (j = 0; j < M0; ++j) {
+    const double z = std::sqrt(znext >= 0.0 ? znext : ddot(dim, v[j]->p, v[j]->p));
+    znext = -1.0;
+    if (z < accur_deflation) {
+      if (j >= pc) {
+        Idef[nd] = j - pc;
+        def[j - pc] = 1;
+        nd++;
+      }
+      pc--;
+      if (pc == 0) break;
+      v[j].reset();
+      for (int k = j; k < j + pc; ++k) v[k] = std::move(v[k + 1]);
+      v[j + pc].reset();
+      j--;
+      continue;
+    }
+    scale(dim, v[j]->p, 1.0 / z, stream());
+    if (j >= pc) T(j, j - pc) = z;
+    // P(k, j) = <phi_k, v_j> and step (6) dots <v_j, v_k>, k = j+1 .. j+pc-1
+    const int jm = j + pc;
+    std::vector<const double *> db;
+    for (int k = 0; k < p; ++k) db.push_back(phi[k]);
+    for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
+    std::vector<double> dv = dots(dim, v[j]->p, db);
+    for (int k = 0; k < p; ++k) Pmat[(size_t)k + (size_t)j * p] = dv[k];
+    if (jm > j + 1) {
+      std::vector<double *> ys;
+      std::vector<double> cs;
+      for (int k = j + 1; k < jm; ++k) {
+        const double zz = dv[p + (k - j - 1)];
+        ys.push_back(v[k]->p);
+        cs.push_back(-zz);
+        if (k >= pc) T(j, k - pc) = zz;
+      }
+      for (size_t s0 = 0; s0 < ys.size(); s0 += 16) {
+        const int m = (int)std::min<size_t>(16, ys.size() - s0);
+        const bool first = s0 == 0;
+        const int nb = multi_axpy_src(dim, v[j]->p, m, ys.data() + s0, cs.data() + s0,
+                                      first ? work().part.p : nullptr, stream());
+        if (first) znext = sum_partials(work().part.p, nb);  // ||v[j+1]||^2 after step (6)
+      }
+    }
+    // step (7): v[j+pc] = H v_j
+    v[j + pc].reset(new DevBuf<double>(dim));
+    H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, false);
+    int k0 = 0;
+    if (j > pc) {
+      k0 = j - pc;
+      if (def[k0 - 1] == 0) v[k0 - 1].reset();
+    }
+    // step (8): v[j+pc] -= sum_{k0 <= k < j} t(k, j) v_k, with <v[j+pc], v_j> when nothing follows
+    std::vector<const double *> src;
+    std::vector<double> cf;
+    for (int k = k0; k < j; ++k) {
+      T(k, j) = T(j, k);
+      src.push_back(v[k]->p);
+      cf.push_back(-T(k, j));
+    }
+    double wdot = 0.0;
+    comb(dim, v[j + pc]->p, 1.0, src, cf, false, v[j]->p, &wdot);
+    // step (9): deflated vectors (classical Gram-Schmidt over the mutually orthogonal set)
+    if (nd > 0) {
+      std::vector<const double *> dvv;
+      for (int k = 0; k < nd; ++k) dvv.push_back(v[Idef[k]]->p);
+      std::vector<double> w = dots(dim, v[j + pc]->p, dvv);
+      std::vector<double> c2(nd);
+      for (int k = 0; k < nd; ++k) {
+        c2[k] = -w[k];
+        T(Idef[k], j) = w[k];
+      }
+      comb(dim, v[j + pc]->p, 1.0, dvv, c2, false, v[j]->p, &wdot);
+    }
+    // final projection on v_j
+    {
+      const double nn = comb(dim, v[j + pc]->p, 1.0, {v[j]->p}, {-wdot}, true);
+      T(j, j) = wdot;
+      if (pc == 1) znext = nn;  // v[j+1] is the vector just built
+    }
+    for (int k = 0; k < nd; ++k) T(j, Idef[k]) = T(Idef[k], j);
+    // step (12): convergence of the lowest eigenvalue of the projected matrix
+    if (j >= 4 * p && j % p == 0) {
+      std::vector<double> Hp((size_t)j * j);
+      for (int c = 0; c < j; ++c)
+        for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
+      double e0, e1 = 0.0;
+      if (nd == 0) {
+        e0 = band_kth_eigenvalue(j, p, Hp.data(), 0);
+        if (no_deg) e1 = band_kth_eigenvalue(j, p, Hp.data(), 1);
+      } else {
+        std::vector<double> ev(j);
+        symeig(j, Hp.data(), ev.data(), nullptr);
+        e0 = ev[0];
+        e1 = j > 1 ? ev[1] : ev[0];
+      }
+      const double test = std::fabs(e0 - ev_old);
+      ev_old = e0;
+      if (no_deg && e1 - e0 < min_gap)
+        fail(ED_ERR_DEGENERATE, "band Lanczos: the gap between the first two eigenvalues is smaller than " +
+                                    std::to_string(min_gap));
+      if (test < accur_bl) break;
+    }
+  }
+  const int M = j;
+  if (M == 0) return;
+  std::vector<double> Hp((size_t)M * M);
+  for (int c = 0; c < M; ++c)
+    for (int r = 0; r < M; ++r) Hp[(size_t)r + (size_t)c * M] = T(r, c);
+  const int kmax = std::min(p, M);
+  Q.assign((size_t)p * M, 0.0);
+  if (nd == 0) {
+    std::vector<double> Rr;
+    band_eig_first_rows(M, p, Hp.data(), kmax, evals, Rr);
+    for (int c = 0; c < M; ++c)
+      for (int i = 0; i < p; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < kmax; ++k) s += Pmat[(size_t)i + (size_t)k * p] * Rr[(size_t)k * M + c];
+        Q[(size_t)i + (size_t)c * p] = s;
+      }
+  } else {
+    evals.resize(M);
+    std::vector<double> U((size_t)M * M);
+    symeig(M, Hp.data(), evals.data(), U.data());
+    for (int c = 0; c < M; ++c)
+      for (int i = 0; i < p; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < kmax; ++k) s += Pmat[(size_t)i + (size_t)k * p] * U[(size_t)k + (size_t)c * M];
+        Q[(size_t)i + (size_t)c * p] = s;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Davidson-Liu (src_ed/Hamiltonian/Davidson.hpp:38-230)
+// ---------------------------------------------------------------------------
+static int gram_schmidt(std::vector<std::unique_ptr<DevBuf<double>>> &b, size_t dim, double accur) {
+  // src_util/vector_num.hpp:217-235 with k = 0
+  for (size_t i = 0; i < b.size(); ++i) {
+    const double z = std::sqrt(ddot(dim, b[i]->p, b[i]->p));
+    if (z < accur) {
+      std::swap(b[i], b.back());
+      b.pop_back();
+      --i;
+      continue;
+    }
+    scale(dim, b[i]->p, 1.0 / z, stream());
+    if (i + 1 < b.size()) {
+      std::vector<const double *> rest;
+      for (size_t j2 = i + 1; j2 < b.size(); ++j2) rest.push_back(b[j2]->p);
+      std::vector<double> zz = dots(dim, b[i]->p, rest);
+      std::vector<double *> ys;
+      std::vector<double> cs;
+      for (size_t j2 = i + 1; j2 < b.size(); ++j2) {
+        ys.push_back(b[j2]->p);
+        cs.push_back(-zz[j2 - i - 1]);
+      }
+      for (size_t s0 = 0; s0 < ys.size(); s0 += 16) {
+        const int m = (int)std::min<size_t>(16, ys.size() - s0);
+        multi_axpy_src(dim, b[i]->p, m, ys.data() + s0, cs.data() + s0, nullptr, stream());
+      }
+    }
+  }
+  return (int)b.size();
+}
+
+int davidson(Hamiltonian &H, int K, double accuracy, std::vector<double> &evals, std::vector<DevBuf<double>> &x) {
+  const size_t dim = H.dim;
+  const int max_iter = (int)params().geti("max_iter_lanczos");
+  const double TINY = 1e-14;
+  if ((size_t)K > dim) K = (int)dim;
+  DevBuf<double> d(dim);
+  H.diag(d.p);
+  std::vector<double> dh = d.to_host();
+  // find_lowest (src_util/vector_num.hpp:244-267)
+  std::vector<size_t> low(K);
+  std::vector<double> minv(K);
+  for (int i = 0; i < K; ++i) minv[i] = 1.0e12 + 1.0 * i;
+  for (size_t i = 0; i < dim; ++i) {
+    int jj;
+    for (jj = K - 1; jj >= 0; jj--)
+      if (dh[i] > minv[jj]) break;
+    jj++;
+    if (jj < K) {
+      for (int k = K - 1; k > jj; k--) { minv[k] = minv[k - 1]; low[k] = low[k - 1]; }
+      minv[jj] = dh[i];
+      low[jj] = i;
+    }
+  }
+  std::vector<std::unique_ptr<DevBuf<double>>> b, h;
+  x.clear();
+  for (int k = 0; k < K; ++k) {
+    b.emplace_back(new DevBuf<double>(dim));
+    h.emplace_back(new DevBuf<double>(dim));
+    x.emplace_back(dim);
+    fill(dim, b[k]->p, 0.0, stream());
+    set_unit(b[k]->p, low[k], stream());
+    H.apply(b[k]->p, h[k]->p, 1.0, 0.0, false);
+  }
+  int nb = K;
+  std::vector<double> bhb((size_t)nb * nb);
+  {
+    // bhb(i, j) = h_j[low_i]
+    for (int jj = 0; jj < nb; ++jj) {
+      std::vector<double> hv = h[jj]->to_host();
+      for (int i = 0; i < nb; ++i) bhb[(size_t)i + (size_t)jj * nb] = hv[low[i]];
+    }
+  }
+  evals.assign(K, 0.0);
+  int iter;
+  for (iter = 0; iter < max_iter; ++iter) {
+    std::vector<double> mu(nb), a((size_t)nb * nb);
+    symeig(nb, bhb.data(), mu.data(), a.data());
+    std::vector<const double *> bp, hp;
+    for (int q = 0; q < nb; ++q) { bp.push_back(b[q]->p); hp.push_back(h[q]->p); }
+    for (int k = 0; k < K; ++k) {
+      std::vector<double> cf(nb);
+      for (int q = 0; q < nb; ++q) cf[q] = a[(size_t)q + (size_t)k * nb];
+      comb(dim, x[k].p, 0.0, bp, cf, false);
+    }
+    std::vector<std::unique_ptr<DevBuf<double>>> r;
+    std::vector<int> rk;
+    for (int k = 0; k < K; ++k) {
+      std::unique_ptr<DevBuf<double>> rt(new DevBuf<double>(dim));
+      std::vector<const double *> src = hp;
+      std::vector<double> cf(nb);
+      for (int q = 0; q < nb; ++q) cf[q] = a[(size_t)q + (size_t)k * nb];
+      src.push_back(x[k].p);
+      cf.push_back(-mu[k]);
+      const double res = std::sqrt(comb(dim, rt->p, 0.0, src, cf, true));
+      if (res > accuracy) { r.push_back(std::move(rt)); rk.push_back(k); }
+    }
+    const int Qn = (int)r.size();
+    if (Qn == 0) {
+      for (int k = 0; k < K; ++k) evals[k] = mu[k];
+      break;
+    }
+    while ((int)b.size() < K + 2 * Qn) b.emplace_back(new DevBuf<double>(dim));
+    for (int c = 0; c < Qn; ++c) std::swap(b[rk[c]], b[K + c]);
+    for (int k = 0; k < K; ++k) {
+      // b[k] <- x[k] (the Ritz vectors); x[k] keeps a buffer for the next round
+      std::swap(b[k]->p, x[k].p);
+      std::swap(b[k]->n, x[k].n);
+    }
+    for (int c = 0; c < Qn; ++c) {
+      precond(dim, r[c]->p, d.p, mu[rk[c]], TINY, stream());
+      b[K + Qn + c] = std::move(r[c]);
+    }
+    b.resize(K + 2 * Qn);
+    nb = gram_schmidt(b, dim, TINY);
+    if (nb == K) {
+      for (int k = 0; k < K; ++k) evals[k] = mu[k];
+      break;
+    }
+    h.clear();
+    for (int k = 0; k < nb; ++k) {
+      h.emplace_back(new DevBuf<double>(dim));
+      H.apply(b[k]->p, h[k]->p, 1.0, 0.0, false);
+    }
+    bhb.assign((size_t)nb * nb, 0.0);
+    for (int i = 0; i < nb; ++i) {
+      std::vector<const double *> hh;
+      for (int k = 0; k <= i; ++k) hh.push_back(h[k]->p);
+      std::vector<double> zz = dots(dim, b[i]->p, hh);
+      for (int k = 0; k <= i; ++k) {
+        bhb[(size_t)i + (size_t)k * nb] = zz[k];
+        bhb[(size_t)k + (size_t)i * nb] = zz[k];
+      }
+    }
+    for (int k = 0; k < K; ++k) evals[k] = mu[k];
+  }
+  if (iter == max_iter) fail(ED_ERR_NOT_CONVERGED, "Davidson method failed to converge!");
+  sync();
+  return iter;
+}
+
+// ---------------------------------------------------------------------------
+// block Lanczos, QR variant (src_ed/Hamiltonian/Lanczos.hpp:426-572)
+// ---------------------------------------------------------------------------
+int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q0, int M0, std::vector<double> &A,
+                  std::vector<double> &B) {
+  Params &Pm = params();
+  const size_t dim = H.dim;
+  const int p = (int)q0.size();
+  const double accur_deflation = Pm.getd("accur_deflation");
+  const double accur_bl = std::max(0.0, Pm.getd("accur_band_lanczos"));
+  const int max_iter_bl = (int)Pm.geti("max_iter_BL");
+  if (M0 > max_iter_bl) M0 = max_iter_bl;
+  A.clear();
+  B.clear();
+  const int pp = p * p;
+  std::vector<DevBuf<double>> Qp, Qc, Z;
+  for (int k = 0; k < p; ++k) {
+    Qp.emplace_back(dim);
+    Qc.emplace_back(dim);
+    Z.emplace_back(dim);
+    d2d(Qc[k].p, q0[k], dim);
+  }
+  // orthonormalize the starting block (MGS)
+  for (int l = 0; l < p; ++l) {
+    for (int k = 0; k < l; ++k) {
+      const double z = ddot(dim, Qc[k].p, Qc[l].p);
+      comb(dim, Qc[l].p, 1.0, {Qc[k].p}, {-z}, false);
+    }
+    const double nrm = std::sqrt(ddot(dim, Qc[l].p, Qc[l].p));
+    if (nrm < accur_deflation)
+      fail(ED_ERR_INVALID, "blockLanczos: starting block is rank-deficient at column " + std::to_string(l));
+    scale(dim, Qc[l].p, 1.0 / nrm, stream());
+  }
+  double ev_old = 1e12;
+  int j;
+  for (j = 0; j < M0; ++j) {
+    for (int k = 0; k < p; ++k) H.apply(Qc[k].p, Z[k].p, 1.0, 0.0, false);
+    std::vector<double> Aj(pp);
+    for (int l = 0; l < p; ++l) {
+      std::vector<const double *> qs;
+      for (int k = 0; k <= l; ++k) qs.push_back(Qc[k].p);
+      std::vector<double> zz = dots(dim, Z[l].p, qs);
+      for (int k = 0; k <= l; ++k) {
+        Aj[k + l * p] = zz[k];
+        Aj[l + k * p] = zz[k];
+      }
+    }
+    A.insert(A.end(), Aj.begin(), Aj.end());
+    for (int l = 0; l < p; ++l) {
+      std::vector<const double *> src;
+      std::vector<double> cf;
+      for (int k = 0; k < p; ++k) { src.push_back(Qc[k].p); cf.push_back(-Aj[k + l * p]); }
+      if (j > 0) {
+        const double *Bp = B.data() + (size_t)(j - 1) * pp;
+        for (int k = 0; k < p; ++k) { src.push_back(Qp[k].p); cf.push_back(-Bp[l + k * p]); }
+      }
+      comb(dim, Z[l].p, 1.0, src, cf, false);
+    }
+    std::vector<double> Bj(pp, 0.0);
+    bool singular = false;
+    for (int l = 0; l < p; ++l) {
+      for (int k = 0; k < l; ++k) {
+        const double z = ddot(dim, Z[k].p, Z[l].p);
+        Bj[k + l * p] = z;
+        comb(dim, Z[l].p, 1.0, {Z[k].p}, {-z}, false);
+      }
+      const double nrm = std::sqrt(ddot(dim, Z[l].p, Z[l].p));
+      if (nrm < accur_deflation) {
+        Bj[l + l * p] = 0.0;
+        for (int ll = l; ll < p; ++ll) fill(dim, Z[ll].p, 0.0, stream());
+        singular = true;
+        break;
+      }
+      Bj[l + l * p] = nrm;
+      scale(dim, Z[l].p, 1.0 / nrm, stream());
+    }
+    B.insert(B.end(), Bj.begin(), Bj.end());
+    bool conv = false;
+    if ((j + 1) >= 2 * p && (j + 1) % p == 0) {
+      const int sz = (j + 1) * p;
+      std::vector<double> Tm((size_t)sz * sz, 0.0);
+      for (int jj = 0; jj <= j; ++jj)
+        for (int r = 0; r < p; ++r)
+          for (int c = 0; c < p; ++c) Tm[(jj * p + r) + (size_t)(jj * p + c) * sz] = A[(size_t)jj * pp + r + c * p];
+      for (int jj = 0; jj < j; ++jj)
+        for (int r = 0; r < p; ++r)
+          for (int c = 0; c < p; ++c) {
+            const double bv = B[(size_t)jj * pp + r + c * p];
+            Tm[((jj + 1) * p + r) + (size_t)(jj * p + c) * sz] = bv;
+            Tm[(jj * p + c) + (size_t)((jj + 1) * p + r) * sz] = bv;
+          }
+      const double e0 = band_kth_eigenvalue(sz, 2 * p, Tm.data(), 0);
+      const double delta = std::fabs(e0 - ev_old);
+      ev_old = e0;
+      if (delta < accur_bl) conv = true;
+    }
+    if (conv || singular) { ++j; break; }
+    for (int k = 0; k < p; ++k) {
+      std::swap(Qp[k].p, Qc[k].p);
+      std::swap(Qc[k].p, Z[k].p);
+    }
+  }
+  sync();
+  return (int)(A.size() / pp);
+}
+
+}  // namespace ed

@@ -1,0 +1,42 @@
+// ed_solvers.hpp -- Krylov drivers (host control, device vectors).
+#pragma once
+
+#include <vector>
+
+#include "ed_model.hpp"
+
+namespace ed {
+
+struct LanczosGS {
+  double E0 = 0.0;
+  int niter = 0;
+  std::vector<double> alpha, beta;
+};
+
+// LanczosEigenvalue + LanczosEigenvector (src_ed/Hamiltonian/Lanczos.hpp:41-143).
+// d_start: normalized start vector (device); d_gs receives the Ritz vector.
+LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs);
+
+// LanczosGreen (src_ed/Hamiltonian/Lanczos.hpp:156-183); d_psi normalized.
+void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, std::vector<double> &b);
+
+// bandLanczos + build_Q_matrix (Lanczos.hpp:214-395, Hamiltonian_base.hpp:230-254).
+// phi: p device vectors (unnormalized).  Returns eigenvalues (ascending) and
+// Q (p x M column-major) = P[:, :p] U[:p, :].
+void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::vector<double> &evals,
+                  std::vector<double> &Q, int M_in = -1);
+
+// Davidson-Liu (src_ed/Hamiltonian/Davidson.hpp:38-230); evecs: K device buffers of dim.
+int davidson(Hamiltonian &H, int K, double accuracy, std::vector<double> &evals, std::vector<DevBuf<double>> &evecs);
+
+// blockLanczos, QR variant (src_ed/Hamiltonian/Lanczos.hpp:426-572); q: p orthonormal device vectors.
+// A, B: floors of p x p column-major.  Returns the number of floors.
+int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q, int M0, std::vector<double> &A,
+                  std::vector<double> &B);
+
+// <a, b> on device
+double ddot(size_t n, const double *a, const double *b);
+// per-thread device buffer for per-block partial sums (16 x partials_cap())
+double *work_partials();
+
+}  // namespace ed

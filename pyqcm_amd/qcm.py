@@ -1,0 +1,385 @@
+"""Python mirror of the reference's ED bindings (module ``qcm``,
+src_python/qcm_ED_wrap.hpp:25-382), backed by the MI355X C-ABI.
+
+Function names, argument meaning, index conventions (1-based matrix
+elements from Python, 0-based in C) and error behaviour (an exception on any
+failure; qcm.error in the reference, EDError here) follow the reference, so
+code written against ``qcm`` ED functions runs unchanged:
+
+    new_model, new_operator, new_model_instanceC, ground_state_solve,
+    Green_function_solveC, Green_function, Green_function_dimensionC,
+    self_energyC, hybridization_functionC, cluster_averages, qmatrix,
+    mixingC, model_sizeC, set_global_parameterC, erase_model_instance
+
+Vectorized variants (``Green_function_grid``) take whole frequency grids, so
+the device kernels evaluate every frequency in one launch.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import EDError, check, lib
+
+__all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_model_instance",
+           "ground_state_solve", "Green_function_solveC", "Green_function", "Green_function_grid",
+           "Green_function_dimensionC", "self_energyC", "hybridization_functionC", "cluster_averages",
+           "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
+           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states"]
+
+_models = {}      # name -> (n_sites, n_bath, [operator names])
+_instances = {}   # label -> (model name, {name: value})
+
+_BOOL, _INT, _DOUBLE, _CHAR = "b", "i", "d", "c"
+# global parameter types (src_util/global_parameter.cpp:44-90)
+_PARAM_TYPES = {
+    "check_lanczos_residual": _BOOL, "no_degenerate_BL": _BOOL, "nosym": _BOOL, "continued_fraction": _BOOL,
+    "verb_ED": _BOOL, "verb_Hilbert": _BOOL, "verb_warning": _BOOL, "parallel_sectors": _BOOL,
+    "merge_states": _BOOL, "combine_mcf": _BOOL, "block_Lanczos_QR": _BOOL, "store_lanczos_basis": _BOOL,
+    "accur_band_lanczos": _DOUBLE, "accur_continued_fraction": _DOUBLE, "accur_Davidson": _DOUBLE,
+    "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
+    "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
+    "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
+    "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
+    "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
+    "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
+}
+
+
+def set_global_parameterC(name, value=None):
+    """pyqcm.set_global_parameter (pyqcm/__init__.py:2259-2273): None sets a boolean to True."""
+    t = _PARAM_TYPES.get(name)
+    if t is None:
+        raise EDError(1, "unknown global parameter " + name)
+    L = lib()
+    if value is None:
+        if t != _BOOL:
+            raise EDError(1, "global parameter %s needs a value" % name)
+        check(L.ed_set_global_bool(name.encode(), 1))
+    elif t == _BOOL:
+        check(L.ed_set_global_bool(name.encode(), int(bool(value))))
+    elif t == _INT:
+        check(L.ed_set_global_int(name.encode(), int(value)))
+    elif t == _DOUBLE:
+        check(L.ed_set_global_double(name.encode(), float(value)))
+    else:
+        check(L.ed_set_global_char(name.encode(), str(value)[0].encode()))
+
+
+set_global_parameter = set_global_parameterC
+
+
+def new_model(name, n_sites, n_bath, generators=None, bath_irrep=False):
+    """qcm.new_model (src_python/qcm_ED_wrap.hpp:145-163); trivial point group only."""
+    ng = 0 if generators is None else len(generators)
+    check(lib().ed_new_model(name.encode(), int(n_sites), int(n_bath), None, ng, int(bool(bath_irrep))))
+    _models.setdefault(name, (int(n_sites), int(n_bath), []))
+
+
+def new_operator(model, op, type, elements):
+    """qcm.new_operator (src_python/qcm_ED_wrap.hpp:174-217): elements are 1-based (r, c, v)
+    with r <= c; they are decremented before reaching ED::new_operator."""
+    rr, cc, vv = [], [], []
+    fac = 0.5 if type == "anomalous" else 1.0
+    for i, e in enumerate(elements):
+        if len(e) != 3:
+            raise EDError(1, "element %d of argument 4 of 'new_operator' should be a 3-tuple" % i)
+        r, c, v = int(e[0]), int(e[1]), float(e[2])
+        if r > c and type != "general_interaction":
+            raise EDError(1, "the first index of element %d of argument 4 of 'new_operator' cannot be bigger "
+                             "than the second index" % i)
+        if r == 0 or c == 0:
+            raise EDError(1, "indices in matrix elements of operators are labelled starting at 1, not at 0.")
+        rr.append(r - 1)
+        cc.append(c - 1)
+        vv.append(fac * v)
+    n = len(rr)
+    ra = (C.c_int * max(n, 1))(*rr)
+    ca = (C.c_int * max(n, 1))(*cc)
+    va = (C.c_double * max(n, 1))(*vv)
+    check(lib().ed_new_operator(model.encode(), op.encode(), type.encode(), n, ra, ca, va))
+    if model in _models and op not in _models[model][2]:
+        _models[model][2].append(op)
+
+
+def new_model_instanceC(name, values, sectors, label=0):
+    """qcm.new_model_instanceC (src_python/qcm_ED_wrap.hpp:165-172)."""
+    names = list(values.keys())
+    vals = [float(values[k]) for k in names]
+    na = (C.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+    va = (C.c_double * max(len(vals), 1))(*vals)
+    check(lib().ed_new_model_instance(name.encode(), len(names), na, va, sectors.encode(), int(label)))
+    _instances[int(label)] = (name, dict(values))
+
+
+new_model_instance = new_model_instanceC
+
+
+def erase_model_instance(label=0):
+    check(lib().ed_erase_model_instance(int(label)))
+    _instances.pop(int(label), None)
+
+
+def ground_state_solve(label=0):
+    """qcm.ground_state_solve: returns (energy, sector string)."""
+    E0 = C.c_double()
+    buf = C.create_string_buffer(1024)
+    check(lib().ed_ground_state_solve(int(label), C.byref(E0), buf, 1024))
+    return E0.value, buf.value.decode()
+
+
+def Green_function_solveC(label=0):
+    check(lib().ed_green_function_solve(int(label)))
+
+
+def Green_function_dimensionC(label=0):
+    d = C.c_int()
+    check(lib().ed_green_function_dimension(int(label), C.byref(d)))
+    return d.value
+
+
+def _grid(z):
+    z = np.atleast_1d(np.asarray(z, dtype=np.complex128))
+    return np.ascontiguousarray(z), z.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _gf_call(fn, label, z, spin_down, *extra):
+    zz, zp = _grid(z)
+    d = Green_function_dimensionC(label)
+    out = np.zeros(len(zz) * d * d, dtype=np.complex128)
+    check(fn(int(label), len(zz), zp, int(spin_down), *extra, out.ctypes.data_as(C.POINTER(C.c_double))))
+    # each block is column-major d x d, exposed with the reference's flat copy
+    # into a (d, d) C-ordered array (src_python/qcm_nb.hpp:27-41)
+    return out.reshape(len(zz), d, d)
+
+
+def Green_function_grid(z, spin_down=0, label=0, blocks=0):
+    """G(z) for a whole frequency grid: array (nz, d, d)."""
+    return _gf_call(lib().ed_green_function, label, z, spin_down, int(blocks))
+
+
+def Green_function(z, spin_down=0, label=0, blocks=0):
+    """qcm.Green_function (src_python/qcm_ED_wrap.hpp:65-72): d x d matrix at one z."""
+    return Green_function_grid([z], spin_down, label, blocks)[0]
+
+
+def self_energyC(z, spin_down=0, label=0):
+    return _gf_call(lib().ed_self_energy, label, [z], spin_down)[0]
+
+
+def hybridization_functionC(z, spin_down=0, label=0):
+    return _gf_call(lib().ed_hybridization_function, label, [z], spin_down)[0]
+
+
+def cluster_averages(label=0):
+    """qcm.cluster_averages: dict name -> (average, variance)."""
+    cap = 256
+    ave = np.zeros(cap)
+    var = np.zeros(cap)
+    n = C.c_int()
+    check(lib().ed_cluster_averages(int(label), cap, ave.ctypes.data_as(C.POINTER(C.c_double)),
+                                    var.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+    names = sorted(_instances.get(int(label), ("", {}))[1].keys())
+    return {names[k] if k < len(names) else str(k): (ave[k], var[k]) for k in range(n.value)}
+
+
+def qmatrix(label=0, spin_down=0):
+    """Lehmann representation (poles e[M], residues Q[L, M]) of the first state."""
+    L_, M_ = C.c_int(), C.c_int()
+    check(lib().ed_qmatrix(int(label), int(spin_down), 0, None, None, C.byref(L_), C.byref(M_)))
+    e = np.zeros(max(M_.value, 1))
+    v = np.zeros(max(M_.value * L_.value, 1))
+    check(lib().ed_qmatrix(int(label), int(spin_down), M_.value, e.ctypes.data_as(C.POINTER(C.c_double)),
+                           v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(L_), C.byref(M_)))
+    return e[:M_.value], v[:M_.value * L_.value].reshape(M_.value, L_.value).T.copy()
+
+
+def states(label=0):
+    cap = 64
+    e = np.zeros(cap)
+    w = np.zeros(cap)
+    n = C.c_int()
+    check(lib().ed_states(int(label), cap, e.ctypes.data_as(C.POINTER(C.c_double)),
+                          w.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+    return e[:n.value], w[:n.value]
+
+
+def mixingC(label=0):
+    m = C.c_int()
+    check(lib().ed_mixing(int(label), C.byref(m)))
+    return m.value
+
+
+def model_sizeC(name):
+    return _models[name][0], _models[name][1]
+
+
+def half_basis(L, N):
+    dim = C.c_int64()
+    check(lib().ed_half_basis(L, N, None, 0, C.byref(dim)))
+    out = np.zeros(max(dim.value, 1), dtype=np.uint32)
+    check(lib().ed_half_basis(L, N, out.ctypes.data_as(C.POINTER(C.c_uint32)), dim.value, C.byref(dim)))
+    return out[:dim.value]
+
+
+def sector_parse(s):
+    N, S, R = C.c_int(), C.c_int(), C.c_int()
+    check(lib().ed_sector_parse(s.encode(), C.byref(N), C.byref(S), C.byref(R)))
+    return N.value, S.value, R.value
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Hamiltonian:
+    """Inner plug-in: a sector Hamiltonian of an instance (Hamiltonian<T>,
+    src_ed/Hamiltonian/Hamiltonian_base.hpp:43-82) living on the device."""
+
+    def __init__(self, label, sector, format="G"):
+        h = C.c_int64()
+        check(lib().ed_hamiltonian_create(int(label), sector.encode(), format.encode(), C.byref(h)))
+        self.h = h.value
+        d = C.c_int64()
+        check(lib().ed_hamiltonian_dim(self.h, C.byref(d)))
+        self.dim = d.value
+
+    def __del__(self):
+        try:
+            lib().ed_hamiltonian_destroy(self.h)
+        except Exception:
+            pass
+
+    def mult_add(self, x, y):
+        """y += H x (host arrays); returns y."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        check(lib().ed_hamiltonian_mult_add(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def apply(self, x):
+        return self.mult_add(x, np.zeros(self.dim))
+
+    def apply_device(self, dx_ptr, dy_ptr, beta=0.0):
+        check(lib().ed_hamiltonian_apply_device(self.h, C.c_void_p(dx_ptr), C.c_void_p(dy_ptr), float(beta)))
+
+    def diag(self):
+        d = np.zeros(self.dim)
+        check(lib().ed_hamiltonian_diag(self.h, _dptr(d)))
+        return d
+
+    def csr_pattern(self):
+        nnz = C.c_int64(0)
+        check(lib().ed_hamiltonian_csr_pattern(self.h, C.byref(nnz), None, None, None))
+        Iptr = np.zeros(self.dim + 1, dtype=np.int32)
+        J = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        v = np.zeros(max(nnz.value, 1))
+        check(lib().ed_hamiltonian_csr_pattern(self.h, C.byref(nnz), Iptr.ctypes.data_as(C.POINTER(C.c_int32)),
+                                               J.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(v)))
+        return Iptr, J[:nnz.value], v[:nnz.value]
+
+    def time(self, nrep=10, variant=0):
+        ms = C.c_double()
+        check(lib().ed_hamiltonian_time(self.h, int(nrep), C.byref(ms), int(variant)))
+        return ms.value
+
+    def set_variant(self, variant):
+        check(lib().ed_hamiltonian_set_variant(self.h, int(variant)))
+
+    def lanczos_gs(self, start):
+        start = np.ascontiguousarray(start, dtype=np.float64)
+        start = start / np.linalg.norm(start)
+        gs = np.zeros(self.dim)
+        E0 = C.c_double()
+        nit = C.c_int()
+        check(lib().ed_lanczos_gs(self.h, _dptr(start), C.byref(E0), _dptr(gs), C.byref(nit)))
+        return E0.value, gs, nit.value
+
+    def lanczos_green(self, psi, cap=2000):
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        a = np.zeros(cap)
+        b = np.zeros(cap + 1)
+        n = C.c_int()
+        check(lib().ed_lanczos_green(self.h, _dptr(psi), cap, _dptr(a), _dptr(b), C.byref(n)))
+        return a[:n.value], b[:n.value + 1]
+
+    def band_lanczos(self, phi, cap=4096):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        p = phi.shape[0]
+        e = np.zeros(cap)
+        Q = np.zeros(cap * p)
+        M = C.c_int()
+        check(lib().ed_band_lanczos(self.h, p, _dptr(phi), cap, _dptr(e), _dptr(Q), C.byref(M)))
+        m = M.value
+        return e[:m], Q[:p * m].reshape(m, p).T.copy()
+
+    def davidson(self, K):
+        ev = np.zeros(K)
+        vec = np.zeros(K * self.dim)
+        nit = C.c_int()
+        check(lib().ed_davidson(self.h, int(K), _dptr(ev), _dptr(vec), C.byref(nit)))
+        return ev, vec.reshape(K, self.dim), nit.value
+
+    def block_lanczos(self, q0, M0, cap=2000):
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        p = q0.shape[0]
+        A = np.zeros(cap * p * p)
+        B = np.zeros(cap * p * p)
+        n = C.c_int()
+        check(lib().ed_block_lanczos(self.h, p, _dptr(q0), int(M0), cap, _dptr(A), _dptr(B), C.byref(n)))
+        k = n.value
+        return (A[:k * p * p].reshape(k, p, p).transpose(0, 2, 1).copy(),
+                B[:k * p * p].reshape(k, p, p).transpose(0, 2, 1).copy())
+
+
+def create_or_destroy(L, pm, a, Nup, Ndw, x, z=1.0):
+    """y = z c_a x (pm=-1) or z c^dagger_a x (pm=+1) between factorized sectors (device)."""
+    spin = 1 if a >= L else 0
+    tNup, tNdw = Nup + (pm if spin == 0 else 0), Ndw + (pm if spin == 1 else 0)
+    dim = len(half_basis(L, tNup)) * len(half_basis(L, tNdw))
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(dim)
+    check(lib().ed_create_or_destroy(L, pm, a, Nup, Ndw, _dptr(x), float(z), _dptr(y)))
+    return y
+
+
+def cf_eval(fractions, z):
+    """Evaluate continued fractions [(a, b), ...] (already transformed) on grid z -> (nz, ncf)."""
+    off = [0]
+    A, B = [], []
+    for (a, b) in fractions:
+        A.extend(list(a))
+        B.extend(list(b)[:len(a)])
+        off.append(len(A))
+    A = np.array(A + [0.0])
+    B = np.array(B + [0.0])
+    offa = (C.c_int * len(off))(*off)
+    zz, zp = _grid(z)
+    out = np.zeros(len(zz) * len(fractions), dtype=np.complex128)
+    check(lib().ed_cf_eval(len(fractions), offa, _dptr(A), _dptr(B), len(zz), zp, _dptr(out.view(np.float64))))
+    return out.reshape(len(zz), len(fractions))
+
+
+def vdvh(e, V, z):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    L, M = V.shape
+    Vf = np.asfortranarray(V, dtype=np.float64)
+    zz, zp = _grid(z)
+    G = np.zeros(len(zz) * L * L, dtype=np.complex128)
+    check(lib().ed_vdvh(L, M, _dptr(e), Vf.ctypes.data_as(C.POINTER(C.c_double)), len(zz), zp,
+                        _dptr(G.view(np.float64))))
+    return G.reshape(len(zz), L, L).transpose(0, 2, 1)
+
+
+def mcf_eval(A, B, W, z):
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    nfl, p, _ = A.shape
+    q = W.shape[1]
+    Af = np.ascontiguousarray(A.transpose(0, 2, 1))
+    Bf = np.ascontiguousarray(B.transpose(0, 2, 1))
+    Wf = np.asfortranarray(W, dtype=np.float64)
+    zz, zp = _grid(z)
+    G = np.zeros(len(zz) * q * q, dtype=np.complex128)
+    check(lib().ed_mcf_eval(p, nfl, _dptr(Af), _dptr(Bf), Wf.ctypes.data_as(C.POINTER(C.c_double)), q, len(zz),
+                            zp, _dptr(G.view(np.float64))))
+    return G.reshape(len(zz), q, q).transpose(0, 2, 1)

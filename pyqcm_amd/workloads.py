@@ -185,3 +185,15 @@ def matsubara(n, beta):
 
 def real_axis(n=512, wmin=-8.0, wmax=8.0, eta=0.1):
     return np.linspace(wmin, wmax, n) + 1j * eta
+
+
+def setup(wl, label=0, sectors=None, model_name=None):
+    """Register ``wl`` through the reference-style API (1-based elements) and
+    create instance ``label``; returns the model name."""
+    from . import qcm
+    name = model_name or wl.name
+    qcm.new_model(name, wl.n_sites, wl.n_bath)
+    for op, (typ, el) in wl.operators.items():
+        qcm.new_operator(name, op, typ, [(r + 1, c + 1, v) for (r, c, v) in el])
+    qcm.new_model_instanceC(name, dict(wl.values), sectors or wl.sector, label)
+    return name

@@ -1,0 +1,69 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+header declares, and the host-only logic (bases, sectors, registries, error
+paths) behaves like the reference.  No GPU work is launched here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pyqcm_amd import _lib, qcm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "pyqcm_ed.h")).read()
+    return sorted(set(re.findall(r"\b(ed_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = _header_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(L, s), s
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_half_basis_host_matches_oracle(oracle):
+    for L, N in [(8, 4), (12, 6), (12, 7), (5, 0), (5, 5), (16, 8)]:
+        assert np.array_equal(qcm.half_basis(L, N), oracle.half_basis(L, N))
+
+
+def test_sector_parse_matches_reference_rules(oracle):
+    for s in ["R0:N8:S0", "R1:N7:S-1", "R0:N7", "R0:S1", "R0", "R3:N12:S2"]:
+        assert qcm.sector_parse(s) == oracle.sector_parse(s)
+
+
+def test_error_paths_raise():
+    qcm.new_model("capi_m", 2, 0)
+    with pytest.raises(_lib.EDError):
+        qcm.new_operator("capi_m", "bad", "one-body", [(2, 1, 1.0)])   # r > c
+    with pytest.raises(_lib.EDError):
+        qcm.new_operator("capi_m", "bad0", "one-body", [(0, 1, 1.0)])  # 0-based index
+    with pytest.raises(_lib.EDError) as e:
+        qcm.new_model_instanceC("no_such_model", {"t": 1.0}, "R0:N2:S0", 5)
+    assert e.value.code == 7
+    qcm.new_operator("capi_m", "t", "one-body", [(1, 2, -1.0), (3, 4, -1.0)])
+    with pytest.raises(_lib.EDError) as e:
+        qcm.new_model_instanceC("capi_m", {"t": 1.0}, "R0:N9:S0", 5)   # N > 2L
+    assert e.value.code == 2
+    with pytest.raises(_lib.EDError):
+        qcm.new_model_instanceC("capi_m", {"t": 1.0}, "R0:N2:S1", 5)   # parity
+    with pytest.raises(_lib.EDError):
+        qcm.set_global_parameterC("no_such_parameter", 1)
+    with pytest.raises(_lib.EDError):
+        qcm.new_model("capi_big", 30, 4)                                # > 32 orbitals
+
+
+def test_global_parameters_roundtrip():
+    qcm.set_global_parameterC("accur_lanczos", 1e-11)
+    import ctypes as C
+    v = C.c_double()
+    _lib.check(_lib.lib().ed_get_global_double(b"accur_lanczos", C.byref(v)))
+    assert v.value == 1e-11
+    qcm.set_global_parameterC("accur_lanczos", 1e-12)
+    n = C.c_long()
+    _lib.check(_lib.lib().ed_get_global_int(b"max_iter_lanczos", C.byref(n)))
+    assert n.value == 1000

@@ -1,0 +1,269 @@
+"""GPU parity of the HIP path against the CPU oracle (and, for tiny sectors, the
+dense Jordan-Wigner Hamiltonian).  Bars: bit-exact for bases / patterns /
+sign tables; 1e-12 relative for H.x; 1e-10 relative for E0; 1e-8 for G(z)."""
+import numpy as np
+import pytest
+
+from pyqcm_amd import qcm, workloads as W, _lib
+from tests.dense_ref import dense_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+_label = [1000]
+
+
+def _new_label():
+    _label[0] += 1
+    return _label[0]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    _lib.require_device()
+    qcm.set_global_parameterC("Hamiltonian_format", "G")
+    qcm.set_global_parameterC("GF_method", "L")
+    yield
+
+
+def _instance(wl, sector=None, name=None):
+    lab = _new_label()
+    W.setup(wl, label=lab, sectors=sector, model_name=name or ("%s_%d" % (wl.name, lab)))
+    return lab
+
+
+def _random_workload(seed, L=6):
+    rng = np.random.default_rng(seed)
+    ops, vals = {}, {}
+    for t in range(3):
+        el = []
+        for _ in range(2 * L):
+            i, j = sorted(rng.choice(L, 2, replace=False))
+            v = float(rng.normal())
+            el.append((int(i), int(j), v))
+            el.append((int(i) + L, int(j) + L, v if t != 1 else float(rng.normal())))
+        for i in range(L):
+            el.append((i, i, float(rng.normal())))
+        ops["h%d" % t] = ("one-body", el)
+        vals["h%d" % t] = float(rng.uniform(0.5, 1.5))
+    ops["V"] = ("interaction", [(i, i + L, 1.0) for i in range(L)] + [(0, 1 + L, 0.3), (1, 2, 0.2)])
+    vals["V"] = 2.0
+    return W.Workload("rnd%d" % seed, L, 0, ops, vals, "R0:N6:S0", list(range(L)))
+
+
+@pytest.mark.parametrize("fmt", ["G", "S", "N"])
+@pytest.mark.parametrize("seed,Nup,Ndw", [(0, 3, 3), (1, 2, 4), (2, 4, 3)])
+def test_hv_matches_dense_and_oracle(oracle, fmt, seed, Nup, Ndw):
+    wl = _random_workload(seed)
+    lab = _instance(wl, "R0:N%d:S%d" % (Nup + Ndw, Nup - Ndw))
+    H = qcm.Hamiltonian(lab, "R0:N%d:S%d" % (Nup + Ndw, Nup - Ndw), fmt)
+    Hd = dense_hamiltonian(wl.L, wl.terms(), Nup, Ndw)
+    assert H.dim == Hd.shape[0]
+    rng = np.random.default_rng(seed + 10)
+    x = rng.normal(size=H.dim)
+    y0 = rng.normal(size=H.dim)
+    got = H.mult_add(x, y0.copy())
+    ref = y0 + Hd @ x
+    M = oracle.Model(wl.L, wl.terms())
+    ref2 = M.hv_factorized(Nup, Ndw, x, y0)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-12 * scale
+    assert np.abs(got - ref2).max() <= 1e-12 * scale
+    assert np.allclose(H.diag(), np.diag(Hd), atol=1e-13)
+
+
+def test_csr_pattern_bit_exact(oracle):
+    wl = W.plaquette_bath(n_bath=2)
+    lab = _instance(wl, "R0:N6:S0")
+    H = qcm.Hamiltonian(lab, "R0:N6:S0", "S")
+    Ip, J, v = H.csr_pattern()
+    M = oracle.Model(wl.L, wl.terms())
+    Ip2, J2, v2, dg2 = M.csr(3, 3)
+    assert np.array_equal(Ip, Ip2)
+    assert np.array_equal(J, J2)
+    assert np.allclose(v, v2, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_factorized_variants_agree(oracle, variant):
+    wl = W.plaquette_bath(n_bath=4)     # L = 8, dim(4,4) = 4900
+    lab = _instance(wl, "R0:N8:S0")
+    H = qcm.Hamiltonian(lab, "R0:N8:S0", "G")
+    H.set_variant(variant)
+    x = np.random.default_rng(3).normal(size=H.dim)
+    got = H.apply(x)
+    ref = oracle.Model(wl.L, wl.terms()).hv_factorized(4, 4, x)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_ground_state_chain8_known_answer():
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    E0, sec = qcm.ground_state_solve(lab)
+    assert abs(E0 - (-20.23580699912965)) < 1e-10 * 20.3
+    assert sec.startswith("R0:N8:S0")
+
+
+def test_ground_state_1D_4_known_answer():
+    wl = W.hubbard_chain(4, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    E0, sec = qcm.ground_state_solve(lab)
+    assert round(E0, 7) == -9.9531453   # examples/1D_4/GS.tsv:2
+
+
+@pytest.mark.parametrize("fmt", ["G", "S", "N"])
+def test_lanczos_gs_matches_oracle(oracle, fmt):
+    wl = W.plaquette_bath(n_bath=4)
+    lab = _instance(wl, "R0:N8:S0")
+    H = qcm.Hamiltonian(lab, "R0:N8:S0", fmt)
+    x = np.random.default_rng(4).normal(size=H.dim)
+    E0, gs, nit = H.lanczos_gs(x)
+    E0r, gsr, al, be = oracle.Model(wl.L, wl.terms()).lanczos_gs(4, 4, x)
+    assert abs(E0 - E0r) <= 1e-10 * abs(E0r)
+    g = gs / np.linalg.norm(gs)
+    gr = gsr / np.linalg.norm(gsr)
+    assert abs(abs(g @ gr) - 1.0) < 1e-9
+
+
+def test_create_destroy_bit_exact(oracle):
+    L, Nup, Ndw = 6, 3, 2
+    x = np.random.default_rng(5).normal(size=len(qcm.half_basis(L, Nup)) * len(qcm.half_basis(L, Ndw)))
+    for a in [0, 3, 5, 6, 9, 11]:
+        for pm in (-1, 1):
+            got = qcm.create_or_destroy(L, pm, a, Nup, Ndw, x)
+            ref = oracle.create_or_destroy(L, pm, a, Nup, Ndw, x)
+            assert np.array_equal(got, ref), (a, pm)
+
+
+def test_gf_kernels_match_oracle(oracle):
+    rng = np.random.default_rng(6)
+    z = W.real_axis(64, -5, 5, 0.1)
+    fr = []
+    for n in [1, 5, 40]:
+        a = rng.normal(size=n)
+        b = np.abs(rng.normal(size=n + 1))
+        fr.append((a, b))
+    got = qcm.cf_eval(fr, z)
+    for k, (a, b) in enumerate(fr):
+        assert np.allclose(got[:, k], oracle.cf_eval(a, b, z), rtol=1e-13, atol=1e-14)
+    e = rng.normal(size=50)
+    V = rng.normal(size=(4, 50))
+    assert np.allclose(qcm.vdvh(e, V, z), oracle.vdvh(e, V, z), rtol=1e-13, atol=1e-14)
+    p, nfl = 3, 7
+    A = rng.normal(size=(nfl, p, p))
+    A = A + A.transpose(0, 2, 1)
+    B = np.triu(rng.normal(size=(nfl, p, p)))
+    Wm = np.triu(rng.normal(size=(p, p)))
+    assert np.allclose(qcm.mcf_eval(A, B, Wm, z), oracle.mcf_eval(A, B, Wm, z), rtol=1e-12, atol=1e-13)
+
+
+def _oracle_gf_band(oracle, wl, Nup, Ndw, z, start):
+    """oracle pipeline: GS + band Lanczos per (pm) + streamline + VDVH."""
+    M = oracle.Model(wl.L, wl.terms())
+    E0, gs, _, _ = M.lanczos_gs(Nup, Ndw, start)
+    gs = gs / np.linalg.norm(gs)
+    L = wl.L
+    orbs = list(range(wl.n_sites))
+    es, Vs = [], []
+    for pm in (-1, 1):
+        phi = np.array([oracle.create_or_destroy(L, pm, a, Nup, Ndw, gs) for a in orbs])
+        e, Q = M.band_lanczos(Nup + pm, Ndw, phi)
+        e = e - E0
+        if pm == -1:
+            e = -e
+        e, Q = oracle.qmatrix_streamline(e, Q)
+        es.append(e)
+        Vs.append(Q)
+    e = np.concatenate(es)
+    V = np.concatenate(Vs, axis=1)
+    return E0, oracle.vdvh(e, V, z)
+
+
+def test_green_function_band_lanczos_chain8(oracle):
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    qcm.set_global_parameterC("GF_method", "L")
+    E0, _ = qcm.ground_state_solve(lab)
+    z = W.matsubara(64, 50.0)
+    G = qcm.Green_function_grid(z, label=lab)
+    x = np.random.default_rng(7).normal(size=70 * 70)
+    E0r, Gr = _oracle_gf_band(oracle, wl, 4, 4, z, x)
+    assert abs(E0 - E0r) < 1e-10 * abs(E0r)
+    assert np.abs(G - Gr.transpose(0, 2, 1)).max() < 1e-8
+
+
+def test_green_function_cf_chain8_real_axis(oracle):
+    """C1: continued-fraction G(w) for 8 orbitals on 512 points w in [-8, 8], eta = 0.1."""
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    qcm.set_global_parameterC("GF_method", "F")
+    lab = _instance(wl)
+    E0, _ = qcm.ground_state_solve(lab)
+    z = W.real_axis(512, -8, 8, 0.1)
+    G = qcm.Green_function_grid(z, label=lab)
+    qcm.set_global_parameterC("GF_method", "L")
+    # oracle: same recurrences on the CPU
+    M = oracle.Model(wl.L, wl.terms())
+    x = np.random.default_rng(8).normal(size=70 * 70)
+    E0r, gs, _, _ = M.lanczos_gs(4, 4, x)
+    gs = gs / np.linalg.norm(gs)
+    assert abs(E0 - E0r) < 1e-10 * abs(E0r)
+    n = 8
+    Gr = np.zeros((len(z), n, n), complex)
+    F = {}
+    for o1 in range(n):
+        for o2 in range(o1 + 1):
+            tot = np.zeros(len(z), complex)
+            for pm in (1, -1):
+                psi = oracle.create_or_destroy(8, pm, o1, 4, 4, gs)
+                if o2 != o1:
+                    psi = psi + oracle.create_or_destroy(8, pm, o2, 4, 4, gs)
+                nrm = psi @ psi
+                a, b = M.lanczos_green(4 + pm, 4, psi / np.sqrt(nrm))
+                a, b = oracle.cf_transform(a, b, E0r, nrm, pm == 1)
+                tot += oracle.cf_eval(a, b, z)
+            F[(o1, o2)] = tot
+    for o in range(n):
+        Gr[:, o, o] = F[(o, o)]
+    for o1 in range(n):
+        for o2 in range(o1):
+            v = 0.5 * (F[(o1, o2)] - F[(o1, o1)] - F[(o2, o2)])
+            Gr[:, o1, o2] = v
+            Gr[:, o2, o1] = v
+    assert np.abs(G - Gr).max() < 1e-8
+
+
+def test_davidson_matches_dense():
+    wl = W.hubbard_chain(6, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    H = qcm.Hamiltonian(lab, "R0:N6:S0", "G")
+    ev, vec, nit = H.davidson(4)
+    Hd = dense_hamiltonian(wl.L, wl.terms(), 3, 3)
+    ref = np.linalg.eigvalsh(Hd)[:4]
+    assert np.allclose(ev, ref, atol=1e-8)
+
+
+def test_block_lanczos_mcf_matches_oracle(oracle):
+    wl = W.hubbard_chain(6, U=4.0, mu=2.0)
+    lab = _instance(wl, "R0:N7:S1")
+    H = qcm.Hamiltonian(lab, "R0:N7:S1", "G")
+    rng = np.random.default_rng(9)
+    q = np.linalg.qr(rng.normal(size=(H.dim, 3)))[0].T.copy()
+    A, B = H.block_lanczos(q, 30)
+    Ar, Br = oracle.Model(wl.L, wl.terms()).block_lanczos(4, 3, q, 30)
+    k = min(len(A), len(Ar), 8)
+    assert np.allclose(A[:k], Ar[:k], atol=1e-9)
+    assert np.allclose(np.abs(B[:k]), np.abs(Br[:k]), atol=1e-9)
+
+
+def test_mcf_green_function_matches_band(oracle):
+    wl = W.hubbard_chain(6, U=4.0, mu=2.0)
+    z = W.matsubara(32, 30.0)
+    qcm.set_global_parameterC("GF_method", "L")
+    lab1 = _instance(wl)
+    G1 = qcm.Green_function_grid(z, label=lab1)
+    qcm.set_global_parameterC("GF_method", "M")
+    qcm.set_global_parameterC("accur_band_lanczos", 0.0)
+    lab2 = _instance(wl)
+    G2 = qcm.Green_function_grid(z, label=lab2)
+    qcm.set_global_parameterC("accur_band_lanczos", 1e-12)
+    qcm.set_global_parameterC("GF_method", "L")
+    assert np.abs(G1 - G2).max() < 1e-7
