@@ -110,9 +110,13 @@ int ed_hamiltonian_diag(int64_t handle, double *d);
 int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v);
 /* time nrep device applications y = H x (HIP events on the library stream) */
 int ed_hamiltonian_time(int64_t handle, int nrep, double *ms_per_apply, int variant);
-/* kernel selection knob for the factorized format: 0 auto, 1 fused row kernel,
-   2 two-pass (row up-part + column-slab down-part) */
+/* kernel selection knob for the factorized format: 0 auto (autotuned), 1 fused
+   row kernel, 2 two-pass (row up-part + column-slab down-part) */
 int ed_hamiltonian_set_variant(int64_t handle, int variant);
+/* autotuner of the factorized schedule (no reference counterpart: the reference
+   has one serial loop): times fused and two-pass x slab width {16,32} x
+   streaming stores {off,on}; ms[k] per candidate in that order, best = chosen index */
+int ed_hamiltonian_autotune(int64_t handle, int cap, double *ms, int *n, int *best);
 
 /* ---- bases (src_ed/ED_basis.cpp:115-179) and sectors (src_ed/sector.hpp) ---- */
 int ed_half_basis(int L, int N, uint32_t *out, int64_t cap, int64_t *dim);

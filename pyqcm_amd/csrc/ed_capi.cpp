@@ -312,6 +312,14 @@ int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int3
   });
 }
 
+int ed_hamiltonian_autotune(int64_t handle, int cap, double *ms, int *n, int *best) {
+  return guard([&] {
+    std::vector<double> t = hamiltonian_autotune(get_h(handle), *best);
+    *n = (int)t.size();
+    for (int k = 0; k < *n && k < cap; ++k) ms[k] = t[k];
+  });
+}
+
 int ed_hamiltonian_set_variant(int64_t handle, int variant) {
   return guard([&] { get_h(handle).variant = variant; });
 }

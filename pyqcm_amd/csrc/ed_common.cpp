@@ -44,6 +44,7 @@ Params::Params() {
   // device-side knobs (no reference counterpart)
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
+  b["sell16"] = false;              // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
   d["lanczos_basis_budget_GB"] = 96.0;
 }
 

@@ -141,6 +141,243 @@ __global__ __launch_bounds__(BS) void k_hv_rows(FusedParams P) {
   }
 }
 
+constexpr int kMaxNb = 128;  // K_dn entries of one row staged in LDS
+
+// Row kernel for one row of X per block iteration (rows up to ~15K states).
+// Every thread owns KMAX elements of the row (u = tid + k * BS) and keeps their
+// accumulators in registers.  The up part gathers from the LDS-staged row; the
+// down part runs neighbour-major: for each K_dn entry (c, v) of the row, staged
+// in LDS, every thread issues KMAX independent coalesced loads of row c, so
+// each wave keeps many HBM/MALL requests in flight instead of one dependent
+// chain per element.
+template <int BS, int KMAX, bool DOWN>
+__global__ __launch_bounds__(BS) void k_hv_rowk(FusedParams P) {
+  extern __shared__ double smem[];
+  __shared__ double Ush[32];
+  __shared__ double vtu[kVtabMax];
+  __shared__ double vtd[kVtabMax];
+  __shared__ double red[BS / kWave];
+  __shared__ uint32_t ncol[kMaxNb];
+  __shared__ double nval[kMaxNb];
+  const FactArgs &a = P.a;
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  if (threadIdx.x < 32) Ush[threadIdx.x] = a.Uc[threadIdx.x];
+  for (int i = threadIdx.x; i < P.ntab_up; i += BS) vtu[i] = a.up.val[i];
+  for (int i = threadIdx.x; i < P.ntab_dn; i += BS) vtd[i] = a.dn.val[i];
+  const double *vtab_u = P.ntab_up ? vtu : a.up.val;
+  const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
+  double dot = 0.0;
+  for (uint32_t d = blockIdx.x; d < dd; d += gridDim.x) {
+    const size_t base = (size_t)d * du;
+    const uint32_t g0 = a.dn.ptr[d], nnb = a.dn.ptr[d + 1] - g0;
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < du; f += BS) smem[f] = P.x[base + f];
+    if (DOWN) {
+      for (uint32_t j = threadIdx.x; j < min(nnb, (uint32_t)kMaxNb); j += BS) {
+        uint32_t c;
+        const double v = half_entry(a.dn, vtab_d, g0 + j, c);
+        ncol[j] = c;
+        nval[j] = v;
+      }
+    }
+    __syncthreads();
+    const uint32_t sd = (a.diag_mode != 0) ? a.dn.states[d] : 0u;
+    double acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t u = threadIdx.x + k * BS;
+      acc[k] = 0.0;
+      if (u < du) {
+        double s = diag_value(a, Ush, base + u, u, d, sd) * smem[u];
+        const uint32_t e1 = a.up.ptr[u + 1];
+        uint32_t e = a.up.ptr[u];
+        for (; e + 4 <= e1; e += 4) {
+          uint32_t c0, c1, c2, c3;
+          const double v0 = half_entry(a.up, vtab_u, e, c0);
+          const double v1 = half_entry(a.up, vtab_u, e + 1, c1);
+          const double v2 = half_entry(a.up, vtab_u, e + 2, c2);
+          const double v3 = half_entry(a.up, vtab_u, e + 3, c3);
+          s += v0 * smem[c0] + v1 * smem[c1] + v2 * smem[c2] + v3 * smem[c3];
+        }
+        for (; e < e1; ++e) {
+          uint32_t c;
+          const double v = half_entry(a.up, vtab_u, e, c);
+          s += v * smem[c];
+        }
+        acc[k] = s;
+      }
+    }
+    if (DOWN) {
+      uint32_t j = 0;
+      for (; j + 2 <= nnb; j += 2) {
+        uint32_t c0, c1;
+        double v0, v1;
+        if (j + 1 < (uint32_t)kMaxNb) {
+          c0 = ncol[j]; v0 = nval[j]; c1 = ncol[j + 1]; v1 = nval[j + 1];
+        } else {
+          v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
+          v1 = half_entry(a.dn, vtab_d, g0 + j + 1, c1);
+        }
+        const double *x0 = P.x + (size_t)c0 * du;
+        const double *x1 = P.x + (size_t)c1 * du;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+          const uint32_t u = threadIdx.x + k * BS;
+          if (u < du) acc[k] += v0 * x0[u] + v1 * x1[u];
+        }
+      }
+      for (; j < nnb; ++j) {
+        uint32_t c0;
+        double v0;
+        if (j < (uint32_t)kMaxNb) { c0 = ncol[j]; v0 = nval[j]; }
+        else v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
+        const double *x0 = P.x + (size_t)c0 * du;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+          const uint32_t u = threadIdx.x + k * BS;
+          if (u < du) acc[k] += v0 * x0[u];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t u = threadIdx.x + k * BS;
+      if (u < du) {
+        const size_t I = base + u;
+        const double out = (P.beta != 0.0) ? P.ax * acc[k] + P.beta * P.y[I] : P.ax * acc[k];
+        P.y[I] = out;
+        if (DOWN) dot += out * smem[u];
+      }
+    }
+  }
+  if (DOWN && P.part) {
+    const double s = block_sum<BS>(dot, red);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = s;
+  }
+}
+
+// TM: 0 packed entry + value table, 1 explicit values, 2 16-bit sign-only
+template <int TM>
+__device__ __forceinline__ double sell_entry(const DevHalf &h, const double *vtab, size_t e, uint32_t &col) {
+  if (TM == 2) {
+    const uint32_t w = h.sell16[e];
+    col = w & 0x7fffu;
+    return (w & 0x8000u) ? -h.t0 : h.t0;
+  }
+  const uint32_t w = h.sell_ent[e];
+  if (TM == 0) {
+    col = w & ((1u << h.colbits) - 1u);
+    return vtab[w >> h.colbits];
+  }
+  col = w;
+  return h.sell_val[e];
+}
+
+// Row kernel with the up-spin operator in sliced-ELL form: each wave takes
+// slices of 64 consecutive up states; entry j of the slice is one coalesced
+// 256-byte read, the gathers hit the LDS-staged row, and the loop bounds are
+// wave-uniform.  The down part (DOWN) reads the K_dn neighbour rows (staged
+// list) for the same 64 states with four independent loads in flight.
+template <int BS, bool DOWN, int TM>
+__global__ __launch_bounds__(BS) void k_hv_rows_sell(FusedParams P) {
+  extern __shared__ double smem[];
+  __shared__ double Ush[32];
+  __shared__ double vtu[kVtabMax];
+  __shared__ double vtd[kVtabMax];
+  __shared__ double red[BS / kWave];
+  __shared__ uint32_t ncol[kMaxNb];
+  __shared__ double nval[kMaxNb];
+  const FactArgs &a = P.a;
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  if (threadIdx.x < 32) Ush[threadIdx.x] = a.Uc[threadIdx.x];
+  for (int i = threadIdx.x; i < P.ntab_up; i += BS) vtu[i] = a.up.val[i];
+  for (int i = threadIdx.x; i < P.ntab_dn; i += BS) vtd[i] = a.dn.val[i];
+  const double *vtab_u = P.ntab_up ? vtu : a.up.val;
+  const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
+  const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  constexpr uint32_t NW = BS / kWave;
+  double dot = 0.0;
+  for (uint32_t d = blockIdx.x; d < dd; d += gridDim.x) {
+    const size_t base = (size_t)d * du;
+    const uint32_t g0 = a.dn.ptr[d], nnb = a.dn.ptr[d + 1] - g0;
+    const uint32_t nst = min(nnb, (uint32_t)kMaxNb);
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < du; f += BS) smem[f] = P.x[base + f];
+    if (threadIdx.x == 0) smem[du] = 0.0;  // target of the sliced-ELL padding
+    if (DOWN) {
+      for (uint32_t j = threadIdx.x; j < nst; j += BS) {
+        uint32_t c;
+        const double v = half_entry(a.dn, vtab_d, g0 + j, c);
+        ncol[j] = c;
+        nval[j] = v;
+      }
+    }
+    __syncthreads();
+    const uint32_t sd = (a.diag_mode != 0) ? a.dn.states[d] : 0u;
+    for (uint32_t s = wave; s < a.up.nslice; s += NW) {
+      const uint32_t u = s * kWave + lane;
+      const bool ok = u < du;
+      const uint32_t uu = ok ? u : du - 1;
+      const size_t o0 = a.up.sell_off[s], o1 = a.up.sell_off[s + 1];
+      double acc = diag_value(a, Ush, base + uu, uu, d, sd) * smem[uu];
+      size_t e = o0 + lane;
+      for (; e + 3 * kWave < o1; e += 4 * kWave) {
+        uint32_t c0, c1, c2, c3;
+        const double v0 = sell_entry<TM>(a.up, vtab_u, e, c0);
+        const double v1 = sell_entry<TM>(a.up, vtab_u, e + kWave, c1);
+        const double v2 = sell_entry<TM>(a.up, vtab_u, e + 2 * kWave, c2);
+        const double v3 = sell_entry<TM>(a.up, vtab_u, e + 3 * kWave, c3);
+        acc += v0 * smem[c0] + v1 * smem[c1] + v2 * smem[c2] + v3 * smem[c3];
+      }
+      for (; e < o1; e += kWave) {
+        uint32_t c;
+        const double v = sell_entry<TM>(a.up, vtab_u, e, c);
+        acc += v * smem[c];
+      }
+      if (DOWN) {
+        uint32_t j = 0;
+        for (; j + 4 <= nnb; j += 4) {
+          uint32_t c0, c1, c2, c3;
+          double v0, v1, v2, v3;
+          if (j + 3 < (uint32_t)kMaxNb) {
+            c0 = ncol[j]; v0 = nval[j];
+            c1 = ncol[j + 1]; v1 = nval[j + 1];
+            c2 = ncol[j + 2]; v2 = nval[j + 2];
+            c3 = ncol[j + 3]; v3 = nval[j + 3];
+          } else {
+            v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
+            v1 = half_entry(a.dn, vtab_d, g0 + j + 1, c1);
+            v2 = half_entry(a.dn, vtab_d, g0 + j + 2, c2);
+            v3 = half_entry(a.dn, vtab_d, g0 + j + 3, c3);
+          }
+          const double x0 = P.x[(size_t)c0 * du + uu];
+          const double x1 = P.x[(size_t)c1 * du + uu];
+          const double x2 = P.x[(size_t)c2 * du + uu];
+          const double x3 = P.x[(size_t)c3 * du + uu];
+          acc += v0 * x0 + v1 * x1 + v2 * x2 + v3 * x3;
+        }
+        for (; j < nnb; ++j) {
+          uint32_t c0;
+          double v0;
+          if (j < (uint32_t)kMaxNb) { c0 = ncol[j]; v0 = nval[j]; }
+          else v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
+          acc += v0 * P.x[(size_t)c0 * du + uu];
+        }
+      }
+      if (ok) {
+        const size_t I = base + u;
+        const double out = (P.beta != 0.0) ? P.ax * acc + P.beta * P.y[I] : P.ax * acc;
+        P.y[I] = out;
+        if (DOWN) dot += out * smem[u];
+      }
+    }
+  }
+  if (DOWN && P.part) {
+    const double s = block_sum<BS>(dot, red);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = s;
+  }
+}
+
 struct SlabParams {
   FactArgs a;
   const double *x;
@@ -157,38 +394,105 @@ struct SlabParams {
 // part re-reads ~nnz(K_dn)/row times; blocks are mapped so that all chunks of a
 // slab run on the same XCD (round-robin dispatch: block b -> XCD b % nxcd), which
 // keeps the slab resident in that XCD's L2.
-template <int SW, int RL>
+constexpr int kSlabEnt = 2048;  // K_dn entries of one row chunk staged in LDS
+constexpr int kSlabRows = 64;   // rows per chunk
+
+// Second pass of the two-pass variant: y += ax (K_dn x 1) x on column slabs.
+// One block per (slab, 64-row chunk), numbered so that block b runs on XCD
+// b % nxcd (round-robin dispatch) and each XCD receives all chunks of one slab
+// before the next: the slab (dim_dn x SW doubles) stays resident in that XCD's
+// L2 while each x element is re-read ~nnz(K_dn) times.  The chunk's K_dn rows
+// are staged in LDS; neighbours are loaded four at a time.  NT streams the
+// once-touched y and own-x accesses past L2.
+template <int SW, int RL, bool NT>
 __global__ __launch_bounds__(SW *RL) void k_hv_down_slab(SlabParams P) {
   __shared__ double vtd[kVtabMax];
   __shared__ double red[(SW * RL) / kWave];
+  __shared__ uint32_t lptr[kSlabRows + 1];
+  __shared__ uint32_t lcol[kSlabEnt];
+  __shared__ double lval[kSlabEnt];
   const FactArgs &a = P.a;
   const uint32_t du = a.up.dim, dd = a.dn.dim;
   for (int i = threadIdx.x; i < P.ntab_dn; i += SW * RL) vtd[i] = a.dn.val[i];
-  __syncthreads();
   const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
   const uint32_t b = blockIdx.x;
   const uint32_t xcd = b % P.nxcd, k = b / P.nxcd;
   const uint32_t slab = xcd + P.nxcd * (k / P.nchunk);
   const uint32_t chunk = k % P.nchunk;
   double dot = 0.0;
-  if (slab < P.nslab) {
-    const uint32_t u = slab * SW + (threadIdx.x % SW);
-    const uint32_t rl = threadIdx.x / SW;
-    const uint32_t r0 = chunk * P.chunk_rows;
-    const uint32_t r1 = min(dd, r0 + P.chunk_rows);
-    if (u < du) {
+  const uint32_t r0 = min(dd, chunk * kSlabRows);
+  const uint32_t r1 = min(dd, r0 + kSlabRows);
+  const uint32_t g0 = a.dn.ptr[r0], g1 = a.dn.ptr[r1];
+  const bool staged = (g1 - g0) <= (uint32_t)kSlabEnt;
+  if (staged) {
+    for (uint32_t i = threadIdx.x; i <= r1 - r0; i += SW * RL) lptr[i] = a.dn.ptr[r0 + i] - g0;
+    for (uint32_t e = threadIdx.x; e < g1 - g0; e += SW * RL) {
+      uint32_t c;
+      lval[e] = half_entry(a.dn, vtab_d, g0 + e, c);
+      lcol[e] = c;
+    }
+  }
+  __syncthreads();
+  const uint32_t u = slab * SW + (threadIdx.x % SW);
+  const uint32_t rl = threadIdx.x / SW;
+  if (slab < P.nslab && u < du) {
+    auto finish = [&](uint32_t d, double acc) {
+      const size_t I = (size_t)d * du + u;
+      double out, xo;
+      if (NT) {
+        out = __builtin_nontemporal_load(P.y + I) + P.ax * acc;
+        __builtin_nontemporal_store(out, P.y + I);
+        xo = __builtin_nontemporal_load(P.x + I);
+      } else {
+        out = P.y[I] + P.ax * acc;
+        P.y[I] = out;
+        xo = P.x[I];
+      }
+      dot += out * xo;
+    };
+    if (staged) {
+      // two rows per iteration: up to eight independent loads in flight
+      for (uint32_t d = r0 + rl; d < r1; d += 2 * RL) {
+        const uint32_t dB = d + RL;
+        const bool hasB = dB < r1;
+        uint32_t eA = lptr[d - r0], eA1 = lptr[d - r0 + 1];
+        uint32_t eB = hasB ? lptr[dB - r0] : 0u, eB1 = hasB ? lptr[dB - r0 + 1] : 0u;
+        double accA = 0.0, accB = 0.0;
+        while (eA + 4 <= eA1 && eB + 4 <= eB1) {
+          const double a0 = P.x[(size_t)lcol[eA] * du + u], a1 = P.x[(size_t)lcol[eA + 1] * du + u];
+          const double a2 = P.x[(size_t)lcol[eA + 2] * du + u], a3 = P.x[(size_t)lcol[eA + 3] * du + u];
+          const double b0 = P.x[(size_t)lcol[eB] * du + u], b1 = P.x[(size_t)lcol[eB + 1] * du + u];
+          const double b2 = P.x[(size_t)lcol[eB + 2] * du + u], b3 = P.x[(size_t)lcol[eB + 3] * du + u];
+          accA += lval[eA] * a0 + lval[eA + 1] * a1 + lval[eA + 2] * a2 + lval[eA + 3] * a3;
+          accB += lval[eB] * b0 + lval[eB + 1] * b1 + lval[eB + 2] * b2 + lval[eB + 3] * b3;
+          eA += 4;
+          eB += 4;
+        }
+        for (; eA + 4 <= eA1; eA += 4) {
+          const double a0 = P.x[(size_t)lcol[eA] * du + u], a1 = P.x[(size_t)lcol[eA + 1] * du + u];
+          const double a2 = P.x[(size_t)lcol[eA + 2] * du + u], a3 = P.x[(size_t)lcol[eA + 3] * du + u];
+          accA += lval[eA] * a0 + lval[eA + 1] * a1 + lval[eA + 2] * a2 + lval[eA + 3] * a3;
+        }
+        for (; eA < eA1; ++eA) accA += lval[eA] * P.x[(size_t)lcol[eA] * du + u];
+        for (; eB + 4 <= eB1; eB += 4) {
+          const double b0 = P.x[(size_t)lcol[eB] * du + u], b1 = P.x[(size_t)lcol[eB + 1] * du + u];
+          const double b2 = P.x[(size_t)lcol[eB + 2] * du + u], b3 = P.x[(size_t)lcol[eB + 3] * du + u];
+          accB += lval[eB] * b0 + lval[eB + 1] * b1 + lval[eB + 2] * b2 + lval[eB + 3] * b3;
+        }
+        for (; eB < eB1; ++eB) accB += lval[eB] * P.x[(size_t)lcol[eB] * du + u];
+        finish(d, accA);
+        if (hasB) finish(dB, accB);
+      }
+    } else {
       for (uint32_t d = r0 + rl; d < r1; d += RL) {
         double acc = 0.0;
-        const uint32_t g1 = a.dn.ptr[d + 1];
-        for (uint32_t e = a.dn.ptr[d]; e < g1; ++e) {
+        const uint32_t e1 = a.dn.ptr[d + 1];
+        for (uint32_t e = a.dn.ptr[d]; e < e1; ++e) {
           uint32_t c;
           const double v = half_entry(a.dn, vtab_d, e, c);
           acc += v * P.x[(size_t)c * du + u];
         }
-        const size_t I = (size_t)d * du + u;
-        const double out = P.y[I] + P.ax * acc;
-        P.y[I] = out;
-        dot += out * P.x[I];
+        finish(d, acc);
       }
     }
   }
@@ -196,6 +500,16 @@ __global__ __launch_bounds__(SW *RL) void k_hv_down_slab(SlabParams P) {
     const double s = block_sum<SW * RL>(dot, red);
     if (threadIdx.x == 0) P.part[blockIdx.x] = s;
   }
+}
+
+// partial sums -> kPartialsCap partials (deterministic, fixed order)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fold_partials(const double *in, size_t n, double *out) {
+  __shared__ double red[BS / kWave];
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) s += in[i];
+  s = block_sum<BS>(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 template <int BS>
@@ -581,7 +895,7 @@ size_t lds_row_budget() {
 }
 
 uint32_t rows_per_block(uint32_t du) {
-  if ((size_t)du * 8 > lds_row_budget()) return 0;  // row does not fit: no LDS staging
+  if ((size_t)(du + 1) * 8 > lds_row_budget()) return 0;  // row does not fit: no LDS staging
   uint32_t R = 1;
   while (R * du < 512 && (size_t)(2 * R) * du * 8 <= 32 * 1024) R *= 2;
   return R;
@@ -621,15 +935,15 @@ static int fused_grid(const FactArgs &a, uint32_t R) {
 
 int hv_factorized_max_partials(const FactArgs &a, int variant) { return kPartialsCap; }
 
-int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
-                  hipStream_t s) {
+int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
+                      const SlabConfig &sc, hipStream_t s) {
   const uint32_t du = a.up.dim, dd = a.dn.dim;
   if ((size_t)du * dd == 0) return 0;
   uint32_t R = rows_per_block(du);
   const bool stage = R > 0;
   if (!stage) R = 1;
   if (variant == HV_AUTO) variant = ((size_t)du * dd >= ((size_t)1 << 26)) ? HV_TWO_PASS : HV_FUSED;
-  const size_t lds = stage ? (size_t)R * du * sizeof(double) : 0;
+  const size_t lds = stage ? ((size_t)R * du + 1) * sizeof(double) : 0;
   constexpr int BS = 1024;
   const bool down_here = variant == HV_FUSED;
   {
@@ -641,7 +955,42 @@ int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, doub
     if (STG) allow_lds(kern, lds);                                              \
     hipLaunchKernelGGL(kern, dim3(g), dim3(BSZ), STG ? lds : 0, s, P);          \
   } while (0)
-    if (down_here) {
+#define ED_LAUNCH_ROWK(KM, DOWN)                                               \
+  do {                                                                          \
+    auto kern = k_hv_rowk<BS, KM, DOWN>;                                        \
+    allow_lds(kern, lds);                                                       \
+    hipLaunchKernelGGL(kern, dim3(g), dim3(BS), lds, s, P);                     \
+  } while (0)
+    const uint32_t kneed = (du + BS - 1) / BS;
+    if (stage && R == 1 && a.up.nslice > 0) {
+      const int tm = a.up.mode16 ? 2 : (a.up.packed ? 0 : 1);
+#define ED_SELL(DOWN, TMV)                                                    \
+  do {                                                                        \
+    auto kern = k_hv_rows_sell<BS, DOWN, TMV>;                                \
+    allow_lds(kern, lds);                                                     \
+    hipLaunchKernelGGL(kern, dim3(g), dim3(BS), lds, s, P);                   \
+  } while (0)
+      if (down_here) {
+        if (tm == 2) ED_SELL(true, 2); else if (tm == 1) ED_SELL(true, 1); else ED_SELL(true, 0);
+      } else {
+        if (tm == 2) ED_SELL(false, 2); else if (tm == 1) ED_SELL(false, 1); else ED_SELL(false, 0);
+      }
+#undef ED_SELL
+    } else if (stage && R == 1 && kneed <= 16) {
+      if (down_here) {
+        if (kneed <= 1) ED_LAUNCH_ROWK(1, true);
+        else if (kneed <= 2) ED_LAUNCH_ROWK(2, true);
+        else if (kneed <= 4) ED_LAUNCH_ROWK(4, true);
+        else if (kneed <= 8) ED_LAUNCH_ROWK(8, true);
+        else ED_LAUNCH_ROWK(16, true);
+      } else {
+        if (kneed <= 1) ED_LAUNCH_ROWK(1, false);
+        else if (kneed <= 2) ED_LAUNCH_ROWK(2, false);
+        else if (kneed <= 4) ED_LAUNCH_ROWK(4, false);
+        else if (kneed <= 8) ED_LAUNCH_ROWK(8, false);
+        else ED_LAUNCH_ROWK(16, false);
+      }
+    } else if (down_here) {
       if (!stage) ED_LAUNCH_ROWS(BS, true, true, false);
       else if (R == 1) ED_LAUNCH_ROWS(BS, true, true, true);
       else ED_LAUNCH_ROWS(256, false, true, true);
@@ -651,35 +1000,45 @@ int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, doub
       else ED_LAUNCH_ROWS(256, false, false, true);
     }
 #undef ED_LAUNCH_ROWS
+#undef ED_LAUNCH_ROWK
     ED_HIP(hipGetLastError());
     if (down_here) return g;
   }
   // second pass of the two-pass variant: column slabs (down part)
-  constexpr int SW = 32, RL = 8;
   SlabParams Q;
   Q.a = a;
   Q.x = x;
   Q.y = y;
   Q.ax = ax;
-  Q.nxcd = 8;
+  Q.nxcd = (sc.nxcd == 1) ? 1 : 8;
+  const int SW = (sc.sw == 16 || sc.sw == 64) ? sc.sw : 32;
   Q.nslab = (du + SW - 1) / SW;
-  uint32_t ch = 64;
-  auto nblocks = [&](uint32_t c) {
-    const uint32_t nch = (dd + c - 1) / c;
-    const uint32_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
-    return (size_t)Q.nxcd * slabs_per_xcd * nch;
-  };
-  while (nblocks(ch) > (size_t)kPartialsCap && ch < dd) ch *= 2;
-  if (nblocks(ch) > (size_t)kPartialsCap)
-    fail(ED_ERR_UNSUPPORTED, "two-pass H.v: too many column slabs for the partial-sum buffer");
-  Q.chunk_rows = ch;
-  Q.nchunk = (dd + ch - 1) / ch;
-  Q.part = part;
+  Q.chunk_rows = kSlabRows;
+  Q.nchunk = (dd + kSlabRows - 1) / kSlabRows;
   Q.ntab_dn = (a.dn.packed && a.dn.nval <= (uint32_t)kVtabMax) ? (int)a.dn.nval : 0;
-  const int g = (int)nblocks(ch);
-  hipLaunchKernelGGL((k_hv_down_slab<SW, RL>), dim3(g), dim3(SW * RL), 0, s, Q);
+  const size_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
+  const size_t nb = (size_t)Q.nxcd * slabs_per_xcd * Q.nchunk;
+  // per-block partials go to a scratch array folded back to <= kPartialsCap
+  static thread_local DevBuf<double> big;
+  if (part && nb > (size_t)kPartialsCap && big.n < nb) big.alloc(nb);
+  Q.part = part ? (nb > (size_t)kPartialsCap ? big.p : part) : nullptr;
+#define ED_SLAB(SWV, NTV) hipLaunchKernelGGL((k_hv_down_slab<SWV, 256 / SWV, NTV>), dim3((unsigned)nb), dim3(256), 0, s, Q)
+  if (SW == 16) { if (sc.nt) ED_SLAB(16, true); else ED_SLAB(16, false); }
+  else if (SW == 64) { if (sc.nt) ED_SLAB(64, true); else ED_SLAB(64, false); }
+  else { if (sc.nt) ED_SLAB(32, true); else ED_SLAB(32, false); }
+#undef ED_SLAB
   ED_HIP(hipGetLastError());
-  return g;
+  if (part && nb > (size_t)kPartialsCap) {
+    hipLaunchKernelGGL((k_fold_partials<256>), dim3(kPartialsCap), dim3(256), 0, s, (const double *)big.p, nb, part);
+    ED_HIP(hipGetLastError());
+    return kPartialsCap;
+  }
+  return (int)nb;
+}
+
+int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
+                  hipStream_t s) {
+  return hv_factorized_cfg(a, x, y, ax, beta, part, variant, SlabConfig(), s);
 }
 
 void fact_full_diag(const FactArgs &a, double *d, hipStream_t s) {

@@ -22,6 +22,25 @@ struct DevHalf {
   int packed = 1;
   int colbits = 0;
   uint32_t nval = 0;               // size of the value table (packed == 1)
+  // the same operator as a sliced ELL table: slices of 64 consecutive states,
+  // each padded to its longest row and stored entry-major (entry j of the 64
+  // rows contiguous), so a wave reads one coalesced 256-byte line per entry
+  uint32_t nslice = 0;
+  const uint32_t *sell_off = nullptr;  // nslice + 1 offsets (entries)
+  const uint32_t *sell_ent = nullptr;  // packed like ent (padding: own column, zero value)
+  const double *sell_val = nullptr;    // explicit values when packed == 0
+  // single-magnitude operators (every value is +-t0, dim < 2^15): 16-bit
+  // entries col | sign << 15; padding points at the zero slot after the row
+  int mode16 = 0;
+  double t0 = 0.0;
+  const uint16_t *sell16 = nullptr;
+};
+
+// slab configuration of the two-pass down kernel (autotuned per plan)
+struct SlabConfig {
+  int sw = 32;        // columns per slab (16, 32, 64)
+  int nt = 0;         // non-temporal y / own-x accesses (keep L2 for the slab)
+  int nxcd = 8;       // 8: slabs partitioned over XCDs (block b -> XCD b % 8); 1: one shared sweep
 };
 
 // Factorized H on X[i_dn][i_up]: H = D + (1 x K_up) + (K_dn x 1).
@@ -33,6 +52,8 @@ struct FactArgs {
 };
 
 enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
+int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
+                      int variant, const SlabConfig &sc, hipStream_t s);
 
 // y = ax H x + beta y (y not read when beta == 0); when dot_partials != nullptr,
 // per-block partial sums of <y_new, x> are written there (nblocks returned).

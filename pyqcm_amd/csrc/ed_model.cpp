@@ -319,12 +319,77 @@ static HalfHost build_half(const TermList &T, int spin, const HalfBasis &B) {
   return H;
 }
 
+// sliced-ELL copy of a half table (slices of 64 rows, entry-major).  Padding
+// entries point at column dim (a zero slot the kernels place after the row).
+static void build_sell(const HalfHost &H, std::vector<uint32_t> &off, std::vector<uint32_t> &ent,
+                       std::vector<double> &val, std::vector<uint16_t> &e16, double &t0) {
+  const uint32_t n = H.dim, ns = (n + 63) / 64;
+  off.assign((size_t)ns + 1, 0);
+  for (uint32_t s = 0; s < ns; ++s) {
+    uint32_t w = 0;
+    for (uint32_t r = s * 64; r < std::min(n, s * 64 + 64); ++r) w = std::max(w, H.ptr[r + 1] - H.ptr[r]);
+    off[s + 1] = off[s] + 64 * w;
+  }
+  // single magnitude?
+  bool single = n < 32767;
+  t0 = 0.0;
+  for (size_t e = 0; e < H.ent.size() && single; ++e) {
+    const double v = H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e];
+    if (t0 == 0.0) t0 = std::fabs(v);
+    if (std::fabs(v) != t0) single = false;
+  }
+  if (t0 == 0.0) single = false;
+  const size_t tot = std::max<size_t>(off[ns], 1);
+  ent.assign(single ? 1 : tot, 0u);
+  val.assign((single || H.packed) ? 1 : tot, 0.0);
+  e16.assign(single ? tot : 1, 0u);
+  const uint32_t pad_packed = H.packed ? n : n;  // column dim, value irrelevant (x slot is 0)
+  for (uint32_t s = 0; s < ns; ++s) {
+    const uint32_t w = (off[s + 1] - off[s]) / 64;
+    for (uint32_t lane = 0; lane < 64; ++lane) {
+      const uint32_t r = s * 64 + lane;
+      for (uint32_t j = 0; j < w; ++j) {
+        const size_t dst = (size_t)off[s] + (size_t)j * 64 + lane;
+        const bool real = r < n && j < H.ptr[r + 1] - H.ptr[r];
+        if (single) {
+          if (real) {
+            const size_t e = H.ptr[r] + j;
+            const uint32_t col = H.packed ? (H.ent[e] & ((1u << H.colbits) - 1u)) : H.ent[e];
+            const double v = H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e];
+            e16[dst] = (uint16_t)(col | (v < 0.0 ? 0x8000u : 0u));
+          } else {
+            e16[dst] = (uint16_t)n;
+          }
+        } else if (real) {
+          ent[dst] = H.ent[H.ptr[r] + j];
+          if (!H.packed) val[dst] = H.val[H.ptr[r] + j];
+        } else {
+          ent[dst] = pad_packed;  // col = dim, value id 0 (any value times the zero slot)
+          if (!H.packed) val[dst] = 0.0;
+        }
+      }
+    }
+  }
+  if (!single) t0 = 0.0;
+}
+
 struct DevHalfStore {
-  DevBuf<uint32_t> ptr, ent, states;
-  DevBuf<double> val, diag;
+  DevBuf<uint32_t> ptr, ent, states, soff, sent;
+  DevBuf<double> val, diag, sval;
+  DevBuf<uint16_t> s16;
   DevHalf view;
   HalfHost host;
-  void upload(const HalfHost &H, const HalfBasis &B) {
+  void upload(const HalfHost &Hin, const HalfBasis &B) {
+    HalfHost H = Hin;
+    std::vector<uint32_t> so, se;
+    std::vector<double> sv;
+    std::vector<uint16_t> e16;
+    double t0 = 0.0;
+    build_sell(H, so, se, sv, e16, t0);
+    soff.upload(so);
+    sent.upload(se);
+    sval.upload(sv);
+    s16.upload(e16);
     host = H;
     ptr.upload(H.ptr);
     ent.upload(H.ent.empty() ? std::vector<uint32_t>{0u} : H.ent);
@@ -340,6 +405,13 @@ struct DevHalfStore {
     view.packed = H.packed;
     view.colbits = H.colbits;
     view.nval = H.nval;
+    view.nslice = (uint32_t)so.size() - 1;
+    view.sell_off = soff.p;
+    view.sell_ent = sent.p;
+    view.sell_val = sval.p;
+    view.mode16 = (t0 != 0.0 && params().getb("sell16")) ? 1 : 0;
+    view.t0 = t0;
+    view.sell16 = s16.p;
   }
 };
 
@@ -391,6 +463,11 @@ struct HamFactorized : Hamiltonian {
   DevBuf<double> dg;
   FactArgs args;
   std::shared_ptr<const HalfBasis> BU, BD;
+  // autotuned schedule (variant 0): kernel variant + slab configuration
+  bool tuned = false;
+  int best_variant = HV_FUSED;
+  SlabConfig best_sc;
+  std::vector<double> tune_ms;
 
   HamFactorized(const TermList &T, const Sector &s) {
     sec = s;
@@ -427,8 +504,59 @@ struct HamFactorized : Hamiltonian {
     partials.alloc(partials_cap());
     variant = (int)params().geti("hv_variant");
   }
+  // candidate schedules, in the order reported by ed_hamiltonian_autotune
+  static std::vector<std::pair<int, SlabConfig>> candidates() {
+    std::vector<std::pair<int, SlabConfig>> c;
+    c.push_back({HV_FUSED, SlabConfig()});
+    for (int sw : {16, 32})
+      for (int nt : {0, 1}) {
+        SlabConfig s;
+        s.sw = sw;
+        s.nt = nt;
+        c.push_back({HV_TWO_PASS, s});
+      }
+    return c;
+  }
+  void autotune() {
+    tuned = true;
+    best_variant = HV_FUSED;
+    if (dim < ((size_t)1 << 20)) return;  // small sectors: one fused launch
+    DevBuf<double> x(dim), y(dim);
+    fill(dim, x.p, 1.0 / std::sqrt((double)dim), stream());
+    fill(dim, y.p, 0.0, stream());
+    hipEvent_t e0, e1;
+    ED_HIP(hipEventCreate(&e0));
+    ED_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    tune_ms.clear();
+    for (const auto &c : candidates()) {
+      hv_factorized_cfg(args, x.p, y.p, 1.0, 0.0, nullptr, c.first, c.second, stream());
+      ED_HIP(hipEventRecord(e0, stream()));
+      for (int r = 0; r < 3; ++r) hv_factorized_cfg(args, x.p, y.p, 1.0, 0.0, nullptr, c.first, c.second, stream());
+      ED_HIP(hipEventRecord(e1, stream()));
+      ED_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      ED_HIP(hipEventElapsedTime(&t, e0, e1));
+      const double ms = t / 3.0;
+      tune_ms.push_back(ms);
+      if (ms < best) {
+        best = ms;
+        best_variant = c.first;
+        best_sc = c.second;
+      }
+    }
+    ED_HIP(hipEventDestroy(e0));
+    ED_HIP(hipEventDestroy(e1));
+  }
   int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
-    return hv_factorized(args, x, y, ax, beta, want_dot ? partials.p : nullptr, variant, stream());
+    int v = variant;
+    SlabConfig sc;
+    if (v == HV_AUTO) {
+      if (!tuned) autotune();
+      v = best_variant;
+      sc = best_sc;
+    }
+    return hv_factorized_cfg(args, x, y, ax, beta, want_dot ? partials.p : nullptr, v, sc, stream());
   }
   void diag(double *d) override { fact_full_diag(args, d, stream()); }
 };
@@ -614,6 +742,19 @@ struct HamOTF : Hamiltonian {
     if (dim) ED_HIP(hipMemcpyAsync(d, dg.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   }
 };
+
+std::vector<double> hamiltonian_autotune(Hamiltonian &H, int &best) {
+  auto *F = dynamic_cast<HamFactorized *>(&H);
+  if (!F) fail(ED_ERR_UNSUPPORTED, "autotuning applies to the factorized format only");
+  F->autotune();
+  const auto cands = HamFactorized::candidates();
+  best = 0;
+  for (size_t k = 0; k < cands.size(); ++k)
+    if (cands[k].first == F->best_variant && cands[k].second.sw == F->best_sc.sw &&
+        cands[k].second.nt == F->best_sc.nt)
+      best = (int)k;
+  return F->tune_ms;
+}
 
 static bool factorizable(const TermList &T) {
   for (const auto &tc : T.terms)

@@ -103,6 +103,9 @@ struct Hamiltonian {
 };
 
 std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format);
+// time the candidate factorized schedules (fused; two-pass x slab widths 16/32/64
+// x streaming on/off) and keep the fastest; returns ms per candidate
+std::vector<double> hamiltonian_autotune(Hamiltonian &H, int &best);
 
 // host strict-lower CSR pattern (Hamiltonian_CSR.hpp:60-81) for parity checks
 void csr_lower(const TermList &T, int Nup, int Ndw, std::vector<int32_t> &Iptr, std::vector<int32_t> &J,

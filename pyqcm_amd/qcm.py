@@ -35,6 +35,7 @@ _PARAM_TYPES = {
     "check_lanczos_residual": _BOOL, "no_degenerate_BL": _BOOL, "nosym": _BOOL, "continued_fraction": _BOOL,
     "verb_ED": _BOOL, "verb_Hilbert": _BOOL, "verb_warning": _BOOL, "parallel_sectors": _BOOL,
     "merge_states": _BOOL, "combine_mcf": _BOOL, "block_Lanczos_QR": _BOOL, "store_lanczos_basis": _BOOL,
+    "sell16": _BOOL,
     "accur_band_lanczos": _DOUBLE, "accur_continued_fraction": _DOUBLE, "accur_Davidson": _DOUBLE,
     "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
     "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
@@ -281,6 +282,13 @@ class Hamiltonian:
         ms = C.c_double()
         check(lib().ed_hamiltonian_time(self.h, int(nrep), C.byref(ms), int(variant)))
         return ms.value
+
+    def autotune(self):
+        """Time the candidate schedules; returns (ms per candidate, index chosen)."""
+        ms = np.zeros(16)
+        n, best = C.c_int(), C.c_int()
+        check(lib().ed_hamiltonian_autotune(self.h, 16, _dptr(ms), C.byref(n), C.byref(best)))
+        return ms[:n.value], best.value
 
     def set_variant(self, variant):
         check(lib().ed_hamiltonian_set_variant(self.h, int(variant)))
