@@ -114,7 +114,7 @@ int ed_hamiltonian_time(int64_t handle, int nrep, double *ms_per_apply, int vari
    row kernel, 2 two-pass (row up-part + column-slab down-part) */
 int ed_hamiltonian_set_variant(int64_t handle, int variant);
 /* autotuner of the factorized schedule (no reference counterpart: the reference
-   has one serial loop): times fused and two-pass x slab width {16,32} x
+   has one serial loop): times fused and two-pass x slab width {8,16,32} x
    streaming stores {off,on}; ms[k] per candidate in that order, best = chosen index */
 int ed_hamiltonian_autotune(int64_t handle, int cap, double *ms, int *n, int *best);
 
@@ -133,6 +133,10 @@ int ed_band_lanczos(int64_t h, int p, const double *phi, int cap, double *evals,
 int ed_davidson(int64_t h, int K, double *evals, double *evecs, int *niter);
 /* blockLanczos (src_ed/Hamiltonian/Lanczos.hpp:426-572), QR variant */
 int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double *A, double *B, int *nfl);
+/* lowest eigenpair of the symmetric tridiagonal (d[n], e[n-1]); z[n] may be NULL
+   (EigensystemTridiagonal, src_ed/Hamiltonian/Lanczos.cpp:25-52, as used by
+   LanczosEigenvalue / LanczosEigenvector for the Ritz pair) */
+int ed_tridiag_lowest(int n, const double *d, const double *e, double *ev0, double *z);
 /* c_a / c^dagger_a between factorized sectors (src_ed/model.cpp:155-182) */
 int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x, double z, double *y);
 

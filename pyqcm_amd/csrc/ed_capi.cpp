@@ -464,6 +464,17 @@ int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x
   });
 }
 
+int ed_tridiag_lowest(int n, const double *d, const double *e, double *ev0, double *z) {
+  return guard([&] {
+    if (n <= 0 || !d || (n > 1 && !e) || !ev0) fail(ED_ERR_INVALID, "ed_tridiag_lowest: bad arguments");
+    std::vector<double> zz;
+    double ev;
+    tridiag_eigvec0(n, d, e, ev, zz);
+    *ev0 = ev;
+    if (z) std::copy(zz.begin(), zz.end(), z);
+  });
+}
+
 // ---- GF kernels ----
 int ed_cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out) {
   return guard([&] {

@@ -1,6 +1,8 @@
 // ed_common.cpp -- global parameters, caching device allocator, library stream.
 #include "ed_common.hpp"
 
+#include <chrono>
+#include <cstdarg>
 #include <unordered_map>
 
 namespace ed {
@@ -43,8 +45,11 @@ Params::Params() {
   c["GF_method"] = 'L';
   // device-side knobs (no reference counterpart)
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
+  i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
-  b["sell16"] = false;              // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
+  b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
+  b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
+  b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
   d["lanczos_basis_budget_GB"] = 96.0;
 }
 
@@ -145,5 +150,24 @@ hipStream_t stream() {
 }
 
 void sync() { ED_HIP(hipStreamSynchronize(stream())); }
+
+bool verbose() { return params().getb("verb_ED"); }
+
+double wall_ms() {
+  using clk = std::chrono::steady_clock;
+  static const clk::time_point t0 = clk::now();
+  return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+void vlog(const char *fmt, ...) {
+  if (!verbose()) return;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  std::fprintf(stderr, "[ed %10.3f ms] %s\n", wall_ms(), buf);
+  std::fflush(stderr);
+}
 
 }  // namespace ed

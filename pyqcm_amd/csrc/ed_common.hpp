@@ -103,4 +103,11 @@ struct DevBuf {
 
 void sync();
 
+// verb_ED progress lines on stderr (the reference prints its solver progress
+// under the same flag), each stamped with the wall time in ms since the
+// library's first stamp
+bool verbose();
+void vlog(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+double wall_ms();
+
 }  // namespace ed

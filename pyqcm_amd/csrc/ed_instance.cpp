@@ -220,16 +220,31 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     char meth = (method == 'L' && ns > 1) ? 'D' : method;
     std::vector<double> ev;
     std::vector<DevBuf<double>> vecs;
+    vlog("ground state: sector %s dim %zu, Hamiltonian ready", s.name().c_str(), dim);
     if (meth == 'D') {
       davidson(*H, (int)ns, acc_dav, ev, vecs);
+      if (verbose()) {
+        std::ostringstream os;
+        os << std::setprecision(15);
+        for (double e : ev) os << " " << e;
+        vlog("Davidson: %ld states, eigenvalues%s", ns, os.str().c_str());
+      }
     } else if (meth == 'L') {
-      std::vector<double> x0;
-      random_start(dim, seed, x0);
-      DevBuf<double> d0;
-      d0.upload(x0);
+      DevBuf<double> d0(dim);
+      if (P.getb("host_random_start")) {
+        // the reference's own stream (libstdc++ default_random_engine)
+        std::vector<double> x0;
+        random_start(dim, seed, x0);
+        d0.upload(x0);
+      } else {
+        random_normal(dim, (uint64_t)seed, d0.p, stream());
+        scale(dim, d0.p, 1.0 / std::sqrt(ddot(dim, d0.p, d0.p)), stream());
+      }
+      vlog("random start vector ready");
       vecs.emplace_back(dim);
       LanczosGS R = lanczos_gs(*H, d0.p, vecs[0].p);
       ev.push_back(R.E0);
+      vlog("Lanczos: %d iterations, E0 = %.15g", R.niter, R.E0);
     } else {
       fail(ED_ERR_INVALID, std::string("Unknown method in Ground_state_method: ") + meth);
     }
@@ -398,7 +413,9 @@ static void build_gf(Instance &I, State &st, int spin) {
       std::vector<const double *> ph;
       for (auto &b : phi) ph.push_back(b.p);
       std::vector<double> e, Q;
+      vlog("GF %s (pm %+d, spin %d): dim %zu, %d start vectors ready", tgt.name().c_str(), pm, spin, dt, ns);
       band_lanczos(*H, ph, e, Q);
+      vlog("band Lanczos: %zu poles", e.size());
       for (auto &x : e) x -= st.energy;
       if (pm == -1)
         for (auto &x : e) x = -x;

@@ -256,27 +256,75 @@ __global__ __launch_bounds__(BS) void k_hv_rowk(FusedParams P) {
   }
 }
 
-// TM: 0 packed entry + value table, 1 explicit values, 2 16-bit sign-only
+// x value of a 16-bit sign-only entry (bit 15 = sign, bits 0-14 = column),
+// the sign applied by flipping the sign bit of the double
+__device__ __forceinline__ double sell16_term(const double *smem, uint32_t e) {
+  const double v = smem[e & 0x7fffu];
+  return __longlong_as_double(__double_as_longlong(v) ^ ((long long)(e & 0x8000u) << 48));
+}
+
+// Up part of one sliced-ELL slice for one lane: sum_j value_j * smem[col_j].
+// Entries come in lane groups of 4 words (8-byte loads for 16-bit entries,
+// 16-byte loads for packed 32-bit ones); up to eight group loads are issued
+// before the first LDS gather, so a wave keeps a whole slice's table reads
+// in flight at once.
 template <int TM>
-__device__ __forceinline__ double sell_entry(const DevHalf &h, const double *vtab, size_t e, uint32_t &col) {
+__device__ __forceinline__ double sell_slice_sum(const DevHalf &h, const SellTable &T, const double *vtab,
+                                                 const double *smem, size_t o0, size_t o1, uint32_t lane) {
+  constexpr uint32_t G = 4, QB = 8;
   if (TM == 2) {
-    const uint32_t w = h.sell16[e];
-    col = w & 0x7fffu;
-    return (w & 0x8000u) ? -h.t0 : h.t0;
+    const uint32_t nq = (uint32_t)((o1 - o0) / (kWave * G));
+    const uint2 *t = reinterpret_cast<const uint2 *>(T.e16 + o0) + lane;
+    double s0 = 0.0, s1 = 0.0;
+    for (uint32_t q = 0; q < nq; q += QB) {
+      uint2 g[QB] = {};
+#pragma unroll
+      for (uint32_t k = 0; k < QB; ++k)
+        if (q + k < nq) g[k] = t[(size_t)(q + k) * kWave];
+#pragma unroll
+      for (uint32_t k = 0; k < QB; ++k) {
+        if (q + k < nq) {
+          s0 += sell16_term(smem, g[k].x & 0xffffu);
+          s1 += sell16_term(smem, g[k].x >> 16);
+          s0 += sell16_term(smem, g[k].y & 0xffffu);
+          s1 += sell16_term(smem, g[k].y >> 16);
+        }
+      }
+    }
+    return (s0 + s1) * h.t0;
+  } else if (TM == 0) {
+    const uint32_t nq = (uint32_t)((o1 - o0) / (kWave * G));
+    const uint4 *t = reinterpret_cast<const uint4 *>(T.ent + o0) + lane;
+    const uint32_t mask = (1u << h.colbits) - 1u;
+    double s0 = 0.0, s1 = 0.0;
+    for (uint32_t q = 0; q < nq; q += QB / 2) {
+      uint4 g[QB / 2] = {};
+#pragma unroll
+      for (uint32_t k = 0; k < QB / 2; ++k)
+        if (q + k < nq) g[k] = t[(size_t)(q + k) * kWave];
+#pragma unroll
+      for (uint32_t k = 0; k < QB / 2; ++k) {
+        if (q + k < nq) {
+          s0 += vtab[g[k].x >> h.colbits] * smem[g[k].x & mask];
+          s1 += vtab[g[k].y >> h.colbits] * smem[g[k].y & mask];
+          s0 += vtab[g[k].z >> h.colbits] * smem[g[k].z & mask];
+          s1 += vtab[g[k].w >> h.colbits] * smem[g[k].w & mask];
+        }
+      }
+    }
+    return s0 + s1;
+  } else {
+    double s = 0.0;
+    for (size_t e = o0 + lane; e < o1; e += kWave) s += T.val[e] * smem[T.ent[e]];
+    return s;
   }
-  const uint32_t w = h.sell_ent[e];
-  if (TM == 0) {
-    col = w & ((1u << h.colbits) - 1u);
-    return vtab[w >> h.colbits];
-  }
-  col = w;
-  return h.sell_val[e];
 }
 
 // Row kernel with the up-spin operator in sliced-ELL form: each wave takes
-// slices of 64 consecutive up states; entry j of the slice is one coalesced
-// 256-byte read, the gathers hit the LDS-staged row, and the loop bounds are
-// wave-uniform.  The down part (DOWN) reads the K_dn neighbour rows (staged
+// slices of 64 consecutive up states and sums their entries against the
+// LDS-staged row (sell_slice_sum); loop bounds are wave-uniform.  TM is the
+// table mode: 0 packed entry + value table, 1 explicit values, 2 16-bit
+// sign-only.  The down part (DOWN) reads the K_dn neighbour rows (staged
 // list) for the same 64 states with four independent loads in flight.
 template <int BS, bool DOWN, int TM>
 __global__ __launch_bounds__(BS) void k_hv_rows_sell(FusedParams P) {
@@ -296,6 +344,7 @@ __global__ __launch_bounds__(BS) void k_hv_rows_sell(FusedParams P) {
   const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
   const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   constexpr uint32_t NW = BS / kWave;
+  const SellTable &T = a.up.sell[DOWN ? 0 : 1];
   double dot = 0.0;
   for (uint32_t d = blockIdx.x; d < dd; d += gridDim.x) {
     const size_t base = (size_t)d * du;
@@ -314,53 +363,34 @@ __global__ __launch_bounds__(BS) void k_hv_rows_sell(FusedParams P) {
     }
     __syncthreads();
     const uint32_t sd = (a.diag_mode != 0) ? a.dn.states[d] : 0u;
-    for (uint32_t s = wave; s < a.up.nslice; s += NW) {
-      const uint32_t u = s * kWave + lane;
+    for (uint32_t s = wave; s < T.nslice; s += NW) {
+      const uint32_t u = T.row ? T.row[s * kWave + lane] : s * kWave + lane;
       const bool ok = u < du;
       const uint32_t uu = ok ? u : du - 1;
-      const size_t o0 = a.up.sell_off[s], o1 = a.up.sell_off[s + 1];
+      const size_t o0 = T.off[s], o1 = T.off[s + 1];
       double acc = diag_value(a, Ush, base + uu, uu, d, sd) * smem[uu];
-      size_t e = o0 + lane;
-      for (; e + 3 * kWave < o1; e += 4 * kWave) {
-        uint32_t c0, c1, c2, c3;
-        const double v0 = sell_entry<TM>(a.up, vtab_u, e, c0);
-        const double v1 = sell_entry<TM>(a.up, vtab_u, e + kWave, c1);
-        const double v2 = sell_entry<TM>(a.up, vtab_u, e + 2 * kWave, c2);
-        const double v3 = sell_entry<TM>(a.up, vtab_u, e + 3 * kWave, c3);
-        acc += v0 * smem[c0] + v1 * smem[c1] + v2 * smem[c2] + v3 * smem[c3];
-      }
-      for (; e < o1; e += kWave) {
-        uint32_t c;
-        const double v = sell_entry<TM>(a.up, vtab_u, e, c);
-        acc += v * smem[c];
-      }
+      acc += sell_slice_sum<TM>(a.up, T, vtab_u, smem, o0, o1, lane);
       if (DOWN) {
+        // staged neighbours: eight (then four) independent row reads in flight
         uint32_t j = 0;
-        for (; j + 4 <= nnb; j += 4) {
-          uint32_t c0, c1, c2, c3;
-          double v0, v1, v2, v3;
-          if (j + 3 < (uint32_t)kMaxNb) {
-            c0 = ncol[j]; v0 = nval[j];
-            c1 = ncol[j + 1]; v1 = nval[j + 1];
-            c2 = ncol[j + 2]; v2 = nval[j + 2];
-            c3 = ncol[j + 3]; v3 = nval[j + 3];
-          } else {
-            v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
-            v1 = half_entry(a.dn, vtab_d, g0 + j + 1, c1);
-            v2 = half_entry(a.dn, vtab_d, g0 + j + 2, c2);
-            v3 = half_entry(a.dn, vtab_d, g0 + j + 3, c3);
-          }
-          const double x0 = P.x[(size_t)c0 * du + uu];
-          const double x1 = P.x[(size_t)c1 * du + uu];
-          const double x2 = P.x[(size_t)c2 * du + uu];
-          const double x3 = P.x[(size_t)c3 * du + uu];
-          acc += v0 * x0 + v1 * x1 + v2 * x2 + v3 * x3;
+        for (; j + 8 <= nst; j += 8) {
+          double xv[8];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) xv[m] = P.x[(size_t)ncol[j + m] * du + uu];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) acc += nval[j + m] * xv[m];
         }
-        for (; j < nnb; ++j) {
+        for (; j + 4 <= nst; j += 4) {
+          double xv[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) xv[m] = P.x[(size_t)ncol[j + m] * du + uu];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) acc += nval[j + m] * xv[m];
+        }
+        for (; j < nst; ++j) acc += nval[j] * P.x[(size_t)ncol[j] * du + uu];
+        for (; j < nnb; ++j) {  // rows with more than kMaxNb neighbours
           uint32_t c0;
-          double v0;
-          if (j < (uint32_t)kMaxNb) { c0 = ncol[j]; v0 = nval[j]; }
-          else v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
+          const double v0 = half_entry(a.dn, vtab_d, g0 + j, c0);
           acc += v0 * P.x[(size_t)c0 * du + uu];
         }
       }
@@ -508,6 +538,17 @@ __global__ __launch_bounds__(BS) void k_fold_partials(const double *in, size_t n
   __shared__ double red[BS / kWave];
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) s += in[i];
+  s = block_sum<BS>(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// m rows of nb partials -> m sums, one block per row, fixed order
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fold_rows(const double *in, int nb, double *out) {
+  __shared__ double red[BS / kWave];
+  const double *p = in + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += BS) s += p[i];
   s = block_sum<BS>(s, red);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
@@ -689,6 +730,31 @@ __global__ __launch_bounds__(BS) void k_fill(size_t n, double *x, double v) {
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] = v;
 }
 __global__ void k_set1(double *x, size_t i) { x[i] = 1.0; }
+
+// Counter-based N(0,1) draws: SplitMix64 (Steele, Lea & Flood, OOPSLA 2014)
+// hashes (seed, pair index) into two 53-bit uniforms, Box-Muller turns them
+// into the pair (r cos t, r sin t); element i takes component i % 2 of pair
+// i / 2, so the vector does not depend on the launch shape.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+template <int BS>
+__global__ __launch_bounds__(BS) void k_random_normal(size_t n, uint64_t seed, double *x) {
+  const uint64_t key = splitmix64(seed);
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    const uint64_t pr = i >> 1;
+    const uint64_t a = splitmix64(key ^ (2 * pr)), b = splitmix64(key ^ (2 * pr + 1));
+    const double u1 = ((a >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+    const double u2 = (b >> 11) * 0x1.0p-53;        // [0, 1)
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    x[i] = (i & 1) ? r * sn : r * cs;
+  }
+}
 
 // Davidson correction (src_ed/Hamiltonian/Davidson.hpp:181-188): r_i /= (mu - d_i + tiny)
 template <int BS>
@@ -962,7 +1028,7 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
     hipLaunchKernelGGL(kern, dim3(g), dim3(BS), lds, s, P);                     \
   } while (0)
     const uint32_t kneed = (du + BS - 1) / BS;
-    if (stage && R == 1 && a.up.nslice > 0) {
+    if (stage && R == 1 && a.up.sell[0].nslice > 0 && a.up.sell[1].nslice > 0) {
       const int tm = a.up.mode16 ? 2 : (a.up.packed ? 0 : 1);
 #define ED_SELL(DOWN, TMV)                                                    \
   do {                                                                        \
@@ -1011,7 +1077,7 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   Q.y = y;
   Q.ax = ax;
   Q.nxcd = (sc.nxcd == 1) ? 1 : 8;
-  const int SW = (sc.sw == 16 || sc.sw == 64) ? sc.sw : 32;
+  const int SW = (sc.sw == 8 || sc.sw == 16 || sc.sw == 64) ? sc.sw : 32;
   Q.nslab = (du + SW - 1) / SW;
   Q.chunk_rows = kSlabRows;
   Q.nchunk = (dd + kSlabRows - 1) / kSlabRows;
@@ -1023,7 +1089,8 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   if (part && nb > (size_t)kPartialsCap && big.n < nb) big.alloc(nb);
   Q.part = part ? (nb > (size_t)kPartialsCap ? big.p : part) : nullptr;
 #define ED_SLAB(SWV, NTV) hipLaunchKernelGGL((k_hv_down_slab<SWV, 256 / SWV, NTV>), dim3((unsigned)nb), dim3(256), 0, s, Q)
-  if (SW == 16) { if (sc.nt) ED_SLAB(16, true); else ED_SLAB(16, false); }
+  if (SW == 8) { if (sc.nt) ED_SLAB(8, true); else ED_SLAB(8, false); }
+  else if (SW == 16) { if (sc.nt) ED_SLAB(16, true); else ED_SLAB(16, false); }
   else if (SW == 64) { if (sc.nt) ED_SLAB(64, true); else ED_SLAB(64, false); }
   else { if (sc.nt) ED_SLAB(32, true); else ED_SLAB(32, false); }
 #undef ED_SLAB
@@ -1114,6 +1181,16 @@ void scale(size_t n, double *x, double a, hipStream_t s) {
 void fill(size_t n, double *x, double v, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL((k_fill<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, x, v);
+  ED_HIP(hipGetLastError());
+}
+void fold_rows(const double *part, int nb, int m, double *out, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL((k_fold_rows<256>), dim3(m), dim3(256), 0, s, part, nb, out);
+  ED_HIP(hipGetLastError());
+}
+void random_normal(size_t n, uint64_t seed, double *x, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL((k_random_normal<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, seed, x);
   ED_HIP(hipGetLastError());
 }
 void replay_step(size_t n, double *p, const double *r, double *g, double c1, double c2, hipStream_t s) {

@@ -12,6 +12,21 @@ namespace ed {
 // summed over terms with their coefficients).  Entries are packed as
 // col | (value_id << colbits) when the value table is small enough,
 // otherwise val[] holds one double per entry.
+// Sliced-ELL form of a half operator: slices of 64 lanes, lane l of slice s
+// holding state row[64 s + l] (or 64 s + l when row is null; dim marks an
+// empty lane).  A slice is padded to its longest row, rounded up to whole
+// lane groups of 4 entries; groups are stored group-major, lane-minor, so a
+// wave's group load is one contiguous run.  Padding entries use column dim,
+// a zero slot after the LDS-staged row.
+struct SellTable {
+  uint32_t nslice = 0;
+  const uint32_t *off = nullptr;   // nslice + 1 entry offsets
+  const uint32_t *row = nullptr;   // nslice * 64 states, or null (identity)
+  const uint32_t *ent = nullptr;   // packed col | value id << colbits (packed), else col
+  const double *val = nullptr;     // explicit values (packed == 0)
+  const uint16_t *e16 = nullptr;   // 16-bit entries (mode16)
+};
+
 struct DevHalf {
   uint32_t dim = 0;
   const uint32_t *ptr = nullptr;   // dim + 1
@@ -22,18 +37,14 @@ struct DevHalf {
   int packed = 1;
   int colbits = 0;
   uint32_t nval = 0;               // size of the value table (packed == 1)
-  // the same operator as a sliced ELL table: slices of 64 consecutive states,
-  // each padded to its longest row and stored entry-major (entry j of the 64
-  // rows contiguous), so a wave reads one coalesced 256-byte line per entry
-  uint32_t nslice = 0;
-  const uint32_t *sell_off = nullptr;  // nslice + 1 offsets (entries)
-  const uint32_t *sell_ent = nullptr;  // packed like ent (padding: own column, zero value)
-  const double *sell_val = nullptr;    // explicit values when packed == 0
+  // the same operator as sliced-ELL tables (see SellTable): [0] in state
+  // order (the fused kernel's down part needs lane u on column u), [1] with
+  // the states sorted by row length, which removes most of the padding
+  SellTable sell[2];
   // single-magnitude operators (every value is +-t0, dim < 2^15): 16-bit
-  // entries col | sign << 15; padding points at the zero slot after the row
+  // entries col | sign << 15 in SellTable::e16
   int mode16 = 0;
   double t0 = 0.0;
-  const uint16_t *sell16 = nullptr;
 };
 
 // slab configuration of the two-pass down kernel (autotuned per plan)
@@ -101,6 +112,11 @@ int combine(size_t n, double *t, double st, int m, const double *const *b, const
             const double *w, double *dot_partials, hipStream_t s);
 void scale(size_t n, double *x, double a, hipStream_t s);
 void fill(size_t n, double *x, double v, hipStream_t s);
+// out[k] = sum_i part[k * nb + i], k < m (deterministic order); out may be
+// pinned host memory mapped for the device
+void fold_rows(const double *part, int nb, int m, double *out, hipStream_t s);
+// x[i] ~ N(0, 1), deterministic in (seed, i)
+void random_normal(size_t n, uint64_t seed, double *x, hipStream_t s);
 void set_unit(double *x, size_t i, hipStream_t s);
 // r_i /= (mu - d_i + tiny)
 void precond(size_t n, double *r, const double *d, double mu, double tiny, hipStream_t s);

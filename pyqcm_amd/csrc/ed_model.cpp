@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <limits>
 #include <numeric>
 #include <unordered_map>
 
@@ -191,29 +192,31 @@ static inline int hop_sign(uint32_t s, int from, int to) {
   return (__builtin_popcount(s & inside) & 1) ? -1 : 1;
 }
 
+// Sums of m rows of nb per-block partials.  The rows are folded on the
+// device in a fixed order (deterministic) and the m results written straight
+// into pinned host memory, so one stream synchronisation returns them.
 double sum_partials(const double *dpart, int nb, int m, std::vector<double> *out) {
-  static thread_local std::vector<double> h;
-  h.resize((size_t)nb * m);
-  if (nb * m) {
-    ED_HIP(hipMemcpyAsync(h.data(), dpart, sizeof(double) * nb * m, hipMemcpyDeviceToHost, stream()));
-    ED_HIP(hipStreamSynchronize(stream()));
-  }
-  if (out) out->assign(m, 0.0);
-  double first = 0.0;
-  for (int k = 0; k < m; ++k) {
-    // pairwise summation in a fixed order: deterministic and accurate
-    std::vector<double> w(h.begin() + (size_t)k * nb, h.begin() + (size_t)(k + 1) * nb);
-    size_t n = w.size();
-    while (n > 1) {
-      size_t half = (n + 1) / 2;
-      for (size_t i = 0; i + half < n; ++i) w[i] += w[i + half];
-      n = half;
+  constexpr int kCap = 1024;
+  struct Pinned {
+    double *h = nullptr, *d = nullptr;
+    Pinned() {
+      ED_HIP(hipHostMalloc((void **)&h, kCap * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+      ED_HIP(hipHostGetDevicePointer((void **)&d, h, 0));
     }
-    const double s = n ? w[0] : 0.0;
-    if (out) (*out)[k] = s;
-    if (k == 0) first = s;
+  };
+  static thread_local Pinned pin;
+  if (out) out->assign(m, 0.0);
+  if (m <= 0) return 0.0;
+  if (m > kCap) fail(ED_ERR_INTERNAL, "sum_partials: too many rows");
+  if (nb > 0) {
+    fold_rows(dpart, nb, m, pin.d, stream());
+    ED_HIP(hipStreamSynchronize(stream()));
+  } else {
+    for (int k = 0; k < m; ++k) pin.h[k] = 0.0;
   }
-  return first;
+  if (out)
+    for (int k = 0; k < m; ++k) (*out)[k] = pin.h[k];
+  return pin.h[0];
 }
 
 double Hamiltonian::dot_result(int nb) { return sum_partials(partials.p, nb); }
@@ -286,8 +289,9 @@ static HalfHost build_half(const TermList &T, int spin, const HalfBasis &B) {
   }
   for (uint32_t r = 0; r < B.dim; ++r) H.ptr[r + 1] += H.ptr[r];
   // value table
+  // columns 0..dim (dim itself is the sliced-ELL padding column) must fit
   H.colbits = 1;
-  while (H.colbits < 32 && (1ull << H.colbits) < (uint64_t)std::max<uint32_t>(B.dim, 2)) ++H.colbits;
+  while (H.colbits < 32 && (1ull << H.colbits) <= (uint64_t)B.dim) ++H.colbits;
   std::vector<double> table;
   std::unordered_map<uint64_t, uint32_t> ids;
   bool ok = H.colbits < 32;
@@ -319,77 +323,126 @@ static HalfHost build_half(const TermList &T, int spin, const HalfBasis &B) {
   return H;
 }
 
-// sliced-ELL copy of a half table (slices of 64 rows, entry-major).  Padding
-// entries point at column dim (a zero slot the kernels place after the row).
-static void build_sell(const HalfHost &H, std::vector<uint32_t> &off, std::vector<uint32_t> &ent,
-                       std::vector<double> &val, std::vector<uint16_t> &e16, double &t0) {
+// single magnitude test for the 16-bit sliced-ELL form: every value is
+// +-t0 and columns 0..dim fit in 15 bits; returns t0 or 0
+static double single_magnitude(const HalfHost &H) {
+  if (!(H.dim < 32767 && params().getb("sell16"))) return 0.0;
+  double t0 = 0.0;
+  for (size_t e = 0; e < H.ent.size(); ++e) {
+    const double v = std::fabs(H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e]);
+    if (t0 == 0.0) t0 = v;
+    if (v != t0) return 0.0;
+  }
+  return t0;
+}
+
+// Host image of one SellTable (layout described with SellTable): lane l of
+// slice s holds state order[64 s + l]; the entries of a slice are padded to
+// its longest row, rounded up to groups of G (G = 4 for 16-bit and packed
+// entries, 1 for explicit values).
+struct SellHost {
+  std::vector<uint32_t> off, row, ent;
+  std::vector<double> val;
+  std::vector<uint16_t> e16;
+};
+
+static SellHost build_sell(const HalfHost &H, const std::vector<uint32_t> &order, double t0) {
   const uint32_t n = H.dim, ns = (n + 63) / 64;
-  off.assign((size_t)ns + 1, 0);
+  const bool single = t0 != 0.0;
+  const uint32_t G = (single || H.packed) ? 4u : 1u;
+  SellHost S;
+  S.row.assign((size_t)ns * 64, n);
+  for (uint32_t p = 0; p < n; ++p) S.row[p] = order[p];
+  S.off.assign((size_t)ns + 1, 0);
   for (uint32_t s = 0; s < ns; ++s) {
     uint32_t w = 0;
-    for (uint32_t r = s * 64; r < std::min(n, s * 64 + 64); ++r) w = std::max(w, H.ptr[r + 1] - H.ptr[r]);
-    off[s + 1] = off[s] + 64 * w;
+    for (uint32_t p = s * 64; p < s * 64 + 64; ++p) {
+      const uint32_t r = S.row[p];
+      if (r < n) w = std::max(w, H.ptr[r + 1] - H.ptr[r]);
+    }
+    w = (w + G - 1) / G * G;
+    S.off[s + 1] = S.off[s] + 64 * w;
   }
-  // single magnitude?
-  bool single = n < 32767;
-  t0 = 0.0;
-  for (size_t e = 0; e < H.ent.size() && single; ++e) {
-    const double v = H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e];
-    if (t0 == 0.0) t0 = std::fabs(v);
-    if (std::fabs(v) != t0) single = false;
-  }
-  if (t0 == 0.0) single = false;
-  const size_t tot = std::max<size_t>(off[ns], 1);
-  ent.assign(single ? 1 : tot, 0u);
-  val.assign((single || H.packed) ? 1 : tot, 0.0);
-  e16.assign(single ? tot : 1, 0u);
-  const uint32_t pad_packed = H.packed ? n : n;  // column dim, value irrelevant (x slot is 0)
+  const size_t tot = std::max<size_t>(S.off[ns], 1);
+  S.ent.assign(single ? 1 : tot, 0u);
+  S.val.assign((single || H.packed) ? 1 : tot, 0.0);
+  S.e16.assign(single ? tot : 1, 0u);
   for (uint32_t s = 0; s < ns; ++s) {
-    const uint32_t w = (off[s + 1] - off[s]) / 64;
+    const uint32_t w = (S.off[s + 1] - S.off[s]) / 64;
     for (uint32_t lane = 0; lane < 64; ++lane) {
-      const uint32_t r = s * 64 + lane;
+      const uint32_t r = S.row[(size_t)s * 64 + lane];
+      const uint32_t len = r < n ? H.ptr[r + 1] - H.ptr[r] : 0;
       for (uint32_t j = 0; j < w; ++j) {
-        const size_t dst = (size_t)off[s] + (size_t)j * 64 + lane;
-        const bool real = r < n && j < H.ptr[r + 1] - H.ptr[r];
+        const size_t dst = (size_t)S.off[s] + (size_t)(j / G) * 64 * G + (size_t)lane * G + j % G;
+        const bool real = j < len;
+        const size_t e = real ? H.ptr[r] + j : 0;
         if (single) {
           if (real) {
-            const size_t e = H.ptr[r] + j;
             const uint32_t col = H.packed ? (H.ent[e] & ((1u << H.colbits) - 1u)) : H.ent[e];
             const double v = H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e];
-            e16[dst] = (uint16_t)(col | (v < 0.0 ? 0x8000u : 0u));
+            S.e16[dst] = (uint16_t)(col | (v < 0.0 ? 0x8000u : 0u));
           } else {
-            e16[dst] = (uint16_t)n;
+            S.e16[dst] = (uint16_t)n;
           }
         } else if (real) {
-          ent[dst] = H.ent[H.ptr[r] + j];
-          if (!H.packed) val[dst] = H.val[H.ptr[r] + j];
+          S.ent[dst] = H.ent[e];
+          if (!H.packed) S.val[dst] = H.val[e];
         } else {
-          ent[dst] = pad_packed;  // col = dim, value id 0 (any value times the zero slot)
-          if (!H.packed) val[dst] = 0.0;
+          S.ent[dst] = n;  // col = dim, value id 0 (any value times the zero slot)
+          if (!H.packed) S.val[dst] = 0.0;
         }
       }
     }
   }
-  if (!single) t0 = 0.0;
+  return S;
 }
 
+struct DevSell {
+  DevBuf<uint32_t> off, row, ent;
+  DevBuf<double> val;
+  DevBuf<uint16_t> e16;
+  uint32_t nslice = 0;
+  SellTable view(bool identity) const {
+    SellTable T;
+    T.nslice = nslice;
+    T.off = off.p;
+    T.row = identity ? nullptr : row.p;
+    T.ent = ent.p;
+    T.val = val.p;
+    T.e16 = e16.p;
+    return T;
+  }
+  void upload(const SellHost &S) {
+    nslice = (uint32_t)S.off.size() - 1;
+    off.upload(S.off);
+    row.upload(S.row);
+    ent.upload(S.ent);
+    val.upload(S.val);
+    e16.upload(S.e16);
+  }
+};
+
 struct DevHalfStore {
-  DevBuf<uint32_t> ptr, ent, states, soff, sent;
-  DevBuf<double> val, diag, sval;
-  DevBuf<uint16_t> s16;
+  DevBuf<uint32_t> ptr, ent, states;
+  DevBuf<double> val, diag;
+  DevSell sell_nat, sell_sorted;
   DevHalf view;
   HalfHost host;
   void upload(const HalfHost &Hin, const HalfBasis &B) {
     HalfHost H = Hin;
-    std::vector<uint32_t> so, se;
-    std::vector<double> sv;
-    std::vector<uint16_t> e16;
-    double t0 = 0.0;
-    build_sell(H, so, se, sv, e16, t0);
-    soff.upload(so);
-    sent.upload(se);
-    sval.upload(sv);
-    s16.upload(e16);
+    const double t0 = single_magnitude(H);
+    std::vector<uint32_t> nat(H.dim), byrow(H.dim);
+    for (uint32_t i = 0; i < H.dim; ++i) nat[i] = byrow[i] = i;
+    // rows sorted by length inside windows of sell_sort_window states (0: no
+    // sorting); a window bounds how far apart one wave's output stores land
+    const long win = params().geti("sell_sort_window");
+    const uint32_t w = win <= 0 ? 1u : (uint32_t)std::min<long>(win, (long)std::max<uint32_t>(H.dim, 1));
+    for (uint32_t b0 = 0; b0 < H.dim; b0 += w)
+      std::stable_sort(byrow.begin() + b0, byrow.begin() + std::min(H.dim, b0 + w), [&](uint32_t p, uint32_t q) {
+        return H.ptr[p + 1] - H.ptr[p] > H.ptr[q + 1] - H.ptr[q];
+      });
+    sell_nat.upload(build_sell(H, nat, t0));
+    sell_sorted.upload(build_sell(H, byrow, t0));
     host = H;
     ptr.upload(H.ptr);
     ent.upload(H.ent.empty() ? std::vector<uint32_t>{0u} : H.ent);
@@ -405,13 +458,10 @@ struct DevHalfStore {
     view.packed = H.packed;
     view.colbits = H.colbits;
     view.nval = H.nval;
-    view.nslice = (uint32_t)so.size() - 1;
-    view.sell_off = soff.p;
-    view.sell_ent = sent.p;
-    view.sell_val = sval.p;
-    view.mode16 = (t0 != 0.0 && params().getb("sell16")) ? 1 : 0;
+    view.sell[0] = sell_nat.view(true);
+    view.sell[1] = sell_sorted.view(w <= 1);
+    view.mode16 = t0 != 0.0 ? 1 : 0;
     view.t0 = t0;
-    view.sell16 = s16.p;
   }
 };
 
@@ -508,7 +558,7 @@ struct HamFactorized : Hamiltonian {
   static std::vector<std::pair<int, SlabConfig>> candidates() {
     std::vector<std::pair<int, SlabConfig>> c;
     c.push_back({HV_FUSED, SlabConfig()});
-    for (int sw : {16, 32})
+    for (int sw : {8, 16, 32})
       for (int nt : {0, 1}) {
         SlabConfig s;
         s.sw = sw;
@@ -837,29 +887,93 @@ void ql_rows(int n, double *d, double *e, double *Z, int nz) {
   }
 }
 
+// Lowest eigenpair of the symmetric tridiagonal matrix (d[0..n), e[0..n-1)).
+// The eigenvalue comes from bisection on the Sturm count (inertia of the
+// LDL^T factorisation of T - x), O(n) per step; the eigenvector from inverse
+// iteration with T - (lambda - delta), which is positive definite, so the
+// unpivoted LDL^T solve is stable; delta / gap is ~1e-12, and two sweeps reach
+// rounding level.  O(n) memory and O(60 n) work, where the dense QL with
+// eigenvectors costs O(n^2) to O(n^3) per call.
+static int sturm_below(int n, const double *d, const double *e, double x) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (q < 0.0) ++cnt;
+  for (int i = 1; i < n; ++i) {
+    if (q == 0.0) q = -1e-300;
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+static double tridiag_min_eig(int n, const double *d, const double *e, double &norm) {
+  double lo = d[0], hi = d[0];
+  norm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+    norm = std::max(norm, std::fabs(d[i]) + r);
+  }
+  // hi: at least one eigenvalue below; lo: none below
+  hi += 1e-300 + 4.0 * std::numeric_limits<double>::epsilon() * norm;
+  lo -= 1e-300 + 4.0 * std::numeric_limits<double>::epsilon() * norm;
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int it = 0; it < 256; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (!(mid > lo && mid < hi)) break;
+    if (hi - lo <= 2.0 * eps * std::max(std::fabs(lo), std::fabs(hi)) + 1e-3 * eps * norm) break;
+    if (sturm_below(n, d, e, mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+static void tridiag_inverse_iteration(int n, const double *d, const double *e, double lam, double norm,
+                                      std::vector<double> &z) {
+  z.assign(n, 1.0);
+  std::vector<double> D(n), l(n > 1 ? n - 1 : 1);
+  double delta = (4.0 * std::max(n, 16)) * std::numeric_limits<double>::epsilon() * std::max(norm, 1e-300);
+  for (int attempt = 0;; ++attempt) {
+    const double sigma = lam - delta;
+    bool ok = true;
+    D[0] = d[0] - sigma;
+    for (int i = 0; i < n - 1 && ok; ++i) {
+      if (!(D[i] > 0.0)) { ok = false; break; }
+      l[i] = e[i] / D[i];
+      D[i + 1] = d[i + 1] - sigma - l[i] * e[i];
+    }
+    if (ok && !(D[n - 1] > 0.0)) ok = false;
+    if (ok) break;
+    if (attempt == 40) fail(ED_ERR_NOT_CONVERGED, "tridiagonal inverse iteration: no positive definite shift");
+    delta *= 8.0;
+  }
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    for (int i = 1; i < n; ++i) z[i] -= l[i - 1] * z[i - 1];       // L y = z
+    for (int i = 0; i < n; ++i) z[i] /= D[i];                       // D w = y
+    for (int i = n - 2; i >= 0; --i) z[i] -= l[i] * z[i + 1];      // L^T x = w
+    double s = 0.0, m = 0.0;
+    for (int i = 0; i < n; ++i) m = std::max(m, std::fabs(z[i]));
+    for (int i = 0; i < n; ++i) { z[i] /= m; s += z[i] * z[i]; }
+    s = 1.0 / std::sqrt(s);
+    for (int i = 0; i < n; ++i) z[i] *= s;
+  }
+}
+
 void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0) {
-  std::vector<double> dd(d, d + n), ee(n, 0.0), z(n, 0.0);
-  for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
-  z[n - 1] = 1.0;  // last row of the identity
-  ql_rows(n, dd.data(), ee.data(), z.data(), 1);
-  int k = 0;
-  for (int i = 1; i < n; ++i)
-    if (dd[i] < dd[k]) k = i;
-  ev0 = dd[k];
-  zlast0 = z[k];
+  if (n <= 0) fail(ED_ERR_INVALID, "empty tridiagonal matrix");
+  double norm;
+  ev0 = tridiag_min_eig(n, d, e, norm);
+  std::vector<double> z;
+  tridiag_inverse_iteration(n, d, e, ev0, norm, z);
+  zlast0 = z[n - 1];
 }
 
 void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0) {
-  std::vector<double> dd(d, d + n), ee(n, 0.0), Z((size_t)n * n, 0.0);
-  for (int i = 0; i < n - 1; ++i) ee[i] = e[i];
-  for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
-  ql_rows(n, dd.data(), ee.data(), Z.data(), n);
-  int k = 0;
-  for (int i = 1; i < n; ++i)
-    if (dd[i] < dd[k]) k = i;
-  ev0 = dd[k];
-  z0.resize(n);
-  for (int i = 0; i < n; ++i) z0[i] = Z[(size_t)i * n + k];
+  if (n <= 0) fail(ED_ERR_INVALID, "empty tridiagonal matrix");
+  double norm;
+  ev0 = tridiag_min_eig(n, d, e, norm);
+  tridiag_inverse_iteration(n, d, e, ev0, norm, z0);
 }
 
 // Householder tridiagonalisation (row-major working copy) + implicit QL

@@ -90,7 +90,8 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
   const double accur = P.getd("accur_lanczos");
   const size_t n = H.dim;
   const double budget = P.getd("lanczos_basis_budget_GB") * 1e9;
-  bool store = P.getb("store_lanczos_basis");
+  const bool reorth = P.getb("lanczos_reorth");
+  bool store = P.getb("store_lanczos_basis") || reorth;
   LanczosGS R;
   std::vector<DevBuf<double>> basis;  // basis[k] = s_k v_k
   std::vector<double> sc;             // s_k
@@ -109,6 +110,9 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
     if (store) {
       // w = H v_k - beta_{k-1} v_{k-1} - alpha v_k into a fresh buffer
       if ((double)(basis.size() + 1) * n * 8.0 > budget) {
+        if (reorth)
+          fail(ED_ERR_OOM, "lanczos_reorth keeps the Krylov basis in HBM: " + std::to_string(basis.size() + 1) +
+                               " vectors exceed lanczos_basis_budget_GB");
         // HBM budget reached: continue with two buffers and replay at the end
         store = false;
         d2d(A.p, basis[k].p, n);
@@ -127,7 +131,24 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
       src.push_back(basis[k].p);
       cf.push_back(-alpha / sc[k]);
       if (k > 0) { src.push_back(basis[k - 1].p); cf.push_back(-R.beta[k - 1] / sc[k - 1]); }
-      beta = std::sqrt(comb(n, W.p, 1.0, src, cf, true));
+      double w2 = comb(n, W.p, 1.0, src, cf, true);
+      if (reorth) {
+        // full FP64 reorthogonalisation against the stored basis (classical
+        // Gram-Schmidt; a second pass when the first removed more than half
+        // of the norm squared, the "twice is enough" rule)
+        std::vector<const double *> all;
+        for (int j = 0; j <= k; ++j) all.push_back(basis[j].p);
+        for (int pass = 0; pass < 2; ++pass) {
+          std::vector<double> c = dots(n, W.p, all);
+          std::vector<double> cfr(all.size());
+          for (size_t j = 0; j < all.size(); ++j) cfr[j] = -c[j] / (sc[j] * sc[j]);
+          const double w2n = comb(n, W.p, 1.0, all, cfr, true);
+          const bool again = w2n < 0.5 * w2;
+          w2 = w2n;
+          if (!again) break;
+        }
+      }
+      beta = std::sqrt(w2);
       basis.push_back(std::move(W));
       sc.push_back(beta);
     } else {
@@ -345,6 +366,7 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     }
   }
   const int M = j;
+  vlog("band Lanczos: %d steps, %d deflations, dim %zu", M, nd, dim);
   if (M == 0) return;
   std::vector<double> Hp((size_t)M * M);
   for (int c = 0; c < M; ++c)

@@ -35,13 +35,14 @@ _PARAM_TYPES = {
     "check_lanczos_residual": _BOOL, "no_degenerate_BL": _BOOL, "nosym": _BOOL, "continued_fraction": _BOOL,
     "verb_ED": _BOOL, "verb_Hilbert": _BOOL, "verb_warning": _BOOL, "parallel_sectors": _BOOL,
     "merge_states": _BOOL, "combine_mcf": _BOOL, "block_Lanczos_QR": _BOOL, "store_lanczos_basis": _BOOL,
-    "sell16": _BOOL,
+    "sell16": _BOOL, "lanczos_reorth": _BOOL, "host_random_start": _BOOL,
     "accur_band_lanczos": _DOUBLE, "accur_continued_fraction": _DOUBLE, "accur_Davidson": _DOUBLE,
     "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
     "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
     "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
     "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
+    "sell_sort_window": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 
@@ -348,6 +349,19 @@ def create_or_destroy(L, pm, a, Nup, Ndw, x, z=1.0):
     y = np.zeros(dim)
     check(lib().ed_create_or_destroy(L, pm, a, Nup, Ndw, _dptr(x), float(z), _dptr(y)))
     return y
+
+
+def tridiag_lowest(d, e):
+    """Lowest eigenvalue and normalized eigenvector of the symmetric tridiagonal
+    matrix with diagonal d and off-diagonal e (the Ritz pair of the Lanczos driver)."""
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    if len(e) != max(len(d) - 1, 0):
+        raise EDError(1, "tridiag_lowest: e must have len(d) - 1 entries")
+    ev = C.c_double()
+    z = np.zeros(len(d))
+    check(lib().ed_tridiag_lowest(len(d), _dptr(d), _dptr(e) if len(e) else _dptr(np.zeros(1)), C.byref(ev), _dptr(z)))
+    return ev.value, z
 
 
 def cf_eval(fractions, z):

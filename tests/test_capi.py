@@ -67,3 +67,46 @@ def test_global_parameters_roundtrip():
     n = C.c_long()
     _lib.check(_lib.lib().ed_get_global_int(b"max_iter_lanczos", C.byref(n)))
     assert n.value == 1000
+
+
+def _tri_dense(d, e):
+    return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+
+
+def _tri_cases():
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in (1, 2, 3, 7, 50, 200, 400):
+        cases.append((rng.normal(size=n), rng.uniform(0.1, 1.0, size=n - 1)))
+    # decoupled blocks (zero couplings), lowest block in the middle
+    d = rng.normal(size=30)
+    e = rng.uniform(0.2, 1.0, size=29)
+    e[[9, 19]] = 0.0
+    d[10:20] -= 5.0
+    cases.append((d, e))
+    # tiny couplings: nearly degenerate lowest pair, separated from the rest
+    d = np.concatenate([[-3.0, -3.0 + 1e-9], rng.uniform(0, 1, size=20)])
+    e = np.concatenate([[1e-12], rng.uniform(0.1, 0.5, size=20)])
+    cases.append((d, e))
+    # Lanczos-like tridiagonal with large entries (ghost-free part of a chain spectrum)
+    n = 120
+    cases.append((np.full(n, -4.0) + 0.01 * rng.normal(size=n), 2.0 + 0.01 * rng.normal(size=n - 1)))
+    return cases
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_tridiag_lowest_matches_dense_eigh(case):
+    """Lowest Ritz pair of the Lanczos driver (bisection + inverse iteration)
+    against numpy.linalg.eigh of the dense tridiagonal matrix."""
+    d, e = _tri_cases()[case]
+    T = _tri_dense(d, e)
+    w, V = np.linalg.eigh(T)
+    ev, z = qcm.tridiag_lowest(d, e)
+    norm = np.abs(T).sum(axis=1).max()
+    assert abs(ev - w[0]) <= 1e-13 * max(norm, 1.0)
+    assert abs(np.linalg.norm(z) - 1.0) <= 1e-12
+    assert np.linalg.norm(T @ z - ev * z) <= 1e-11 * max(norm, 1.0)
+    if len(w) == 1 or w[1] - w[0] > 1e-6:
+        assert abs(abs(z @ V[:, 0]) - 1.0) <= 1e-10
+        # the component the Lanczos stop rule uses, |z_last|
+        assert abs(abs(z[-1]) - abs(V[-1, 0])) <= 1e-12

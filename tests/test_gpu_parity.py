@@ -95,12 +95,74 @@ def test_factorized_variants_agree(oracle, variant):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("sell16", [True, False])
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("kind", ["chain12", "plaquette8b"])
+def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16):
+    """Spin-up rows of >= 512 states take the sliced-ELL row kernel; a single
+    hopping magnitude (chain) allows the 16-bit sign-only table, several
+    magnitudes (plaquette + bath) the packed 32-bit one."""
+    if kind == "chain12":
+        wl, sec, Nup, Ndw = W.hubbard_chain(12, U=4.0, mu=2.0), "R0:N11:S-1", 5, 6   # du = 792
+    else:
+        wl, sec, Nup, Ndw = W.plaquette_bath(seed=2025), "R0:N12:S0", 6, 6        # du = 924
+    qcm.set_global_parameterC("sell16", sell16)
+    try:
+        lab = _instance(wl, sec)
+        H = qcm.Hamiltonian(lab, sec, "G")
+    finally:
+        qcm.set_global_parameterC("sell16", True)
+    H.set_variant(variant)
+    x = np.random.default_rng(5).normal(size=H.dim)
+    got = H.apply(x)
+    ref = oracle.Model(wl.L, wl.terms()).hv_factorized(Nup, Ndw, x)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_torus4x4_full_sector_hv_matches_oracle(oracle):
+    """The 4x4 periodic cluster at half filling (dim 165 636 900): every kernel
+    schedule against the oracle's OpenMP product, and symmetry of H."""
+    wl = W.torus(4, 4, U=8.0)
+    sec = "R0:N16:S0"
+    lab = _instance(wl, sec)
+    H = qcm.Hamiltonian(lab, sec, "G")
+    assert H.dim == 165636900
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=H.dim)
+    ref = oracle.Model(wl.L, wl.terms()).hv_factorized_omp(8, 8, x, nrep=1)
+    scale = np.abs(ref).max()
+    for variant in (1, 2, 0):
+        H.set_variant(variant)
+        got = H.apply(x)
+        err = np.abs(got - ref).max()
+        print("variant", variant, "max |err| / scale", err / scale, flush=True)
+        assert err <= 1e-12 * scale
+        del got
+    w = rng.normal(size=H.dim)
+    lhs = float(w @ H.apply(x))
+    rhs = float(H.apply(w) @ x)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
 def test_ground_state_chain8_known_answer():
     wl = W.hubbard_chain(8, U=4.0, mu=2.0)
     lab = _instance(wl)
     E0, sec = qcm.ground_state_solve(lab)
     assert abs(E0 - (-20.23580699912965)) < 1e-10 * 20.3
     assert sec.startswith("R0:N8:S0")
+
+
+def test_ground_state_chain8_full_reorth():
+    """Lanczos with full FP64 reorthogonalisation against the stored basis
+    gives the same ground state (dense eigh: -20.23580699912965)."""
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    qcm.set_global_parameterC("lanczos_reorth", True)
+    try:
+        E0, sec = qcm.ground_state_solve(lab)
+    finally:
+        qcm.set_global_parameterC("lanczos_reorth", False)
+    assert abs(E0 - (-20.23580699912965)) <= 1e-10 * 20.2358
 
 
 def test_ground_state_1D_4_known_answer():
