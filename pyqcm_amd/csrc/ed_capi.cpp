@@ -192,11 +192,11 @@ int ed_hybridization_function(int label, int nz, const double *z, int spin_down,
 int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n) {
   return guard([&] {
     Instance &I = get_instance(label);
-    I.ground_state_solve();
-    *n = (int)I.averages.size();
+    const auto &A = I.cluster_averages();
+    *n = (int)A.size();
     for (int k = 0; k < *n && k < cap; ++k) {
-      ave[k] = std::get<1>(I.averages[k]);
-      var[k] = std::get<2>(I.averages[k]);
+      ave[k] = std::get<1>(A[k]);
+      var[k] = std::get<2>(A[k]);
     }
   });
 }

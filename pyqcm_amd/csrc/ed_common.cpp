@@ -45,8 +45,10 @@ Params::Params() {
   c["GF_method"] = 'L';
   // device-side knobs (no reference counterpart)
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
-  i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
+  i["sell_sort_window"] = 256;
+  i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads    // two-pass up table: rows sorted by length within windows of this many states
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
+  b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings

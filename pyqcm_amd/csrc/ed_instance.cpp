@@ -301,7 +301,16 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     E0 += x->weight * x->energy;
     GS_energy = std::min(GS_energy, x->energy);
   }
-  // averages and variances of every instance operator (Hermitian_operator.hpp:67-83)
+  averages.clear();
+  averages_done = false;
+  gs_solved = true;
+  return {GS_energy, gs_string()};
+}
+
+const std::vector<std::tuple<std::string, double, double>> &Instance::cluster_averages() {
+  if (!gs_solved) ground_state_solve();
+  if (averages_done) return averages;
+  // <O> and <O^2> - <O>^2 per operator (Hermitian_operator.hpp:67-83)
   averages.clear();
   for (const auto &kv : value) {
     double ave = 0.0, var = 0.0;
@@ -319,8 +328,8 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     }
     averages.emplace_back(kv.first, ave, var - ave * ave);
   }
-  gs_solved = true;
-  return {GS_energy, gs_string()};
+  averages_done = true;
+  return averages;
 }
 
 // ---------------------------------------------------------------------------
@@ -400,6 +409,7 @@ static void build_gf(Instance &I, State &st, int spin) {
     if (!sector_valid(L, tgt)) continue;
     const size_t dt = sector_dim(L, tgt);
     if (dt == 0) continue;
+    vlog("GF %s: start", tgt.name().c_str());
     auto H = make_hamiltonian(T, tgt, I.format);
     // phi_i = c^{pm}_{i, spin} |Omega>
     std::vector<DevBuf<double>> phi;
@@ -506,6 +516,7 @@ static void build_gf(Instance &I, State &st, int spin) {
 
 void Instance::green_function_solve() {
   if (gf_solved) return;
+  vlog("Green function solve: instance %d", label);
   if (!gs_solved) ground_state_solve();
   for (auto &s : states) {
     build_gf(*this, *s, 0);

@@ -46,9 +46,14 @@ struct Instance {
   double GS_energy = 0.0, E0 = 0.0;
   std::vector<std::unique_ptr<State>> states;
   std::vector<std::tuple<std::string, double, double>> averages;
+  bool averages_done = false;
 
   TermList terms() const;
   std::pair<double, std::string> ground_state_solve();
+  // ground-state averages and variances of every instance operator, computed
+  // on first use (the reference fills them at the end of low_energy_states,
+  // src_ed/model_instance.hpp:341-344; the values are the same)
+  const std::vector<std::tuple<std::string, double, double>> &cluster_averages();
   void green_function_solve();
   // nz frequencies -> G (nz blocks n x n complex column-major, interleaved)
   void green_function(int nz, const double *z, bool spin_down, double *out);

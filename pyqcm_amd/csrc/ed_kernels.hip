@@ -542,15 +542,27 @@ __global__ __launch_bounds__(BS) void k_fold_partials(const double *in, size_t n
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// m rows of nb partials -> m sums, one block per row, fixed order
+// m rows of nb partials -> m sums in one block (fixed order), written to
+// host-mapped memory; then a system-scope fence and the sequence number
+// 'seq' in *flag, which the host polls instead of synchronising the stream
 template <int BS>
-__global__ __launch_bounds__(BS) void k_fold_rows(const double *in, int nb, double *out) {
-  __shared__ double red[BS / kWave];
-  const double *p = in + (size_t)blockIdx.x * nb;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += BS) s += p[i];
-  s = block_sum<BS>(s, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
+__global__ __launch_bounds__(BS) void k_fold_rows_flag(const double *in, int nb, int m, size_t stride, double *out,
+                                                       uint64_t *flag, uint64_t seq) {
+  // wave w folds rows w, w + BS/64, ...: lanes take strided partials, then a
+  // shuffle reduction (fixed order, no block barrier per row)
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int r = w; r < m; r += BS / kWave) {
+    const double *p = in + (size_t)r * stride;
+    double s = 0.0;
+    for (int i = lane; i < nb; i += kWave) s += p[i];
+    s = wave_sum(s);
+    if (lane == 0) out[r] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 template <int BS>
@@ -660,7 +672,7 @@ struct CoefPack { double c[kMaxVec]; };
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, int m, PtrPack b, double *part) {
-  __shared__ double red[BS / kWave];
+  __shared__ double red[kMaxVec][BS / kWave];
   double acc[kMaxVec];
 #pragma unroll
   for (int k = 0; k < kMaxVec; ++k) acc[k] = 0.0;
@@ -670,19 +682,34 @@ __global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, int
     for (int k = 0; k < kMaxVec; ++k)
       if (k < m) acc[k] += av * b.p[k][i];
   }
-  for (int k = 0; k < m; ++k) {
-    const double s = block_sum<BS>(acc[k], red);
-    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = s;
+  // one shuffle reduction per vector, one barrier per block
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kMaxVec; ++k)
+    if (k < m) {
+      const double s = wave_sum(acc[k]);
+      if (lane == 0) red[k][w] = s;
+    }
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < BS / kWave; ++q) s += red[threadIdx.x][q];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
   }
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, const double *x, int m, MPtrPack y, CoefPack c,
-                                                       double *part) {
+__global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, double *x, int m, MPtrPack y, CoefPack c,
+                                                       double *part, double xs) {
   __shared__ double red[BS / kWave];
   double nn = 0.0;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-    const double xv = x[i];
+    double xv = x[i];
+    if (xs != 1.0) {
+      xv *= xs;
+      x[i] = xv;
+    }
 #pragma unroll
     for (int k = 0; k < kMaxVec; ++k)
       if (k < m) {
@@ -1030,17 +1057,26 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
     const uint32_t kneed = (du + BS - 1) / BS;
     if (stage && R == 1 && a.up.sell[0].nslice > 0 && a.up.sell[1].nslice > 0) {
       const int tm = a.up.mode16 ? 2 : (a.up.packed ? 0 : 1);
-#define ED_SELL(DOWN, TMV)                                                    \
+      // short rows: 4-wave blocks, so that all rows of a small sector are resident at once
+      const bool small = a.block == 256 || (a.block == 0 && du < 2048);
+#define ED_SELL(BSZ, DOWN, TMV)                                               \
   do {                                                                        \
-    auto kern = k_hv_rows_sell<BS, DOWN, TMV>;                                \
+    auto kern = k_hv_rows_sell<BSZ, DOWN, TMV>;                               \
     allow_lds(kern, lds);                                                     \
-    hipLaunchKernelGGL(kern, dim3(g), dim3(BS), lds, s, P);                   \
+    hipLaunchKernelGGL(kern, dim3(g), dim3(BSZ), lds, s, P);                  \
   } while (0)
-      if (down_here) {
-        if (tm == 2) ED_SELL(true, 2); else if (tm == 1) ED_SELL(true, 1); else ED_SELL(true, 0);
+#define ED_SELL_TM(BSZ, DOWN)                                                 \
+  do {                                                                        \
+    if (tm == 2) ED_SELL(BSZ, DOWN, 2);                                       \
+    else if (tm == 1) ED_SELL(BSZ, DOWN, 1);                                  \
+    else ED_SELL(BSZ, DOWN, 0);                                               \
+  } while (0)
+      if (small) {
+        if (down_here) ED_SELL_TM(256, true); else ED_SELL_TM(256, false);
       } else {
-        if (tm == 2) ED_SELL(false, 2); else if (tm == 1) ED_SELL(false, 1); else ED_SELL(false, 0);
+        if (down_here) ED_SELL_TM(BS, true); else ED_SELL_TM(BS, false);
       }
+#undef ED_SELL_TM
 #undef ED_SELL
     } else if (stage && R == 1 && kneed <= 16) {
       if (down_here) {
@@ -1141,21 +1177,21 @@ int multi_dot(size_t n, const double *a, int m, const double *const *b, double *
   PtrPack pk{};
   for (int k = 0; k < m; ++k) pk.p[k] = b[k];
   constexpr int BS = 256;
-  const int g = grid_for(n, BS, 4);
+  const int g = grid_for(n, BS, 2);
   hipLaunchKernelGGL((k_multi_dot<BS>), dim3(g), dim3(BS), 0, s, n, a, m, pk, part);
   ED_HIP(hipGetLastError());
   return g;
 }
 
-int multi_axpy_src(size_t n, const double *x, int m, double *const *y, const double *c, double *part,
-                   hipStream_t s) {
+int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c, double *part, hipStream_t s,
+                   double xs) {
   if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_axpy: too many vectors");
   MPtrPack pk{};
   CoefPack cp{};
   for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
   constexpr int BS = 256;
   const int g = grid_for(n, BS, 4);
-  hipLaunchKernelGGL((k_multi_axpy_src<BS>), dim3(g), dim3(BS), 0, s, n, x, m, pk, cp, part);
+  hipLaunchKernelGGL((k_multi_axpy_src<BS>), dim3(g), dim3(BS), 0, s, n, x, m, pk, cp, part, xs);
   ED_HIP(hipGetLastError());
   return g;
 }
@@ -1183,9 +1219,9 @@ void fill(size_t n, double *x, double v, hipStream_t s) {
   hipLaunchKernelGGL((k_fill<256>), dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, x, v);
   ED_HIP(hipGetLastError());
 }
-void fold_rows(const double *part, int nb, int m, double *out, hipStream_t s) {
-  if (m <= 0) return;
-  hipLaunchKernelGGL((k_fold_rows<256>), dim3(m), dim3(256), 0, s, part, nb, out);
+void fold_rows_flag(const double *part, int nb, int m, size_t stride, double *out, uint64_t *flag, uint64_t seq,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((k_fold_rows_flag<1024>), dim3(1), dim3(1024), 0, s, part, nb, m, stride, out, flag, seq);
   ED_HIP(hipGetLastError());
 }
 void random_normal(size_t n, uint64_t seed, double *x, hipStream_t s) {

@@ -60,6 +60,7 @@ struct FactArgs {
   int diag_mode = 0;      // 0 streamed full diagonal, 1 per-site cross U_i, 2 uniform cross U
   const double *dg = nullptr;  // full diagonal (diag_mode 0)
   double Uc[32];          // cross (up-down same-site) coefficients (diag_mode 1, 2)
+  int block = 0;          // row-kernel block size: 0 auto (256 below 2048 up states, else 1024), 256, 1024
 };
 
 enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
@@ -104,17 +105,20 @@ int hv_otf_max_partials(const OtfArgs &a);
 int partials_cap();
 // out[k * nb + block] = partial of <a, b_k>, k < m (m <= 16). returns nb
 int multi_dot(size_t n, const double *a, int m, const double *const *b, double *partials, hipStream_t s);
-// y_k += c_k x for k < m; if norm_partials, partials of ||y_0||^2 after the update. returns nb
-int multi_axpy_src(size_t n, const double *x, int m, double *const *y, const double *c, double *norm_partials,
-                   hipStream_t s);
+// x *= xs (written back when xs != 1), then y_k += c_k x for k < m; if
+// norm_partials, partials of ||y_0||^2 after the update. returns nb
+int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c, double *norm_partials,
+                   hipStream_t s, double xs = 1.0);
 // t = st * t + sum_k c_k b_k ; optional partials of ||t||^2 and <t, w>. returns nb
 int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *norm_partials,
             const double *w, double *dot_partials, hipStream_t s);
 void scale(size_t n, double *x, double a, hipStream_t s);
 void fill(size_t n, double *x, double v, hipStream_t s);
-// out[k] = sum_i part[k * nb + i], k < m (deterministic order); out may be
-// pinned host memory mapped for the device
-void fold_rows(const double *part, int nb, int m, double *out, hipStream_t s);
+// out[r] = sum_{i < nb} part[r * stride + i], r < m, in one block (fixed
+// order); out is host-mapped pinned memory; then *flag = seq (release,
+// system scope) for the host to poll
+void fold_rows_flag(const double *part, int nb, int m, size_t stride, double *out, uint64_t *flag, uint64_t seq,
+                    hipStream_t s);
 // x[i] ~ N(0, 1), deterministic in (seed, i)
 void random_normal(size_t n, uint64_t seed, double *x, hipStream_t s);
 void set_unit(double *x, size_t i, hipStream_t s);

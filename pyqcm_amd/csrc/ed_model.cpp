@@ -3,6 +3,7 @@
 #include "ed_dense.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -192,31 +193,62 @@ static inline int hop_sign(uint32_t s, int from, int to) {
   return (__builtin_popcount(s & inside) & 1) ? -1 : 1;
 }
 
-// Sums of m rows of nb per-block partials.  The rows are folded on the
-// device in a fixed order (deterministic) and the m results written straight
-// into pinned host memory, so one stream synchronisation returns them.
-double sum_partials(const double *dpart, int nb, int m, std::vector<double> *out) {
-  constexpr int kCap = 1024;
-  struct Pinned {
-    double *h = nullptr, *d = nullptr;
-    Pinned() {
-      ED_HIP(hipHostMalloc((void **)&h, kCap * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
-      ED_HIP(hipHostGetDevicePointer((void **)&d, h, 0));
-    }
-  };
-  static thread_local Pinned pin;
-  if (out) out->assign(m, 0.0);
-  if (m <= 0) return 0.0;
-  if (m > kCap) fail(ED_ERR_INTERNAL, "sum_partials: too many rows");
-  if (nb > 0) {
-    fold_rows(dpart, nb, m, pin.d, stream());
-    ED_HIP(hipStreamSynchronize(stream()));
-  } else {
-    for (int k = 0; k < m; ++k) pin.h[k] = 0.0;
+// Reduction results come back through pinned, host-mapped memory: a fold
+// kernel (one block, fixed order) stores the m sums there and then a
+// sequence word.  The host waits for the word by polling (poll_reductions),
+// which is cheaper than a stream synchronisation; the kernel runs after
+// everything queued before it, so seeing the word also means that work is
+// done.
+namespace {
+constexpr int kFoldCap = 1024;
+struct PinnedFold {
+  double *h = nullptr, *d = nullptr;
+  uint64_t *fh = nullptr, *fd = nullptr;
+  uint64_t seq = 0;
+  PinnedFold() {
+    ED_HIP(hipHostMalloc((void **)&h, kFoldCap * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    ED_HIP(hipHostGetDevicePointer((void **)&d, h, 0));
+    ED_HIP(hipHostMalloc((void **)&fh, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    ED_HIP(hipHostGetDevicePointer((void **)&fd, fh, 0));
+    *fh = 0;
   }
-  if (out)
-    for (int k = 0; k < m; ++k) (*out)[k] = pin.h[k];
+};
+}  // namespace
+
+double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std::vector<double> *out) {
+  if (m <= 0) {
+    if (out) out->clear();
+    return 0.0;
+  }
+  if (nb <= 0) {
+    if (out) out->assign(m, 0.0);
+    return 0.0;
+  }
+  if (m > kFoldCap) fail(ED_ERR_INTERNAL, "sum_partials: too many rows");
+  static thread_local PinnedFold pin;
+  const uint64_t want = ++pin.seq;
+  fold_rows_flag(dpart, nb, m, stride, pin.d, pin.fd, want, stream());
+  if (params().getb("poll_reductions")) {
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned spins = 0;
+    while (__atomic_load_n(pin.fh, __ATOMIC_ACQUIRE) != want) {
+      if ((++spins & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+        // long wait: let the runtime block (and report a failed stream)
+        ED_HIP(hipStreamSynchronize(stream()));
+        if (__atomic_load_n(pin.fh, __ATOMIC_ACQUIRE) != want)
+          fail(ED_ERR_INTERNAL, "reduction result did not arrive after the stream completed");
+        break;
+      }
+    }
+  } else {
+    ED_HIP(hipStreamSynchronize(stream()));
+  }
+  if (out) out->assign(pin.h, pin.h + m);
   return pin.h[0];
+}
+
+double sum_partials(const double *dpart, int nb, int m, std::vector<double> *out) {
+  return sum_partials_rows(dpart, nb, m, (size_t)nb, out);
 }
 
 double Hamiltonian::dot_result(int nb) { return sum_partials(partials.p, nb); }
@@ -553,6 +585,7 @@ struct HamFactorized : Hamiltonian {
     }
     partials.alloc(partials_cap());
     variant = (int)params().geti("hv_variant");
+    args.block = (int)params().geti("hv_block");
   }
   // candidate schedules, in the order reported by ed_hamiltonian_autotune
   static std::vector<std::pair<int, SlabConfig>> candidates() {
@@ -820,6 +853,7 @@ std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &s
   for (const auto &tc : T.terms)
     if (tc.first->spin_flip && tc.second != 0.0)
       fail(ED_ERR_UNSUPPORTED, "operator " + tc.first->name + " flips spin: spin-nonconserving sectors are not supported");
+  vlog("Hamiltonian %s (format %c): build", sec.name().c_str(), format);
   std::unique_ptr<Hamiltonian> H;
   if (format == 'G' || format == 'F') {
     if (factorizable(T)) H.reset(new HamFactorized(T, sec));
@@ -833,6 +867,7 @@ std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &s
   }
   H->sec = sec;
   H->terms = T;
+  vlog("Hamiltonian %s: dim %zu ready", sec.name().c_str(), H->dim);
   return H;
 }
 

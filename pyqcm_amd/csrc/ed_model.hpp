@@ -123,5 +123,7 @@ void symeig(int n, const double *A, double *w, double *V /* may be null */);
 
 // sum of per-block partials read back from the device
 double sum_partials(const double *dpart, int nb, int m = 1, std::vector<double> *out = nullptr);
+// the same with row r at dpart + r * stride
+double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std::vector<double> *out);
 
 }  // namespace ed

@@ -44,24 +44,37 @@ std::vector<double> dots(size_t n, const double *a, const std::vector<const doub
 // t = st t + sum c_k b_k (chunks); returns ||t||^2 if want_norm
 double comb(size_t n, double *t, double st, const std::vector<const double *> &b, const std::vector<double> &c,
             bool want_norm, const double *w = nullptr, double *dot_w = nullptr) {
+  // partials: the norm in row 0 and the dot in row 1 (stride partials_cap),
+  // folded together by one reduction
+  auto run = [&](double stv, int m, const double *const *bb, const double *cc, bool wn, bool wd) -> double {
+    if (!wn && !wd) {
+      combine(n, t, stv, m, bb, cc, nullptr, nullptr, nullptr, stream());
+      return 0.0;
+    }
+    double *np = work().part.p, *dp = work().part.p + partials_cap();
+    const int nb = combine(n, t, stv, m, bb, cc, np, wd ? w : nullptr, wd ? dp : nullptr, stream());
+    if (!wn) {
+      *dot_w = sum_partials(dp, nb);
+      return 0.0;
+    }
+    if (!wd) return sum_partials(np, nb);
+    std::vector<double> r;
+    sum_partials_rows(np, nb, 2, partials_cap(), &r);
+    *dot_w = r[1];
+    return r[0];
+  };
   if (b.empty()) {
-    if (st != 1.0) scale(n, t, st, stream());
-    if (!want_norm && !dot_w) return 0.0;
-    const int nb = combine(n, t, 1.0, 0, nullptr, nullptr, work().part.p, w, w ? work().part.p + partials_cap() : nullptr,
-                           stream());
-    const double nn = sum_partials(work().part.p, nb);
-    if (dot_w) *dot_w = sum_partials(work().part.p + partials_cap(), nb);
-    return nn;
+    if (!want_norm && !dot_w) {
+      if (st != 1.0) scale(n, t, st, stream());
+      return 0.0;
+    }
+    return run(st, 0, nullptr, nullptr, want_norm, dot_w != nullptr);
   }
   double nn = 0.0;
   for (size_t s = 0; s < b.size(); s += 16) {
     const int m = (int)std::min<size_t>(16, b.size() - s);
     const bool last = s + 16 >= b.size();
-    const bool wn = last && want_norm, wd = last && dot_w;
-    const int nb = combine(n, t, s == 0 ? st : 1.0, m, b.data() + s, c.data() + s, wn ? work().part.p : nullptr,
-                           wd ? w : nullptr, wd ? work().part.p + partials_cap() : nullptr, stream());
-    if (wn) nn = sum_partials(work().part.p, nb);
-    if (wd) *dot_w = sum_partials(work().part.p + partials_cap(), nb);
+    nn = run(s == 0 ? st : 1.0, m, b.data() + s, c.data() + s, last && want_norm, last && dot_w != nullptr);
   }
   return nn;
 }
@@ -261,10 +274,18 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
   }
   int nd = 0, j;
   double ev_old = 1e12;
-  double znext = -1.0;  // ||v[j]||^2 carried from the previous step when known
   for (j = 0; j < M0; ++j) {
-    const double z = std::sqrt(znext >= 0.0 ? znext : ddot(dim, v[j]->p, v[j]->p));
-    znext = -1.0;
+    // one multi-dot per step: ||v_j||^2, <v_j, phi_k> (P matrix), the step (6)
+    // overlaps <v_j, v_k> (k = j+1 .. j+pc-1) and <v_j, v_k> for the vectors
+    // step (8) subtracts (k0 <= k < j), all against the unnormalized v_j
+    const int jm = j + pc;
+    const int k0 = (j > pc) ? j - pc : 0;
+    std::vector<const double *> db{v[j]->p};
+    for (int k = 0; k < p; ++k) db.push_back(phi[k]);
+    for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
+    for (int k = k0; k < j; ++k) db.push_back(v[k]->p);
+    const std::vector<double> dv = dots(dim, v[j]->p, db);
+    const double z = std::sqrt(std::max(dv[0], 0.0));
     if (z < accur_deflation) {
       if (j >= pc) {
         Idef[nd] = j - pc;
@@ -279,52 +300,50 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
       j--;
       continue;
     }
-    scale(dim, v[j]->p, 1.0 / z, stream());
+    const double iz = 1.0 / z;
     if (j >= pc) T(j, j - pc) = z;
-    // P(k, j) = <phi_k, v_j> and step (6) dots <v_j, v_k>, k = j+1 .. j+pc-1
-    const int jm = j + pc;
-    std::vector<const double *> db;
-    for (int k = 0; k < p; ++k) db.push_back(phi[k]);
-    for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
-    std::vector<double> dv = dots(dim, v[j]->p, db);
-    for (int k = 0; k < p; ++k) Pmat[(size_t)k + (size_t)j * p] = dv[k];
+    for (int k = 0; k < p; ++k) Pmat[(size_t)k + (size_t)j * p] = dv[1 + k] * iz;
     if (jm > j + 1) {
       std::vector<double *> ys;
       std::vector<double> cs;
       for (int k = j + 1; k < jm; ++k) {
-        const double zz = dv[p + (k - j - 1)];
+        const double zz = dv[1 + p + (k - j - 1)] * iz;
         ys.push_back(v[k]->p);
         cs.push_back(-zz);
         if (k >= pc) T(j, k - pc) = zz;
       }
+      // v_j /= z in the same pass (only the first chunk scales)
       for (size_t s0 = 0; s0 < ys.size(); s0 += 16) {
         const int m = (int)std::min<size_t>(16, ys.size() - s0);
-        const bool first = s0 == 0;
-        const int nb = multi_axpy_src(dim, v[j]->p, m, ys.data() + s0, cs.data() + s0,
-                                      first ? work().part.p : nullptr, stream());
-        if (first) znext = sum_partials(work().part.p, nb);  // ||v[j+1]||^2 after step (6)
+        multi_axpy_src(dim, v[j]->p, m, ys.data() + s0, cs.data() + s0, nullptr, stream(), s0 == 0 ? iz : 1.0);
       }
+    } else {
+      scale(dim, v[j]->p, iz, stream());
     }
-    // step (7): v[j+pc] = H v_j
+    // step (7): v[j+pc] = H v_j, with <H v_j, v_j>
     v[j + pc].reset(new DevBuf<double>(dim));
-    H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, false);
-    int k0 = 0;
-    if (j > pc) {
-      k0 = j - pc;
-      if (def[k0 - 1] == 0) v[k0 - 1].reset();
-    }
-    // step (8): v[j+pc] -= sum_{k0 <= k < j} t(k, j) v_k, with <v[j+pc], v_j> when nothing follows
+    const int hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
+    const double hd = H.dot_result(hnb);
+    if (j > pc && def[k0 - 1] == 0) v[k0 - 1].reset();
+    // step (8): v[j+pc] -= sum_{k0 <= k < j} t(k, j) v_k
     std::vector<const double *> src;
     std::vector<double> cf;
+    double wdot = hd;
     for (int k = k0; k < j; ++k) {
       T(k, j) = T(j, k);
       src.push_back(v[k]->p);
       cf.push_back(-T(k, j));
+      wdot -= T(k, j) * dv[1 + p + (jm - j - 1) + (k - k0)] * iz;
     }
-    double wdot = 0.0;
-    comb(dim, v[j + pc]->p, 1.0, src, cf, false, v[j]->p, &wdot);
-    // step (9): deflated vectors (classical Gram-Schmidt over the mutually orthogonal set)
-    if (nd > 0) {
+    if (nd == 0) {
+      // t(j, j) = <v[j+pc], v_j> after step (8), from the dots above; the
+      // subtraction and the projection on v_j are one pass
+      src.push_back(v[j]->p);
+      cf.push_back(-wdot);
+      comb(dim, v[j + pc]->p, 1.0, src, cf, false);
+    } else {
+      comb(dim, v[j + pc]->p, 1.0, src, cf, false, v[j]->p, &wdot);
+      // step (9): deflated vectors (classical Gram-Schmidt over the mutually orthogonal set)
       std::vector<const double *> dvv;
       for (int k = 0; k < nd; ++k) dvv.push_back(v[Idef[k]]->p);
       std::vector<double> w = dots(dim, v[j + pc]->p, dvv);
@@ -334,13 +353,9 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
         T(Idef[k], j) = w[k];
       }
       comb(dim, v[j + pc]->p, 1.0, dvv, c2, false, v[j]->p, &wdot);
+      comb(dim, v[j + pc]->p, 1.0, {v[j]->p}, {-wdot}, false);
     }
-    // final projection on v_j
-    {
-      const double nn = comb(dim, v[j + pc]->p, 1.0, {v[j]->p}, {-wdot}, true);
-      T(j, j) = wdot;
-      if (pc == 1) znext = nn;  // v[j+1] is the vector just built
-    }
+    T(j, j) = wdot;
     for (int k = 0; k < nd; ++k) T(j, Idef[k]) = T(Idef[k], j);
     // step (12): convergence of the lowest eigenvalue of the projected matrix
     if (j >= 4 * p && j % p == 0) {

@@ -36,13 +36,14 @@ _PARAM_TYPES = {
     "verb_ED": _BOOL, "verb_Hilbert": _BOOL, "verb_warning": _BOOL, "parallel_sectors": _BOOL,
     "merge_states": _BOOL, "combine_mcf": _BOOL, "block_Lanczos_QR": _BOOL, "store_lanczos_basis": _BOOL,
     "sell16": _BOOL, "lanczos_reorth": _BOOL, "host_random_start": _BOOL,
+    "poll_reductions": _BOOL,
     "accur_band_lanczos": _DOUBLE, "accur_continued_fraction": _DOUBLE, "accur_Davidson": _DOUBLE,
     "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
     "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
     "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
     "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
-    "sell_sort_window": _INT,
+    "sell_sort_window": _INT, "hv_block": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 

@@ -95,10 +95,11 @@ def test_factorized_variants_agree(oracle, variant):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("block", [0, 1024])
 @pytest.mark.parametrize("sell16", [True, False])
 @pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("kind", ["chain12", "plaquette8b"])
-def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16):
+def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16, block):
     """Spin-up rows of >= 512 states take the sliced-ELL row kernel; a single
     hopping magnitude (chain) allows the 16-bit sign-only table, several
     magnitudes (plaquette + bath) the packed 32-bit one."""
@@ -107,11 +108,13 @@ def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16):
     else:
         wl, sec, Nup, Ndw = W.plaquette_bath(seed=2025), "R0:N12:S0", 6, 6        # du = 924
     qcm.set_global_parameterC("sell16", sell16)
+    qcm.set_global_parameterC("hv_block", block)
     try:
         lab = _instance(wl, sec)
         H = qcm.Hamiltonian(lab, sec, "G")
     finally:
         qcm.set_global_parameterC("sell16", True)
+        qcm.set_global_parameterC("hv_block", 0)
     H.set_variant(variant)
     x = np.random.default_rng(5).normal(size=H.dim)
     got = H.apply(x)
@@ -144,10 +147,17 @@ def test_torus4x4_full_sector_hv_matches_oracle(oracle):
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
-def test_ground_state_chain8_known_answer():
+@pytest.mark.parametrize("poll", [True, False])
+def test_ground_state_chain8_known_answer(poll):
+    """Dense eigh: -20.23580699912965; reductions returned by polling pinned
+    memory or by stream synchronisation."""
     wl = W.hubbard_chain(8, U=4.0, mu=2.0)
     lab = _instance(wl)
-    E0, sec = qcm.ground_state_solve(lab)
+    qcm.set_global_parameterC("poll_reductions", poll)
+    try:
+        E0, sec = qcm.ground_state_solve(lab)
+    finally:
+        qcm.set_global_parameterC("poll_reductions", True)
     assert abs(E0 - (-20.23580699912965)) < 1e-10 * 20.3
     assert sec.startswith("R0:N8:S0")
 
@@ -170,6 +180,25 @@ def test_ground_state_1D_4_known_answer():
     lab = _instance(wl)
     E0, sec = qcm.ground_state_solve(lab)
     assert round(E0, 7) == -9.9531453   # examples/1D_4/GS.tsv:2
+
+
+def test_cluster_averages_1D_4_known_answer():
+    """examples/1D_4/averages.tsv:2 (U=4, mu=2, t=1): averages and variances
+    per site.  The lattice layer scales the cluster operators by their norms
+    (1/4 for t and U, -1/4 for mu on the 4-site cluster); the ED-level values
+    are <O> and <O^2> - <O>^2 of the operators as defined."""
+    wl = W.hubbard_chain(4, U=4.0, mu=2.0)
+    lab = _instance(wl)
+    E0, _ = qcm.ground_state_solve(lab)
+    ave = qcm.cluster_averages(lab)
+    norm = {"t": 0.25, "U": 0.25, "mu": -0.25}
+    ref = {"U": (0.084896413, 0.01648638), "mu": (1.0, 0.0), "t": (-0.82787198, 0.26378208)}
+    for name, (a, v) in ref.items():
+        got_a, got_v = ave[name]
+        assert abs(got_a * norm[name] - a) <= 1e-8, name
+        assert abs(got_v * norm[name] ** 2 - v) <= 1e-8, name
+    # the energy is the sum of value * <O> over the instance operators
+    assert abs(sum(wl.values[n] * ave[n][0] for n in ave) - E0) <= 1e-10 * abs(E0)
 
 
 @pytest.mark.parametrize("fmt", ["G", "S", "N"])
@@ -251,6 +280,33 @@ def test_green_function_band_lanczos_chain8(oracle):
     E0r, Gr = _oracle_gf_band(oracle, wl, 4, 4, z, x)
     assert abs(E0 - E0r) < 1e-10 * abs(E0r)
     assert np.abs(G - Gr.transpose(0, 2, 1)).max() < 1e-8
+
+
+def test_band_lanczos_deflation_matches_exact(oracle):
+    """Band Lanczos run to exhaustion on a 56-state sector with a dependent
+    start vector: initial and late deflations.  G(z) from (e, Q) equals the
+    exact resolvent phi^T (z - H)^-1 phi and the oracle's."""
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    lab = _instance(wl, "R0:N3:S3")
+    H = qcm.Hamiltonian(lab, "R0:N3:S3", "G")
+    assert H.dim == 56
+    rng = np.random.default_rng(21)
+    phi = rng.normal(size=(5, 56))
+    phi[4] = phi[0] - 2.0 * phi[1]
+    qcm.set_global_parameterC("accur_band_lanczos", 0.0)
+    try:
+        e, Q = H.band_lanczos(phi)
+    finally:
+        qcm.set_global_parameterC("accur_band_lanczos", 1e-12)
+    eo, Qo = oracle.Model(wl.L, wl.terms()).band_lanczos(3, 0, phi, accur_bl=0.0)
+    Hd = dense_hamiltonian(wl.L, wl.terms(), 3, 0)
+    z = W.matsubara(16, 10.0)
+    for zz in z:
+        G = (Q / (zz - e)) @ Q.T
+        Go = (Qo / (zz - eo)) @ Qo.T
+        Gx = phi @ np.linalg.solve(zz * np.eye(56) - Hd, phi.T)
+        assert np.abs(G - Gx).max() < 1e-10 * np.abs(Gx).max()
+        assert np.abs(G - Go).max() < 1e-10 * np.abs(Gx).max()
 
 
 def _cf_reference(oracle, wl, E0r, gs, z):
