@@ -442,9 +442,10 @@ static void build_gf(Instance &I, State &st, int spin) {
           std::vector<double> cf{1.0};
           if (o2 != o1) { src.push_back(phi[o2].p); cf.push_back(1.0); }
           // psi = phi_o1 (+ phi_o2), norm^2
-          const int nb = combine(dt, psi.p, 0.0, (int)src.size(), src.data(), cf.data(), work_partials(), nullptr,
-                                 nullptr, stream());
-          const double nrm = sum_partials(work_partials(), nb);
+          double *pp = partials_acquire();
+          const int nb = combine(dt, psi.p, 0.0, (int)src.size(), src.data(), cf.data(), pp, nullptr, nullptr,
+                                 stream());
+          const double nrm = sum_partials(pp, nb);
           if (!(nrm > 1e-14)) continue;
           scale(dt, psi.p, 1.0 / std::sqrt(nrm), stream());
           std::vector<double> a, b;

@@ -3,6 +3,7 @@
 #include "ed_dense.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -193,65 +194,158 @@ static inline int hop_sign(uint32_t s, int from, int to) {
   return (__builtin_popcount(s & inside) & 1) ? -1 : 1;
 }
 
-// Reduction results come back through pinned, host-mapped memory: a fold
-// kernel (one block, fixed order) stores the m sums there and then a
-// sequence word.  The host waits for the word by polling (poll_reductions),
-// which is cheaper than a stream synchronisation; the kernel runs after
-// everything queued before it, so seeing the word also means that work is
-// done.
+// Reductions.  A kernel that reduces writes one partial per block (and per
+// row) into a slot of pinned, host-mapped memory.  Every slot entry holds a
+// sentinel (a signalling-NaN bit pattern that FP64 arithmetic never
+// produces) until the device overwrites it; the host polls until no entry of
+// the rows it expects is the sentinel, sums each row in a fixed order
+// (pairwise), restores the sentinels and releases the slot.  No fold kernel
+// and no stream synchronisation is needed (poll_reductions; otherwise the
+// stream is synchronised first).  Each 8-byte store is atomic, so no fence
+// is needed on the device side.
 namespace {
-constexpr int kFoldCap = 1024;
-struct PinnedFold {
+constexpr uint64_t kSentinel = 0x7ff4dead5eed0001ull;
+constexpr int kSlots = 8;
+struct PartialArena {
+  size_t slot_len = 0;  // doubles per slot
   double *h = nullptr, *d = nullptr;
-  uint64_t *fh = nullptr, *fd = nullptr;
-  uint64_t seq = 0;
-  PinnedFold() {
-    ED_HIP(hipHostMalloc((void **)&h, kFoldCap * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+  bool busy[kSlots] = {};
+  int next = 0;
+  PartialArena() {
+    slot_len = (size_t)partials_cap() * 16;
+    ED_HIP(hipHostMalloc((void **)&h, kSlots * slot_len * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
     ED_HIP(hipHostGetDevicePointer((void **)&d, h, 0));
-    ED_HIP(hipHostMalloc((void **)&fh, 64, hipHostMallocMapped | hipHostMallocCoherent));
-    ED_HIP(hipHostGetDevicePointer((void **)&fd, fh, 0));
-    *fh = 0;
+    uint64_t *u = reinterpret_cast<uint64_t *>(h);
+    for (size_t i = 0; i < kSlots * slot_len; ++i) u[i] = kSentinel;
+  }
+  int slot_of(const double *p) const {
+    if (p < d || p >= d + kSlots * slot_len) return -1;
+    return (int)((size_t)(p - d) / slot_len);
   }
 };
+PartialArena &arena() {
+  static thread_local PartialArena a;
+  return a;
+}
+inline bool is_sentinel(const double *p) {
+  uint64_t u;
+  std::memcpy(&u, (const void *)p, 8);
+  return u == kSentinel;
+}
+double pairwise(std::vector<double> &w) {
+  size_t n = w.size();
+  while (n > 1) {
+    const size_t half = (n + 1) / 2;
+    for (size_t i = 0; i + half < n; ++i) w[i] += w[i + half];
+    n = half;
+  }
+  return n ? w[0] : 0.0;
+}
 }  // namespace
 
+double *partials_acquire() {
+  PartialArena &A = arena();
+  for (int t = 0; t < kSlots; ++t) {
+    const int s = (A.next + t) % kSlots;
+    if (!A.busy[s]) {
+      A.busy[s] = true;
+      A.next = (s + 1) % kSlots;
+      return A.d + (size_t)s * A.slot_len;
+    }
+  }
+  // every slot taken: only possible after reductions abandoned by an error
+  // path; once the stream is idle, restore all sentinels and start over
+  ED_HIP(hipStreamSynchronize(stream()));
+  uint64_t *u = reinterpret_cast<uint64_t *>(A.h);
+  for (size_t i = 0; i < kSlots * A.slot_len; ++i) u[i] = kSentinel;
+  for (int s = 0; s < kSlots; ++s) A.busy[s] = false;
+  A.busy[0] = true;
+  A.next = 1;
+  return A.d;
+}
+
 double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std::vector<double> *out) {
-  if (m <= 0) {
-    if (out) out->clear();
-    return 0.0;
+  PartialArena &A = arena();
+  const int slot = A.slot_of(dpart);
+  if (out) out->assign(std::max(m, 0), 0.0);
+  if (slot < 0) {
+    // device partials: fold on the host after a copy
+    if (m <= 0 || nb <= 0) return 0.0;
+    std::vector<double> hbuf((size_t)(m - 1) * stride + nb);
+    ED_HIP(hipMemcpyAsync(hbuf.data(), dpart, hbuf.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    ED_HIP(hipStreamSynchronize(stream()));
+    double first = 0.0;
+    for (int r = 0; r < m; ++r) {
+      std::vector<double> w(hbuf.begin() + (size_t)r * stride, hbuf.begin() + (size_t)r * stride + nb);
+      const double s = pairwise(w);
+      if (out) (*out)[r] = s;
+      if (r == 0) first = s;
+    }
+    return first;
   }
-  if (nb <= 0) {
-    if (out) out->assign(m, 0.0);
-    return 0.0;
-  }
-  if (m > kFoldCap) fail(ED_ERR_INTERNAL, "sum_partials: too many rows");
-  static thread_local PinnedFold pin;
-  const uint64_t want = ++pin.seq;
-  fold_rows_flag(dpart, nb, m, stride, pin.d, pin.fd, want, stream());
-  if (params().getb("poll_reductions")) {
+  const double *hp = A.h + (dpart - A.d);
+  if (m > 0 && nb > 0) {
+    if ((size_t)(dpart - A.d) - (size_t)slot * A.slot_len + (size_t)(m - 1) * stride + nb > A.slot_len)
+      fail(ED_ERR_INTERNAL, "reduction larger than a partials slot");
+    bool synced = !params().getb("poll_reductions");
+    if (synced) ED_HIP(hipStreamSynchronize(stream()));
     const auto t0 = std::chrono::steady_clock::now();
     unsigned spins = 0;
-    while (__atomic_load_n(pin.fh, __ATOMIC_ACQUIRE) != want) {
-      if ((++spins & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
-        // long wait: let the runtime block (and report a failed stream)
-        ED_HIP(hipStreamSynchronize(stream()));
-        if (__atomic_load_n(pin.fh, __ATOMIC_ACQUIRE) != want)
-          fail(ED_ERR_INTERNAL, "reduction result did not arrive after the stream completed");
-        break;
+    for (int r = 0; r < m; ++r) {
+      const double *row = hp + (size_t)r * stride;
+      for (int i = 0; i < nb; ++i) {
+        while (is_sentinel(row + i)) {
+          if (synced) fail(ED_ERR_INTERNAL, "reduction partial missing after the stream completed");
+          if ((++spins & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+            // long wait: let the runtime block (and report a failed stream)
+            ED_HIP(hipStreamSynchronize(stream()));
+            synced = true;
+          }
+        }
       }
     }
-  } else {
-    ED_HIP(hipStreamSynchronize(stream()));
+    std::atomic_thread_fence(std::memory_order_acquire);
   }
-  if (out) out->assign(pin.h, pin.h + m);
-  return pin.h[0];
+  double first = 0.0;
+  for (int r = 0; r < m; ++r) {
+    double *row = const_cast<double *>(hp) + (size_t)r * stride;
+    std::vector<double> w(row, row + std::max(nb, 0));
+    const double s = pairwise(w);
+    if (out) (*out)[r] = s;
+    if (r == 0) first = s;
+    uint64_t *u = reinterpret_cast<uint64_t *>(row);
+    for (int i = 0; i < nb; ++i) u[i] = kSentinel;
+  }
+  A.busy[slot] = false;
+  return first;
 }
 
 double sum_partials(const double *dpart, int nb, int m, std::vector<double> *out) {
-  return sum_partials_rows(dpart, nb, m, (size_t)nb, out);
+  return sum_partials_rows(dpart, nb, m, (size_t)std::max(nb, 0), out);
 }
 
-double Hamiltonian::dot_result(int nb) { return sum_partials(partials.p, nb); }
+Hamiltonian::~Hamiltonian() {
+  if (pending) {
+    try {
+      sum_partials(pending, pending_nb);
+    } catch (...) {
+    }
+  }
+}
+
+double *Hamiltonian::take_partials() {
+  if (pending) sum_partials(pending, pending_nb);  // an unread result: wait for it and drop it
+  pending = partials_acquire();
+  pending_nb = 0;
+  return pending;
+}
+
+double Hamiltonian::dot_result(int nb) {
+  if (!pending) fail(ED_ERR_INTERNAL, "dot_result without a pending product");
+  double *p = pending;
+  pending = nullptr;
+  return sum_partials(p, nb);
+}
 
 // ---------------------------------------------------------------------------
 // per-spin merged symmetric one-body table + diagonal (host build)
@@ -583,7 +677,6 @@ struct HamFactorized : Hamiltonian {
       dg.upload(host_full_diag(hu, hd, *BU, *BD, W, L));
       args.dg = dg.p;
     }
-    partials.alloc(partials_cap());
     variant = (int)params().geti("hv_variant");
     args.block = (int)params().geti("hv_block");
   }
@@ -639,7 +732,10 @@ struct HamFactorized : Hamiltonian {
       v = best_variant;
       sc = best_sc;
     }
-    return hv_factorized_cfg(args, x, y, ax, beta, want_dot ? partials.p : nullptr, v, sc, stream());
+    double *part = want_dot ? take_partials() : nullptr;
+    const int nb = hv_factorized_cfg(args, x, y, ax, beta, part, v, sc, stream());
+    if (part) pending_nb = nb;
+    return nb;
   }
   void diag(double *d) override { fact_full_diag(args, d, stream()); }
 };
@@ -754,10 +850,12 @@ struct HamCSR : Hamiltonian {
     args.col = col.p;
     args.val = val.p;
     args.diag = dgv.p;
-    partials.alloc(partials_cap());
   }
   int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
-    return hv_csr(args, x, y, ax, beta, want_dot ? partials.p : nullptr, stream());
+    double *part = want_dot ? take_partials() : nullptr;
+    const int nb = hv_csr(args, x, y, ax, beta, part, stream());
+    if (part) pending_nb = nb;
+    return nb;
   }
   void diag(double *d) override {
     if (dim) ED_HIP(hipMemcpyAsync(d, dgv.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));
@@ -816,10 +914,12 @@ struct HamOTF : Hamiltonian {
     args.hops = hops.p;
     args.dg = dg.p;
     args.binom = binom.p;
-    partials.alloc(partials_cap());
   }
   int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
-    return hv_otf(args, x, y, ax, beta, want_dot ? partials.p : nullptr, stream());
+    double *part = want_dot ? take_partials() : nullptr;
+    const int nb = hv_otf(args, x, y, ax, beta, part, stream());
+    if (part) pending_nb = nb;
+    return nb;
   }
   void diag(double *d) override {
     if (dim) ED_HIP(hipMemcpyAsync(d, dg.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));

@@ -92,14 +92,18 @@ struct Hamiltonian {
   char format = 'G';
   int variant = 0;
   TermList terms;           // the instance terms this plan realizes
-  DevBuf<double> partials;  // per-block dot partials
-  virtual ~Hamiltonian() = default;
+  virtual ~Hamiltonian();
   // y = ax H x + beta y on device; returns the number of per-block partials of
-  // <y_new, x> written to `partials` when want_dot
+  // <y_new, x> written to a partials slot (take_partials) when want_dot
   virtual int apply(const double *x, double *y, double ax, double beta, bool want_dot) = 0;
   // diagonal of H on device
   virtual void diag(double *d) = 0;
-  double dot_result(int nb);  // sum of the partials (deterministic order)
+  double dot_result(int nb);  // sum of the partials of the last apply (deterministic order)
+ protected:
+  // slot for the partials of the apply being launched (an unread earlier one is discarded)
+  double *take_partials();
+  double *pending = nullptr;
+  int pending_nb = 0;
 };
 
 std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format);
@@ -121,7 +125,11 @@ void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::
 // dense symmetric eigensystem, A column-major n x n, eigenvalues ascending
 void symeig(int n, const double *A, double *w, double *V /* may be null */);
 
-// sum of per-block partials read back from the device
+// Per-block partials of a reduction go to a slot of pinned host memory
+// (partials_acquire: up to 16 rows of partials_cap() doubles); sum_partials
+// waits for them, sums each row in a fixed order and releases the slot.
+// Other (device) pointers are folded by a kernel.
+double *partials_acquire();
 double sum_partials(const double *dpart, int nb, int m = 1, std::vector<double> *out = nullptr);
 // the same with row r at dpart + r * stride
 double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std::vector<double> *out);

@@ -15,15 +15,6 @@ namespace ed {
 
 namespace {
 
-struct Work {
-  DevBuf<double> part;
-  Work() { part.alloc((size_t)partials_cap() * 16); }
-};
-Work &work() {
-  static thread_local Work w;
-  return w;
-}
-
 void d2d(double *dst, const double *src, size_t n) {
   if (n) ED_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
 }
@@ -33,9 +24,10 @@ std::vector<double> dots(size_t n, const double *a, const std::vector<const doub
   std::vector<double> out(b.size(), 0.0);
   for (size_t s = 0; s < b.size(); s += 16) {
     const int m = (int)std::min<size_t>(16, b.size() - s);
-    const int nb = multi_dot(n, a, m, b.data() + s, work().part.p, stream());
+    double *pp = partials_acquire();
+    const int nb = multi_dot(n, a, m, b.data() + s, pp, stream());
     std::vector<double> r;
-    sum_partials(work().part.p, nb, m, &r);
+    sum_partials(pp, nb, m, &r);
     for (int k = 0; k < m; ++k) out[s + k] = r[k];
   }
   return out;
@@ -51,7 +43,7 @@ double comb(size_t n, double *t, double st, const std::vector<const double *> &b
       combine(n, t, stv, m, bb, cc, nullptr, nullptr, nullptr, stream());
       return 0.0;
     }
-    double *np = work().part.p, *dp = work().part.p + partials_cap();
+    double *np = partials_acquire(), *dp = np + partials_cap();
     const int nb = combine(n, t, stv, m, bb, cc, np, wd ? w : nullptr, wd ? dp : nullptr, stream());
     if (!wn) {
       *dot_w = sum_partials(dp, nb);
@@ -81,12 +73,11 @@ double comb(size_t n, double *t, double st, const std::vector<const double *> &b
 
 }  // namespace
 
-double *work_partials() { return work().part.p; }
-
 double ddot(size_t n, const double *a, const double *b) {
   const double *bb[1] = {b};
-  const int nb = multi_dot(n, a, 1, bb, work().part.p, stream());
-  return sum_partials(work().part.p, nb);
+  double *pp = partials_acquire();
+  const int nb = multi_dot(n, a, 1, bb, pp, stream());
+  return sum_partials(pp, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -284,9 +275,24 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     for (int k = 0; k < p; ++k) db.push_back(phi[k]);
     for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
     for (int k = k0; k < j; ++k) db.push_back(v[k]->p);
-    const std::vector<double> dv = dots(dim, v[j]->p, db);
+    // multi-dot and H v_j (on the unnormalized v_j) are queued together; the
+    // host then waits once for both results
+    std::vector<const double *> db1(db.begin(), db.begin() + std::min<size_t>(db.size(), 16));
+    double *pdot = partials_acquire();
+    const int dnb = multi_dot(dim, v[j]->p, (int)db1.size(), db1.data(), pdot, stream());
+    v[j + pc].reset(new DevBuf<double>(dim));
+    const int hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
+    std::vector<double> dv;
+    sum_partials(pdot, dnb, (int)db1.size(), &dv);
+    if (db.size() > db1.size()) {  // more than 16 vectors (p > 5): remaining dots
+      std::vector<const double *> rest(db.begin() + 16, db.end());
+      const std::vector<double> dr = dots(dim, v[j]->p, rest);
+      dv.insert(dv.end(), dr.begin(), dr.end());
+    }
+    const double hd_raw = H.dot_result(hnb);  // <H v_j, v_j> before normalization
     const double z = std::sqrt(std::max(dv[0], 0.0));
     if (z < accur_deflation) {
+      v[j + pc].reset();
       if (j >= pc) {
         Idef[nd] = j - pc;
         def[j - pc] = 1;
@@ -320,10 +326,9 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     } else {
       scale(dim, v[j]->p, iz, stream());
     }
-    // step (7): v[j+pc] = H v_j, with <H v_j, v_j>
-    v[j + pc].reset(new DevBuf<double>(dim));
-    const int hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
-    const double hd = H.dot_result(hnb);
+    // step (7): v[j+pc] = H v_j for the normalized v_j: the product above
+    // times 1/z, applied in the combination below
+    const double hd = hd_raw * iz * iz;
     if (j > pc && def[k0 - 1] == 0) v[k0 - 1].reset();
     // step (8): v[j+pc] -= sum_{k0 <= k < j} t(k, j) v_k
     std::vector<const double *> src;
@@ -337,12 +342,12 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     }
     if (nd == 0) {
       // t(j, j) = <v[j+pc], v_j> after step (8), from the dots above; the
-      // subtraction and the projection on v_j are one pass
+      // scaling, the subtraction and the projection on v_j are one pass
       src.push_back(v[j]->p);
       cf.push_back(-wdot);
-      comb(dim, v[j + pc]->p, 1.0, src, cf, false);
+      comb(dim, v[j + pc]->p, iz, src, cf, false);
     } else {
-      comb(dim, v[j + pc]->p, 1.0, src, cf, false, v[j]->p, &wdot);
+      comb(dim, v[j + pc]->p, iz, src, cf, false, v[j]->p, &wdot);
       // step (9): deflated vectors (classical Gram-Schmidt over the mutually orthogonal set)
       std::vector<const double *> dvv;
       for (int k = 0; k < nd; ++k) dvv.push_back(v[Idef[k]]->p);

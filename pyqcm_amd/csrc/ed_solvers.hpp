@@ -37,6 +37,5 @@ int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q, int M0, 
 // <a, b> on device
 double ddot(size_t n, const double *a, const double *b);
 // per-thread device buffer for per-block partial sums (16 x partials_cap())
-double *work_partials();
 
 }  // namespace ed
