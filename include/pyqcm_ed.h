@@ -133,10 +133,11 @@ int ed_band_lanczos(int64_t h, int p, const double *phi, int cap, double *evals,
 int ed_davidson(int64_t h, int K, double *evals, double *evecs, int *niter);
 /* blockLanczos (src_ed/Hamiltonian/Lanczos.hpp:426-572), QR variant */
 int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double *A, double *B, int *nfl);
-/* lowest eigenpair of the symmetric tridiagonal (d[n], e[n-1]); z[n] may be NULL
+/* lowest eigenpair of the symmetric tridiagonal (d[n], e[n-1]); z[n] may be NULL;
+   guess (NaN: none) seeds a Rayleigh-quotient iteration, e.g. the previous Ritz value
    (EigensystemTridiagonal, src_ed/Hamiltonian/Lanczos.cpp:25-52, as used by
    LanczosEigenvalue / LanczosEigenvector for the Ritz pair) */
-int ed_tridiag_lowest(int n, const double *d, const double *e, double *ev0, double *z);
+int ed_tridiag_lowest(int n, const double *d, const double *e, double guess, double *ev0, double *z);
 /* c_a / c^dagger_a between factorized sectors (src_ed/model.cpp:155-182) */
 int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x, double z, double *y);
 

@@ -77,7 +77,7 @@ SIGNATURES = {
     "ed_davidson": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, _ip]),
     "ed_block_lanczos": (C.c_int, [C.c_int64, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _ip]),
     "ed_create_or_destroy": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp]),
-    "ed_tridiag_lowest": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp]),
+    "ed_tridiag_lowest": (C.c_int, [C.c_int, _dp, _dp, C.c_double, _dp, _dp]),
     "ed_cf_eval": (C.c_int, [C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]),
     "ed_vdvh": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]),
     "ed_mcf_eval": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),

@@ -464,12 +464,12 @@ int ed_create_or_destroy(int L, int pm, int a, int Nup, int Ndw, const double *x
   });
 }
 
-int ed_tridiag_lowest(int n, const double *d, const double *e, double *ev0, double *z) {
+int ed_tridiag_lowest(int n, const double *d, const double *e, double guess, double *ev0, double *z) {
   return guard([&] {
     if (n <= 0 || !d || (n > 1 && !e) || !ev0) fail(ED_ERR_INVALID, "ed_tridiag_lowest: bad arguments");
     std::vector<double> zz;
     double ev;
-    tridiag_eigvec0(n, d, e, ev, zz);
+    tridiag_eigvec0(n, d, e, ev, zz, guess);
     *ev0 = ev;
     if (z) std::copy(zz.begin(), zz.end(), z);
   });

@@ -962,6 +962,12 @@ int num_cu() {
   return g_ncu;
 }
 
+// blocks per CU of the Krylov vector kernels (multi-dot, axpy, combine)
+int reduction_blocks_per_cu() {
+  const long v = params().geti("reduction_blocks_per_cu");
+  return (int)std::max(1L, std::min(v, 64L));
+}
+
 int grid_for(size_t n, int bs, int per_cu = 8) {
   size_t g = (n + bs - 1) / bs;
   size_t cap = (size_t)num_cu() * per_cu;
@@ -1177,7 +1183,7 @@ int multi_dot(size_t n, const double *a, int m, const double *const *b, double *
   PtrPack pk{};
   for (int k = 0; k < m; ++k) pk.p[k] = b[k];
   constexpr int BS = 256;
-  const int g = grid_for(n, BS, 2);
+  const int g = grid_for(n, BS, reduction_blocks_per_cu());
   hipLaunchKernelGGL((k_multi_dot<BS>), dim3(g), dim3(BS), 0, s, n, a, m, pk, part);
   ED_HIP(hipGetLastError());
   return g;
@@ -1190,7 +1196,7 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
   CoefPack cp{};
   for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
   constexpr int BS = 256;
-  const int g = grid_for(n, BS, 4);
+  const int g = grid_for(n, BS, reduction_blocks_per_cu());
   hipLaunchKernelGGL((k_multi_axpy_src<BS>), dim3(g), dim3(BS), 0, s, n, x, m, pk, cp, part, xs);
   ED_HIP(hipGetLastError());
   return g;
@@ -1203,7 +1209,7 @@ int combine(size_t n, double *t, double st, int m, const double *const *b, const
   CoefPack cp{};
   for (int k = 0; k < m; ++k) { pk.p[k] = b[k]; cp.c[k] = c[k]; }
   constexpr int BS = 256;
-  const int g = grid_for(n, BS, 4);
+  const int g = grid_for(n, BS, reduction_blocks_per_cu());
   hipLaunchKernelGGL((k_combine<BS>), dim3(g), dim3(BS), 0, s, n, t, st, m, pk, cp, npart, w, dpart);
   ED_HIP(hipGetLastError());
   return g;

@@ -1095,20 +1095,77 @@ static void tridiag_inverse_iteration(int n, const double *d, const double *e, d
   }
 }
 
-void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0) {
-  if (n <= 0) fail(ED_ERR_INVALID, "empty tridiagonal matrix");
-  double norm;
-  ev0 = tridiag_min_eig(n, d, e, norm);
-  std::vector<double> z;
-  tridiag_inverse_iteration(n, d, e, ev0, norm, z);
-  zlast0 = z[n - 1];
+// Rayleigh-quotient iteration from the shift 'guess' (the previous Ritz value
+// of a Lanczos run, close to the new lowest one): each sweep solves
+// (T - sigma) x = z with the pivoted-free LDL^T of a shifted matrix and moves
+// sigma to the Rayleigh quotient; cubic convergence, a few O(n) sweeps.  The
+// result is accepted only when the Sturm count shows no eigenvalue below it
+// (otherwise the caller falls back to bisection).
+static bool tridiag_lowest_rqi(int n, const double *d, const double *e, double guess, double norm, double &lam,
+                               std::vector<double> &z) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  z.assign(n, 1.0 / std::sqrt((double)n));
+  std::vector<double> D(n), l(n > 1 ? n - 1 : 1), x(n);
+  double sigma = guess;
+  for (int it = 0; it < 8; ++it) {
+    // LDL^T of T - sigma (no pivoting; a zero pivot is nudged)
+    D[0] = d[0] - sigma;
+    for (int i = 0; i < n - 1; ++i) {
+      if (D[i] == 0.0) D[i] = eps * std::max(norm, 1e-300);
+      l[i] = e[i] / D[i];
+      D[i + 1] = d[i + 1] - sigma - l[i] * e[i];
+    }
+    if (D[n - 1] == 0.0) D[n - 1] = eps * std::max(norm, 1e-300);
+    x = z;
+    for (int i = 1; i < n; ++i) x[i] -= l[i - 1] * x[i - 1];
+    for (int i = 0; i < n; ++i) x[i] /= D[i];
+    for (int i = n - 2; i >= 0; --i) x[i] -= l[i] * x[i + 1];
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += x[i] * x[i];
+    if (!(s > 0.0) || !std::isfinite(s)) return false;
+    s = 1.0 / std::sqrt(s);
+    for (int i = 0; i < n; ++i) z[i] = x[i] * s;
+    // Rayleigh quotient and residual
+    double rq = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double tz = d[i] * z[i];
+      if (i > 0) tz += e[i - 1] * z[i - 1];
+      if (i < n - 1) tz += e[i] * z[i + 1];
+      rq += z[i] * tz;
+    }
+    double r2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double tz = (d[i] - rq) * z[i];
+      if (i > 0) tz += e[i - 1] * z[i - 1];
+      if (i < n - 1) tz += e[i] * z[i + 1];
+      r2 += tz * tz;
+    }
+    sigma = rq;
+    if (std::sqrt(r2) <= 8.0 * n * eps * std::max(norm, 1e-300)) break;
+  }
+  lam = sigma;
+  const double tol = 16.0 * n * eps * std::max(norm, 1e-300);
+  return sturm_below(n, d, e, lam - tol) == 0;
 }
 
-void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0) {
+void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0, double guess) {
   if (n <= 0) fail(ED_ERR_INVALID, "empty tridiagonal matrix");
-  double norm;
+  double norm = 0.0;
+  for (int i = 0; i < n; ++i)
+    norm = std::max(norm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0));
+  if (std::isfinite(guess) && n > 2 && tridiag_lowest_rqi(n, d, e, guess, norm, ev0, z0)) {
+    // refine the eigenvector with the shifted-definite inverse iteration
+    tridiag_inverse_iteration(n, d, e, ev0, norm, z0);
+    return;
+  }
   ev0 = tridiag_min_eig(n, d, e, norm);
   tridiag_inverse_iteration(n, d, e, ev0, norm, z0);
+}
+
+void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0, double guess) {
+  std::vector<double> z;
+  tridiag_eigvec0(n, d, e, ev0, z, guess);
+  zlast0 = z[n - 1];
 }
 
 // Householder tridiagonalisation (row-major working copy) + implicit QL

@@ -6,6 +6,7 @@
 #include <memory>
 #include <set>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "ed_common.hpp"
@@ -120,8 +121,13 @@ void csr_lower(const TermList &T, int Nup, int Ndw, std::vector<int32_t> &Iptr, 
 // ---------------------------------------------------------------------------
 // symmetric tridiagonal: lowest eigenvalue and the last components of all
 // eigenvectors (for the Ritz residual); full eigenvector of the lowest state.
-void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0);
-void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0);
+// lowest eigenvalue and last eigenvector component of a symmetric tridiagonal
+// matrix; guess (e.g. the previous Ritz value) seeds a Rayleigh-quotient
+// iteration, with bisection as the fallback
+void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double &zlast0,
+                    double guess = std::numeric_limits<double>::quiet_NaN());
+void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0,
+                     double guess = std::numeric_limits<double>::quiet_NaN());
 // dense symmetric eigensystem, A column-major n x n, eigenvalues ascending
 void symeig(int n, const double *A, double *w, double *V /* may be null */);
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 
 #include "ed_dense.hpp"
@@ -167,7 +168,8 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
     R.alpha.push_back(alpha);
     R.beta.push_back(beta);
     double e0, zl;
-    tridiag_lowest((int)R.alpha.size(), R.alpha.data(), R.beta.data(), e0, zl);
+    tridiag_lowest((int)R.alpha.size(), R.alpha.data(), R.beta.data(), e0, zl,
+                   R.alpha.size() > 1 ? R.E0 : std::numeric_limits<double>::quiet_NaN());
     R.E0 = e0;
     ritz = std::fabs(zl * beta);
     if ((int)R.alpha.size() > max_iter)

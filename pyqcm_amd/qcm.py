@@ -43,7 +43,7 @@ _PARAM_TYPES = {
     "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
     "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
-    "sell_sort_window": _INT, "hv_block": _INT,
+    "sell_sort_window": _INT, "hv_block": _INT, "reduction_blocks_per_cu": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 
@@ -352,16 +352,20 @@ def create_or_destroy(L, pm, a, Nup, Ndw, x, z=1.0):
     return y
 
 
-def tridiag_lowest(d, e):
+def tridiag_lowest(d, e, guess=None):
     """Lowest eigenvalue and normalized eigenvector of the symmetric tridiagonal
-    matrix with diagonal d and off-diagonal e (the Ritz pair of the Lanczos driver)."""
+    matrix with diagonal d and off-diagonal e (the Ritz pair of the Lanczos
+    driver); guess, e.g. the previous Ritz value, seeds a Rayleigh-quotient
+    iteration."""
     d = np.ascontiguousarray(d, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.float64)
     if len(e) != max(len(d) - 1, 0):
         raise EDError(1, "tridiag_lowest: e must have len(d) - 1 entries")
     ev = C.c_double()
     z = np.zeros(len(d))
-    check(lib().ed_tridiag_lowest(len(d), _dptr(d), _dptr(e) if len(e) else _dptr(np.zeros(1)), C.byref(ev), _dptr(z)))
+    g = float("nan") if guess is None else float(guess)
+    check(lib().ed_tridiag_lowest(len(d), _dptr(d), _dptr(e) if len(e) else _dptr(np.zeros(1)), g, C.byref(ev),
+                                  _dptr(z)))
     return ev.value, z
 
 

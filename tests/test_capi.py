@@ -94,14 +94,19 @@ def _tri_cases():
     return cases
 
 
+@pytest.mark.parametrize("seed", ["none", "near", "second", "far"])
 @pytest.mark.parametrize("case", range(10))
-def test_tridiag_lowest_matches_dense_eigh(case):
-    """Lowest Ritz pair of the Lanczos driver (bisection + inverse iteration)
-    against numpy.linalg.eigh of the dense tridiagonal matrix."""
+def test_tridiag_lowest_matches_dense_eigh(case, seed):
+    """Lowest Ritz pair of the Lanczos driver (bisection, or Rayleigh-quotient
+    iteration from a guess, + inverse iteration) against numpy.linalg.eigh of
+    the dense tridiagonal matrix; a guess at the second eigenvalue or far away
+    must still give the lowest one."""
     d, e = _tri_cases()[case]
     T = _tri_dense(d, e)
     w, V = np.linalg.eigh(T)
-    ev, z = qcm.tridiag_lowest(d, e)
+    guess = {"none": None, "near": w[0] + 1e-3 * (1 + abs(w[0])),
+             "second": w[min(1, len(w) - 1)], "far": w[-1] + 1.0}[seed]
+    ev, z = qcm.tridiag_lowest(d, e, guess)
     norm = np.abs(T).sum(axis=1).max()
     assert abs(ev - w[0]) <= 1e-13 * max(norm, 1.0)
     assert abs(np.linalg.norm(z) - 1.0) <= 1e-12
