@@ -76,7 +76,23 @@ def cpu_hv_baseline(wl, nrep=2):
     return 24.0 * dim / per / 1e9, per, orc.openmp_threads(), dim
 
 
-def ed_solve_gpu(nz=1024, beta=50.0, reps=2):
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of a known size: the reference point for the
+    FETCH_SIZE / WRITE_SIZE counters (bytes per KiB reported)."""
+    import ctypes
+    from pyqcm_amd import _lib
+    L = _lib.lib()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(L.ed_device_alloc(nbytes, ctypes.byref(a)))
+    _lib.check(L.ed_device_alloc(nbytes, ctypes.byref(b)))
+    _lib.check(L.ed_device_copy(b, a, nbytes, 2))
+    _lib.check(L.ed_device_sync())
+    _lib.check(L.ed_device_free(a, nbytes))
+    _lib.check(L.ed_device_free(b, nbytes))
+    return nbytes
+
+
+def ed_solve_gpu(nz=1024, beta=50.0, reps=5):
     import numpy as np
     from pyqcm_amd import qcm, workloads as W
     wl = W.plaquette_bath(seed=2025)
@@ -102,8 +118,14 @@ def ed_solve_gpu(nz=1024, beta=50.0, reps=2):
     return wl, E0, float(np.median(times))
 
 
-def ed_solve_cpu(wl, nz=1024, beta=50.0):
-    """Oracle pipeline on the CPU: Lanczos GS, band Lanczos per (pm), VDVH."""
+def ed_solve_cpu(wl, nz=1024, beta=50.0, reps=3):
+    """Oracle pipeline on the CPU (median of reps): Lanczos GS, band Lanczos per (pm), VDVH."""
+    import numpy as np
+    runs = [_ed_solve_cpu_once(wl, nz, beta) for _ in range(reps)]
+    return runs[0][0], float(np.median([t for _, t in runs]))
+
+
+def _ed_solve_cpu_once(wl, nz, beta):
     import numpy as np
     from oracle import oracle as orc
     from pyqcm_amd import workloads as W
@@ -135,6 +157,9 @@ def main():
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-hv", action="store_true", help="skip the H.v leg (profiling of the solve leg alone)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this ED solver is built and measured on a single MI355X "
@@ -142,7 +167,14 @@ def main():
         sys.exit(2)
     from pyqcm_amd import _lib
     _lib.require_device()
+    if args.copy_calibration:
+        print(json.dumps({"copy_bytes": copy_calibration()}))
+        return
 
+    if args.no_hv:
+        out = {"ed_solves": _solve_leg(args)}
+        print(json.dumps(out))
+        return
     H, wl, ms, wall = bench_hv(args.steps, args.warmup, args.variant)
     dim = H.dim
     gbs = 24.0 * dim / (ms * 1e-3) / 1e9
@@ -181,19 +213,24 @@ def main():
                                "sample": "oracle OpenMP factorized H.v on the same sector, 2 timed products "
                                          "(%.2f s each) after one untimed build" % per}
     if not args.no_solve:
-        wl2, E0g, tg = ed_solve_gpu()
-        solve = {"workload": "2x2 plaquette t=1 t'=-0.3 U=8 mu=4 + 8 bath (seed 2025), sector N=12 S=0, "
-                             "GS + band-Lanczos G on 1024 Matsubara points (beta=50)",
-                 "gpu_solves_per_s": round(1.0 / tg, 3), "gpu_s_per_solve": round(tg, 4), "E0_gpu": E0g}
-        if not args.no_cpu:
-            E0c, tc = ed_solve_cpu(wl2)
-            import oracle.oracle as orc
-            solve.update({"cpu_solves_per_s": round(1.0 / tc, 4), "cpu_s_per_solve": round(tc, 3),
-                          "cpu_cores": orc.openmp_threads(), "E0_cpu": E0c,
-                          "dE0_rel": abs(E0g - E0c) / abs(E0c), "speedup": round(tc / tg, 2),
-                          "accepted": abs(E0g - E0c) / abs(E0c) < 1e-10})
-        out["ed_solves"] = solve
+        out["ed_solves"] = _solve_leg(args)
     print(json.dumps(out))
+
+
+def _solve_leg(args):
+    wl2, E0g, tg = ed_solve_gpu()
+    solve = {"workload": "2x2 plaquette t=1 t'=-0.3 U=8 mu=4 + 8 bath (seed 2025), sector N=12 S=0, "
+                         "GS + band-Lanczos G on 1024 Matsubara points (beta=50)",
+             "gpu_solves_per_s": round(1.0 / tg, 3), "gpu_s_per_solve": round(tg, 4), "E0_gpu": E0g,
+             "timing": "median of 5 GPU solves after one warm-up; CPU: median of 3"}
+    if not args.no_cpu:
+        E0c, tc = ed_solve_cpu(wl2)
+        import oracle.oracle as orc
+        solve.update({"cpu_solves_per_s": round(1.0 / tc, 4), "cpu_s_per_solve": round(tc, 3),
+                      "cpu_cores": orc.openmp_threads(), "E0_cpu": E0c,
+                      "dE0_rel": abs(E0g - E0c) / abs(E0c), "speedup": round(tc / tg, 2),
+                      "accepted": abs(E0g - E0c) / abs(E0c) < 1e-10})
+    return solve
 
 
 if __name__ == "__main__":
