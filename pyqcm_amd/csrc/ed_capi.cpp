@@ -9,6 +9,7 @@
 
 #include "../../include/pyqcm_ed.h"
 #include "ed_common.hpp"
+#include "ed_batch.hpp"
 #include "ed_instance.hpp"
 #include "ed_model.hpp"
 #include "ed_solvers.hpp"
@@ -235,11 +236,10 @@ int ed_mixing(int label, int *mixing) {
 
 int ed_solve_batch(const int *labels, int n, int flags) {
   return guard([&] {
-    for (int k = 0; k < n; ++k) {
-      Instance &I = get_instance(labels[k]);
-      I.ground_state_solve();
-      if (flags & 1) I.green_function_solve();
-    }
+    if (n < 0 || (n > 0 && !labels)) fail(ED_ERR_INVALID, "ed_solve_batch: bad arguments");
+    std::vector<Instance *> insts;
+    for (int k = 0; k < n; ++k) insts.push_back(&get_instance(labels[k]));
+    solve_batch(insts, (flags & 1) != 0);
   });
 }
 

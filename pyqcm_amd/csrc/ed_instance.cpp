@@ -130,16 +130,21 @@ std::string Instance::gs_string() const {
 // c_a / c^dagger_a on factorized vectors (src_ed/model.cpp:155-182,
 // src_ed/ED_basis.cpp:79-110): gather over the target sector.
 // ---------------------------------------------------------------------------
-Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, double z, double *y) {
-  const int spin = a >= L ? 1 : 0;
-  const int orb = a - spin * L;
-  Sector tgt = shift_sector(src, pm, spin);
+CdagMaps cdag_maps(int L, int pm, int a, const Sector &src) {
+  CdagMaps C;
+  C.spin = a >= L ? 1 : 0;
+  const int orb = a - C.spin * L;
+  C.tgt = shift_sector(src, pm, C.spin);
   auto BUs = half_basis(L, src.Nup()), BDs = half_basis(L, src.Ndw());
-  auto BUt = half_basis(L, tgt.Nup()), BDt = half_basis(L, tgt.Ndw());
-  const HalfBasis &Bt = spin ? *BDt : *BUt;   // changed spin, target
-  const HalfBasis &Bs = spin ? *BDs : *BUs;   // changed spin, source
-  std::vector<int32_t> map(Bt.dim, -1);
-  std::vector<int8_t> sgn(Bt.dim, 1), upar;
+  auto BUt = half_basis(L, C.tgt.Nup()), BDt = half_basis(L, C.tgt.Ndw());
+  C.du_t = BUt->dim;
+  C.dd_t = BDt->dim;
+  C.du_s = BUs->dim;
+  C.dd_s = BDs->dim;
+  const HalfBasis &Bt = C.spin ? *BDt : *BUt;   // changed spin, target
+  const HalfBasis &Bs = C.spin ? *BDs : *BUs;   // changed spin, source
+  C.map.assign(Bt.dim, -1);
+  C.sgn.assign(Bt.dim, 1);
   const uint32_t m = 1u << orb;
   for (uint32_t t = 0; t < Bt.dim; ++t) {
     const uint32_t st = Bt.bin[t];
@@ -151,15 +156,24 @@ Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, doub
       if (!(st & m)) continue;
       ss = st & ~m;
     }
-    const uint32_t k = Bs.index(ss);
-    if (k == Bs.dim) continue;
-    map[t] = (int32_t)k;
-    sgn[t] = (__builtin_popcount(st & (m - 1u)) & 1) ? -1 : 1;
+    const uint32_t k2 = Bs.index(ss);
+    if (k2 == Bs.dim) continue;
+    C.map[t] = (int32_t)k2;
+    C.sgn[t] = (__builtin_popcount(st & (m - 1u)) & 1) ? -1 : 1;
   }
-  if (spin) {
-    upar.resize(BUt->dim);
-    for (uint32_t u = 0; u < BUt->dim; ++u) upar[u] = (__builtin_popcount(BUt->bin[u]) & 1) ? -1 : 1;
+  if (C.spin) {
+    C.upar.resize(BUt->dim);
+    for (uint32_t u = 0; u < BUt->dim; ++u) C.upar[u] = (__builtin_popcount(BUt->bin[u]) & 1) ? -1 : 1;
   }
+  return C;
+}
+
+Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, double z, double *y) {
+  const CdagMaps C = cdag_maps(L, pm, a, src);
+  const int spin = C.spin;
+  const std::vector<int32_t> &map = C.map;
+  const std::vector<int8_t> &sgn = C.sgn, &upar = C.upar;
+  const Sector tgt = C.tgt;
   DevBuf<int32_t> dmap;
   DevBuf<int8_t> dsgn, dupar;
   dmap.upload(map.empty() ? std::vector<int32_t>{-1} : map);
@@ -167,9 +181,9 @@ Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, doub
   if (spin) dupar.upload(upar.empty() ? std::vector<int8_t>{1} : upar);
   CdagArgs A;
   A.spin = spin;
-  A.du_t = BUt->dim;
-  A.dd_t = BDt->dim;
-  A.du_s = BUs->dim;
+  A.du_t = C.du_t;
+  A.dd_t = C.dd_t;
+  A.du_s = C.du_s;
   A.map = dmap.p;
   A.sgn = dsgn.p;
   A.upar = spin ? dupar.p : nullptr;
@@ -261,7 +275,20 @@ std::pair<double, std::string> Instance::ground_state_solve() {
       states.push_back(std::move(st));
     }
   }
+  finalize_states();
+  return {GS_energy, gs_string()};
+}
+
+// compute_weights (src_ed/model_instance.hpp:356-417) and the minimum-weight
+// filter over the candidate states, then the instance's E0 and mixing
+void Instance::finalize_states() {
+  Params &P = params();
+  const double temperature = P.getd("temperature");
+  double max_gap = -std::log(P.getd("minimum_weight")) * temperature;
+  if (max_gap < kMinGap) max_gap = kMinGap;
   if (states.empty()) fail(ED_ERR_SECTOR, "instance " + std::to_string(label) + " has no low-energy state!");
+  GS_energy = 1.0e12;
+  for (auto &x : states) GS_energy = std::min(GS_energy, x->energy);
   // compute_weights
   if (temperature < 1.0e-8) {
     double e0 = 1e12;
@@ -304,7 +331,6 @@ std::pair<double, std::string> Instance::ground_state_solve() {
   averages.clear();
   averages_done = false;
   gs_solved = true;
-  return {GS_energy, gs_string()};
 }
 
 const std::vector<std::tuple<std::string, double, double>> &Instance::cluster_averages() {

@@ -50,6 +50,9 @@ struct Instance {
 
   TermList terms() const;
   std::pair<double, std::string> ground_state_solve();
+  // weights, filters and E0 of the candidate states in 'states' (each with its
+  // sector, energy and normalized vector); marks the ground state solved
+  void finalize_states();
   // ground-state averages and variances of every instance operator, computed
   // on first use (the reference fills them at the end of low_energy_states,
   // src_ed/model_instance.hpp:341-344; the values are the same)
@@ -73,6 +76,18 @@ void erase_instance(int label);
 // c_a / c^dagger_a (pm = -1 / +1) of spin-orbital a on a factorized sector vector
 // (device); returns the target sector.  y must hold the target dimension.
 Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, double z, double *y);
+// the half-basis maps of c_a (pm = -1) / c^dagger_a (pm = +1) from sector src:
+// for the changed spin, map[t] = source half index (or -1) and sgn[t] the
+// Jordan-Wigner sign of target half state t; for a down orbital, upar[u] =
+// (-1)^popcount(up state u) (all up orbitals precede the down ones)
+struct CdagMaps {
+  Sector tgt;
+  int spin = 0;
+  uint32_t du_t = 0, dd_t = 0, du_s = 0, dd_s = 0;
+  std::vector<int32_t> map;
+  std::vector<int8_t> sgn, upar;
+};
+CdagMaps cdag_maps(int L, int pm, int a, const Sector &src);
 size_t sector_dim(int L, const Sector &s);
 bool sector_valid(int L, const Sector &s);
 Sector shift_sector(const Sector &s, int pm, int spin);

@@ -8,6 +8,7 @@
 
 #include "ed_common.hpp"
 #include "ed_kernels.hpp"
+#include "ed_rng.hpp"
 
 namespace ed {
 namespace {
@@ -758,29 +759,10 @@ __global__ __launch_bounds__(BS) void k_fill(size_t n, double *x, double v) {
 }
 __global__ void k_set1(double *x, size_t i) { x[i] = 1.0; }
 
-// Counter-based N(0,1) draws: SplitMix64 (Steele, Lea & Flood, OOPSLA 2014)
-// hashes (seed, pair index) into two 53-bit uniforms, Box-Muller turns them
-// into the pair (r cos t, r sin t); element i takes component i % 2 of pair
-// i / 2, so the vector does not depend on the launch shape.
-__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
-  z += 0x9e3779b97f4a7c15ull;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  return z ^ (z >> 31);
-}
+// N(0,1) start vectors (normal_draw, ed_rng.hpp)
 template <int BS>
 __global__ __launch_bounds__(BS) void k_random_normal(size_t n, uint64_t seed, double *x) {
-  const uint64_t key = splitmix64(seed);
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-    const uint64_t pr = i >> 1;
-    const uint64_t a = splitmix64(key ^ (2 * pr)), b = splitmix64(key ^ (2 * pr + 1));
-    const double u1 = ((a >> 11) + 1) * 0x1.0p-53;  // (0, 1]
-    const double u2 = (b >> 11) * 0x1.0p-53;        // [0, 1)
-    const double r = sqrt(-2.0 * log(u1));
-    double sn, cs;
-    sincospi(2.0 * u2, &sn, &cs);
-    x[i] = (i & 1) ? r * sn : r * cs;
-  }
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] = normal_draw(seed, i);
 }
 
 // Davidson correction (src_ed/Hamiltonian/Davidson.hpp:181-188): r_i /= (mu - d_i + tiny)

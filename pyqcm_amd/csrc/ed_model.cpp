@@ -248,6 +248,14 @@ double *partials_acquire() {
   for (int t = 0; t < kSlots; ++t) {
     const int s = (A.next + t) % kSlots;
     if (!A.busy[s]) {
+      // a slot is handed out clean: a leftover partial would be taken for a
+      // new one (the first entries of the first two rows are where small
+      // reductions write)
+      const double *hs = A.h + (size_t)s * A.slot_len;
+      for (size_t i = 0; i < 64; ++i)
+        if (!is_sentinel(hs + i) || !is_sentinel(hs + partials_cap() + i))
+          fail(ED_ERR_INTERNAL, "reduction slot " + std::to_string(s) + " holds a stale partial at entry " +
+                                    std::to_string(is_sentinel(hs + i) ? partials_cap() + i : i));
       A.busy[s] = true;
       A.next = (s + 1) % kSlots;
       return A.d + (size_t)s * A.slot_len;
@@ -943,6 +951,34 @@ static bool factorizable(const TermList &T) {
   for (const auto &tc : T.terms)
     if (tc.first->kind == OP_ONE_BODY && tc.first->spin_flip && tc.second != 0.0) return false;
   return true;
+}
+
+bool terms_factorizable(const TermList &T) { return factorizable(T); }
+
+FactorizedHost factorized_tables(const TermList &T, const Sector &sec) {
+  if (!factorizable(T)) fail(ED_ERR_UNSUPPORTED, "the instance's operators do not factorize by spin");
+  if (sec.Nup() < 0 || sec.Ndw() < 0 || sec.Nup() > T.L || sec.Ndw() > T.L || sec.Nup() >= Sector::odd ||
+      sec.Ndw() >= Sector::odd)
+    fail(ED_ERR_SECTOR, "sector " + sec.name() + " has no factorized basis");
+  const auto BU = half_basis(T.L, sec.Nup()), BD = half_basis(T.L, sec.Ndw());
+  const HalfHost hu = build_half(T, 0, *BU), hd = build_half(T, 1, *BD);
+  FactorizedHost F;
+  F.du = BU->dim;
+  F.dd = BD->dim;
+  auto unpack = [](const HalfHost &H, std::vector<uint32_t> &ptr, std::vector<uint32_t> &col,
+                   std::vector<double> &val) {
+    ptr = H.ptr;
+    col.resize(H.ent.size());
+    val.resize(H.ent.size());
+    for (size_t e = 0; e < H.ent.size(); ++e) {
+      col[e] = H.packed ? (H.ent[e] & ((1u << H.colbits) - 1u)) : H.ent[e];
+      val[e] = H.packed ? H.val[H.ent[e] >> H.colbits] : H.val[e];
+    }
+  };
+  unpack(hu, F.up_ptr, F.up_col, F.up_val);
+  unpack(hd, F.dn_ptr, F.dn_col, F.dn_val);
+  F.diag = host_full_diag(hu, hd, *BU, *BD, cross_terms(T), T.L);
+  return F;
 }
 
 std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format) {

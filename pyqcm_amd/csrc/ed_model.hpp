@@ -108,6 +108,17 @@ struct Hamiltonian {
 };
 
 std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format);
+
+// Explicit host form of the factorized Hamiltonian of one sector: per-spin
+// merged symmetric one-body tables (CSR with values; both triangles) and the
+// full diagonal (dim_up * dim_dn, label order).  Used by the batched solver.
+struct FactorizedHost {
+  uint32_t du = 0, dd = 0;
+  std::vector<uint32_t> up_ptr, up_col, dn_ptr, dn_col;
+  std::vector<double> up_val, dn_val, diag;
+};
+bool terms_factorizable(const TermList &T);
+FactorizedHost factorized_tables(const TermList &T, const Sector &sec);
 // time the candidate factorized schedules (fused; two-pass x slab widths 16/32/64
 // x streaming on/off) and keep the fastest; returns ms per candidate
 std::vector<double> hamiltonian_autotune(Hamiltonian &H, int &best);

@@ -45,7 +45,7 @@ double comb(size_t n, double *t, double st, const std::vector<const double *> &b
       return 0.0;
     }
     double *np = partials_acquire(), *dp = np + partials_cap();
-    const int nb = combine(n, t, stv, m, bb, cc, np, wd ? w : nullptr, wd ? dp : nullptr, stream());
+    const int nb = combine(n, t, stv, m, bb, cc, wn ? np : nullptr, wd ? w : nullptr, wd ? dp : nullptr, stream());
     if (!wn) {
       *dot_w = sum_partials(dp, nb);
       return 0.0;
@@ -293,6 +293,7 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     }
     const double hd_raw = H.dot_result(hnb);  // <H v_j, v_j> before normalization
     const double z = std::sqrt(std::max(dv[0], 0.0));
+    if (verbose() && dim < 4096) vlog("band Lanczos step j %d pc %d: |v_j| %.3e, <Hv,v> %.6e", j, pc, z, hd_raw);
     if (z < accur_deflation) {
       v[j + pc].reset();
       if (j >= pc) {

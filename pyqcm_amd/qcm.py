@@ -24,7 +24,7 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "ground_state_solve", "Green_function_solveC", "Green_function", "Green_function_grid",
            "Green_function_dimensionC", "self_energyC", "hybridization_functionC", "cluster_averages",
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
-           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states"]
+           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -128,6 +128,18 @@ def ground_state_solve(label=0):
     buf = C.create_string_buffer(1024)
     check(lib().ed_ground_state_solve(int(label), C.byref(E0), buf, 1024))
     return E0.value, buf.value.decode()
+
+
+def solve_batch(labels, green_function=True):
+    """Ground states (and, with green_function, the Green functions) of many
+    instances at once.  Instances whose operators factorize by spin and that
+    use the Lanczos ground state run batched on the device, one workgroup per
+    (instance, sector), and with GF_method 'F' one workgroup per continued
+    fraction; the others are solved one after the other.  Afterwards every
+    instance answers the usual queries (ground_state_solve, Green_function...)."""
+    labels = [int(x) for x in labels]
+    arr = (C.c_int * max(len(labels), 1))(*labels)
+    check(lib().ed_solve_batch(arr, len(labels), 1 if green_function else 0))
 
 
 def Green_function_solveC(label=0):

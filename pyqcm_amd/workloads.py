@@ -187,6 +187,47 @@ def real_axis(n=512, wmin=-8.0, wmax=8.0, eta=0.1):
     return np.linspace(wmin, wmax, n) + 1j * eta
 
 
+def vca_cluster(nx=2, ny=2):
+    """nx x ny open Hubbard cluster with the Neel Weiss field of an
+    antiferromagnetic VCA sweep: operators t (nearest neighbours), U, mu and
+    M = sum_i (-1)^(x+y) (n_i,up - n_i,down); target sectors N = n-1 (S = +-1),
+    n (S = 0), n+1 (S = +-1), n = nx * ny."""
+    L = nx * ny
+    bonds = []
+    for y in range(ny):
+        for x in range(nx):
+            i = x + nx * y
+            if x + 1 < nx:
+                bonds.append((i, i + 1))
+            if y + 1 < ny:
+                bonds.append((i, i + nx))
+    neel = []
+    for y in range(ny):
+        for x in range(nx):
+            i = x + nx * y
+            s = 1.0 if (x + y) % 2 == 0 else -1.0
+            neel.append((i, i, s))
+            neel.append((i + L, i + L, -s))
+    ops = {"t": ("one-body", _hop(L, bonds)),
+           "U": ("interaction", _hubbard_U(L, range(L))),
+           "mu": ("one-body", _mu(L, range(L))),
+           "M": ("one-body", neel)}
+    sectors = "/".join([_sector(L - 1, 1), _sector(L - 1, -1), _sector(L, 0), _sector(L + 1, 1),
+                        _sector(L + 1, -1)])
+    return Workload("vca%dx%d" % (nx, ny), L, 0, ops, {"t": 1.0, "U": 4.0, "mu": 0.0, "M": 0.1}, sectors,
+                    list(range(L)))
+
+
+def vca_points(n=512, seed=4):
+    """n parameter points of the sweep: U ~ U[2, 10], M ~ U[0, 0.3], mu ~ U[-1, 1]
+    (numpy default_rng(seed)), t = 1."""
+    rng = np.random.default_rng(seed)
+    U = rng.uniform(2.0, 10.0, n)
+    M = rng.uniform(0.0, 0.3, n)
+    mu = rng.uniform(-1.0, 1.0, n)
+    return [{"t": 1.0, "U": float(U[k]), "M": float(M[k]), "mu": float(mu[k])} for k in range(n)]
+
+
 def setup(wl, label=0, sectors=None, model_name=None):
     """Register ``wl`` through the reference-style API (1-based elements) and
     create instance ``label``; returns the model name."""
