@@ -76,6 +76,87 @@ def cpu_hv_baseline(wl, nrep=2):
     return 24.0 * dim / per / 1e9, per, orc.openmp_threads(), dim
 
 
+def batch_gpu(nx, ny, npts=512, nz=256, reps=3):
+    """Batched VCA sweep: npts parameter points (U, Neel field M, mu from
+    default_rng(4)) of the nx x ny cluster; ground state over the five target
+    sectors and continued-fraction G on nz real frequencies (both spins) per
+    point, all points in one solve_batch.  Median over reps after a warm-up."""
+    import numpy as np
+    from pyqcm_amd import qcm, workloads as W
+    qcm.set_global_parameterC("GF_method", "F")
+    wl = W.vca_cluster(nx, ny)
+    name = "bench_" + wl.name
+    qcm.new_model(name, wl.n_sites, wl.n_bath)
+    for op, (typ, el) in wl.operators.items():
+        qcm.new_operator(name, op, typ, [(r + 1, c + 1, v) for (r, c, v) in el])
+    pts = W.vca_points(npts, seed=4)
+    z = W.real_axis(nz, -8.0, 8.0, 0.1)
+    times, E0 = [], None
+    label = 8000
+    for rep in range(reps + 1):
+        labels = []
+        for v in pts:
+            label += 1
+            qcm.new_model_instanceC(name, v, wl.sector, label)
+            labels.append(label)
+        t0 = time.perf_counter()
+        qcm.solve_batch(labels)
+        for s in (0, 1):
+            qcm.Green_function_batch(z, labels, spin_down=s)
+        t1 = time.perf_counter()
+        if rep > 0:
+            times.append(t1 - t0)
+        E0 = qcm.ground_state_solve(labels[0])[0]
+        for lab in labels:
+            qcm.erase_model_instance(lab)
+    qcm.set_global_parameterC("GF_method", "L")
+    return wl, pts, float(np.median(times)), E0
+
+
+def batch_cpu(wl, pts, nsample=16, nz=256):
+    """The oracle's CPU pipeline for the same points (bounded sample): Lanczos
+    ground state in every target sector, then LanczosGreen for every orbital
+    pair, both spins and +-, evaluated on the same frequencies."""
+    import dataclasses
+    import numpy as np
+    from oracle import oracle as orc
+    from pyqcm_amd import workloads as W
+    z = W.real_axis(nz, -8.0, 8.0, 0.1)
+    L, ns = wl.L, wl.n_sites
+    rng = np.random.default_rng(0)
+    E0_first = None
+    t0 = time.perf_counter()
+    for k, vals in enumerate(pts[:nsample]):
+        w = dataclasses.replace(wl, values=vals)
+        M = orc.Model(L, w.terms())
+        best = None
+        for sec in wl.sector.split("/"):
+            N, S = int(sec.split(":")[1][1:]), int(sec.split(":")[2][1:])
+            Nup, Ndw = (N + S) // 2, (N - S) // 2
+            E, gs, _, _ = M.lanczos_gs(Nup, Ndw, rng.normal(size=M.dim(Nup, Ndw)))
+            if best is None or E < best[0]:
+                best = (E, gs / np.linalg.norm(gs), Nup, Ndw)
+        E, gs, Nup, Ndw = best
+        if k == 0:
+            E0_first = E
+        for spin in (0, 1):
+            for pm in (-1, 1):
+                tN = (Nup + (pm if spin == 0 else 0), Ndw + (pm if spin == 1 else 0))
+                if min(tN) < 0 or max(tN) > L:
+                    continue
+                phi = [orc.create_or_destroy(L, pm, o + spin * L, Nup, Ndw, gs) for o in range(ns)]
+                for o1 in range(ns):
+                    for o2 in range(o1 + 1):
+                        psi = phi[o1] + (phi[o2] if o2 != o1 else 0.0)
+                        nrm = float(psi @ psi)
+                        if not nrm > 1e-14:
+                            continue
+                        a, b = M.lanczos_green(tN[0], tN[1], psi / np.sqrt(nrm))
+                        a, b = orc.cf_transform(a, b, E, nrm, pm == 1)
+                        orc.cf_eval(a, b, z)
+    return E0_first, (time.perf_counter() - t0) / nsample
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of a known size: the reference point for the
     FETCH_SIZE / WRITE_SIZE counters (bytes per KiB reported)."""
@@ -158,6 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-hv", action="store_true", help="skip the H.v leg (profiling of the solve leg alone)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched VCA-sweep leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
@@ -172,7 +254,11 @@ def main():
         return
 
     if args.no_hv:
-        out = {"ed_solves": _solve_leg(args)}
+        out = {}
+        if not args.no_solve:
+            out["ed_solves"] = _solve_leg(args)
+        if not args.no_batch:
+            out["batch"] = _batch_leg(args)
         print(json.dumps(out))
         return
     H, wl, ms, wall = bench_hv(args.steps, args.warmup, args.variant)
@@ -214,7 +300,27 @@ def main():
                                          "(%.2f s each) after one untimed build" % per}
     if not args.no_solve:
         out["ed_solves"] = _solve_leg(args)
+    if not args.no_batch:
+        out["batch"] = _batch_leg(args)
     print(json.dumps(out))
+
+
+def _batch_leg(args):
+    res = {}
+    for (nx, ny) in ((2, 2), (2, 3)):
+        wl, pts, tg, E0g = batch_gpu(nx, ny)
+        r = {"workload": "%dx%d Hubbard cluster, 512 points U~U[2,10] M~U[0,0.3] mu~U[-1,1] (default_rng(4)), "
+                         "5 target sectors, continued-fraction G on 256 real frequencies, both spins" % (nx, ny),
+             "gpu_points_per_s": round(512 / tg, 1), "gpu_s_per_512": round(tg, 4),
+             "timing": "median of 3 batched solves after one warm-up"}
+        if not args.no_cpu:
+            E0c, tc = batch_cpu(wl, pts)
+            import oracle.oracle as orc
+            r.update({"cpu_points_per_s": round(1.0 / tc, 2), "cpu_cores": orc.openmp_threads(),
+                      "cpu_sample": "first 16 points, oracle pipeline",
+                      "speedup": round((512 / tg) * tc, 1), "dE0_rel_first_point": abs(E0g - E0c) / abs(E0c)})
+        res["%dx%d" % (nx, ny)] = r
+    return res
 
 
 def _solve_leg(args):

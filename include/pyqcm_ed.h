@@ -74,6 +74,10 @@ int ed_green_function_dimension(int label, int *d);
 /* ED::Green_function (src_ed/qcm_ED.cpp:230-236): nz frequencies (re,im pairs);
    out receives nz blocks of d x d complex, column-major. */
 int ed_green_function(int label, int nz, const double *z, int spin_down, int blocks, double *out);
+/* ED::Green_function for many instances on one grid (no reference counterpart: the
+   reference evaluates one instance per call); out holds, in label order, nz blocks of
+   d x d complex per instance (d = ed_green_function_dimension of that instance) */
+int ed_green_function_batch(const int *labels, int n, int nz, const double *z, int spin_down, double *out);
 /* ED::self_energy (src_ed/model_instance.hpp:683-700) */
 int ed_self_energy(int label, int nz, const double *z, int spin_down, double *out);
 /* ED::hybridization_function (src_ed/model_instance.hpp:1004-1018) */

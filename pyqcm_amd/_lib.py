@@ -59,6 +59,7 @@ SIGNATURES = {
     "ed_states": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _ip]),
     "ed_mixing": (C.c_int, [C.c_int, _ip]),
     "ed_solve_batch": (C.c_int, [_ip, C.c_int, C.c_int]),
+    "ed_green_function_batch": (C.c_int, [_ip, C.c_int, C.c_int, _dp, C.c_int, _dp]),
     "ed_hamiltonian_create": (C.c_int, [C.c_int, C.c_char_p, C.c_char, _i64p]),
     "ed_hamiltonian_destroy": (C.c_int, [C.c_int64]),
     "ed_hamiltonian_dim": (C.c_int, [C.c_int64, _i64p]),

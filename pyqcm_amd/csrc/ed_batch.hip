@@ -164,8 +164,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_gs(const BatchGSItem *items, int max_iter, double accur) {
   __shared__ double sh[BS / kW];
   __shared__ double s_lam, s_zl;
-  const BatchGSItem it = items[blockIdx.x];
-  const BatchSector &S = it.S;
+  const BatchGSItem &it = items[blockIdx.x];
+  const BatchSector S = it.S;
   const uint32_t dim = S.du * S.dd;
   double *v = it.ws, *vp = it.ws + dim, *w = it.ws + 2 * (size_t)dim;
   double *al = it.ab, *be = it.ab + (max_iter + 2);
@@ -241,22 +241,37 @@ __global__ __launch_bounds__(BS) void k_batch_gs(const BatchGSItem *items, int m
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_cf(const BatchCFItem *items, int max_cf, double acc) {
   __shared__ double sh[BS / kW];
-  const BatchCFItem it = items[blockIdx.x];
-  const BatchSector &S = it.S;
+  // item fields in registers (the two-orbital arrays are read by name, not
+  // by a run-time index, so nothing is spilled to scratch)
+  const BatchCFItem &itg = items[blockIdx.x];
+  const BatchSector S = itg.S;
+  const double *gs = itg.gs;
+  const uint32_t du_s = itg.du_s;
+  const int spin = itg.spin, norb = itg.norb;
+  const int32_t *map0 = itg.map[0], *map1 = itg.map[1];
+  const int8_t *sgn0 = itg.sgn[0], *sgn1 = itg.sgn[1], *upar = itg.upar;
+  double *ws = itg.ws, *av = itg.a, *bv = itg.b, *out = itg.out;
   const uint32_t dim = S.du * S.dd;
-  double *v = it.ws, *vp = it.ws + dim, *w = it.ws + 2 * (size_t)dim;
-  // psi = sum_o c_o |gs> (gather over the target sector)
+  double *v = ws, *vp = ws + dim, *w = ws + 2 * (size_t)dim;
+  // psi = c_o1 |gs> (+ c_o2 |gs>), gathered over the target sector
   double nn = 0.0;
   for (uint32_t I = threadIdx.x; I < dim; I += BS) {
     const uint32_t id = I / S.du, iu = I - id * S.du;
     double val = 0.0;
-    for (int o = 0; o < it.norb; ++o) {
-      if (it.spin == 0) {
-        const int32_t s = it.map[o][iu];
-        if (s >= 0) val += (double)it.sgn[o][iu] * it.gs[(size_t)s + (size_t)it.du_s * id];
-      } else {
-        const int32_t s = it.map[o][id];
-        if (s >= 0) val += (double)it.sgn[o][id] * (double)it.upar[iu] * it.gs[iu + (size_t)it.du_s * s];
+    if (spin == 0) {
+      const int32_t s0 = map0[iu];
+      if (s0 >= 0) val += (double)sgn0[iu] * gs[(size_t)s0 + (size_t)du_s * id];
+      if (norb == 2) {
+        const int32_t s1 = map1[iu];
+        if (s1 >= 0) val += (double)sgn1[iu] * gs[(size_t)s1 + (size_t)du_s * id];
+      }
+    } else {
+      const double pu = (double)upar[iu];
+      const int32_t s0 = map0[id];
+      if (s0 >= 0) val += (double)sgn0[id] * pu * gs[iu + (size_t)du_s * s0];
+      if (norb == 2) {
+        const int32_t s1 = map1[id];
+        if (s1 >= 0) val += (double)sgn1[id] * pu * gs[iu + (size_t)du_s * s1];
       }
     }
     v[I] = val;
@@ -266,14 +281,14 @@ __global__ __launch_bounds__(BS) void k_batch_cf(const BatchCFItem *items, int m
   nn = bsum<BS>(nn, sh);
   if (!(nn > 1e-14)) {
     if (threadIdx.x == 0) {
-      it.out[0] = -1.0;
-      it.out[1] = nn;
+      out[0] = -1.0;
+      out[1] = nn;
     }
     return;
   }
   const double inv = 1.0 / sqrt(nn);
   for (uint32_t I = threadIdx.x; I < dim; I += BS) v[I] *= inv;
-  if (threadIdx.x == 0) it.b[0] = 1.0;
+  if (threadIdx.x == 0) bv[0] = 1.0;
   __syncthreads();
   int n = 0;
   double bprev = 0.0, prod = 1.0, blast = 1.0;
@@ -281,8 +296,8 @@ __global__ __launch_bounds__(BS) void k_batch_cf(const BatchCFItem *items, int m
     double alpha, beta;
     lanczos_step<BS>(S, dim, v, vp, w, bprev, sh, alpha, beta);
     if (threadIdx.x == 0) {
-      it.a[n] = alpha;
-      it.b[n + 1] = beta;
+      av[n] = alpha;
+      bv[n + 1] = beta;
     }
     prod *= fabs(beta);
     blast = beta;
@@ -297,8 +312,8 @@ __global__ __launch_bounds__(BS) void k_batch_cf(const BatchCFItem *items, int m
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    it.out[0] = (double)n;
-    it.out[1] = nn;
+    out[0] = (double)n;
+    out[1] = nn;
   }
 }
 

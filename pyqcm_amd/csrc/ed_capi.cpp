@@ -182,6 +182,15 @@ int ed_green_function(int label, int nz, const double *z, int spin_down, int blo
   });
 }
 
+int ed_green_function_batch(const int *labels, int n, int nz, const double *z, int spin_down, double *out) {
+  return guard([&] {
+    if (n < 0 || (n > 0 && !labels)) fail(ED_ERR_INVALID, "ed_green_function_batch: bad arguments");
+    std::vector<Instance *> insts;
+    for (int k = 0; k < n; ++k) insts.push_back(&get_instance(labels[k]));
+    green_function_batch(insts, nz, z, spin_down != 0, out);
+  });
+}
+
 int ed_self_energy(int label, int nz, const double *z, int spin_down, double *out) {
   return guard([&] { get_instance(label).self_energy(nz, z, spin_down != 0, out); });
 }

@@ -555,11 +555,101 @@ void Instance::green_function_solve() {
 // ---------------------------------------------------------------------------
 // Green-function evaluation on frequency grids (device kernels)
 // ---------------------------------------------------------------------------
-void Instance::green_function(int nz, const double *z, bool spin_down, double *out) {
-  if (!gf_solved) green_function_solve();
+// continued_fraction_set::Green_function (src_ed/continued_fraction_set.cpp:63-98):
+// h holds the values of ncf fractions per frequency (fraction-major per z);
+// the representation's fractions start at frac0, two per orbital pair
+// (electron, hole), pairs o1 >= o2 in order.  G_aa = F_aa, G_ab =
+// (F_ab - G_aa - G_bb) / 2, added to acc (nz blocks n x n, column-major).
+static void assemble_cf(const double *h, int ncf, int frac0, int n, int nz, double *acc) {
+  for (int iz = 0; iz < nz; ++iz) {
+    auto F = [&](int q) {
+      const double *p0 = &h[2 * ((size_t)iz * ncf + frac0 + 2 * q)];
+      return cplx(p0[0] + p0[2], p0[1] + p0[3]);
+    };
+    std::vector<cplx> Gd(n);
+    int q = 0;
+    for (int o1 = 0; o1 < n; ++o1)
+      for (int o2 = 0; o2 <= o1; ++o2, ++q)
+        if (o1 == o2) Gd[o1] = F(q);
+    double *g = &acc[(size_t)iz * n * n * 2];
+    q = 0;
+    for (int o1 = 0; o1 < n; ++o1)
+      for (int o2 = 0; o2 <= o1; ++o2, ++q) {
+        cplx v = (o1 == o2) ? Gd[o1] : 0.5 * (F(q) - Gd[o1] - Gd[o2]);
+        g[2 * (o1 + (size_t)o2 * n)] += v.real();
+        g[2 * (o1 + (size_t)o2 * n) + 1] += v.imag();
+        if (o1 != o2) {
+          g[2 * (o2 + (size_t)o1 * n)] += v.real();
+          g[2 * (o2 + (size_t)o1 * n) + 1] += v.imag();
+        }
+      }
+  }
+}
+
+// the fractions of one representation, appended to a packed list
+static void pack_cf(const GFRep &G, int n, std::vector<int> &off, std::vector<double> &A, std::vector<double> &B) {
+  const int np = n * (n + 1) / 2;
+  for (int q = 0; q < np; ++q)
+    for (int eh = 0; eh < 2; ++eh) {
+      const auto &a = eh ? G.ha[q] : G.ea[q];
+      const auto &b = eh ? G.hb[q] : G.eb[q];
+      A.insert(A.end(), a.begin(), a.end());
+      B.insert(B.end(), b.begin(), b.begin() + a.size());
+      off.push_back((int)A.size());
+    }
+}
+
+int Instance::gf_spin(bool spin_down) const {
   if (spin_down && !(mixing & 4) && mixing != 0)
     fail(ED_ERR_INVALID, "spin_down=True impossible with Hilbert space mixing " + std::to_string(mixing));
-  const int spin = (spin_down && (mixing & 4)) ? 1 : 0;
+  return (spin_down && (mixing & 4)) ? 1 : 0;
+}
+
+void green_function_batch(const std::vector<Instance *> &insts, int nz, const double *z, bool spin_down,
+                          double *out) {
+  std::vector<size_t> base;
+  size_t tot = 0;
+  for (Instance *I : insts) {
+    if (!I->gf_solved) I->green_function_solve();
+    base.push_back(tot);
+    tot += (size_t)nz * I->gf_dim() * I->gf_dim() * 2;
+  }
+  std::fill(out, out + tot, 0.0);
+  // every continued-fraction representation goes into one evaluation launch
+  struct Part { size_t at; int frac0, n; };
+  std::vector<Part> parts;
+  std::vector<int> off{0};
+  std::vector<double> A, B;
+  for (size_t k = 0; k < insts.size(); ++k) {
+    Instance &I = *insts[k];
+    const int spin = I.gf_spin(spin_down), n = I.gf_dim();
+    bool all_cf = true;
+    for (auto &s : I.states) all_cf = all_cf && s->gf[spin].kind == 'F';
+    if (!all_cf) {
+      I.green_function(nz, z, spin_down, out + base[k]);
+      continue;
+    }
+    for (auto &s : I.states) {
+      parts.push_back({base[k], (int)off.size() - 1, n});
+      pack_cf(s->gf[spin], n, off, A, B);
+    }
+  }
+  if (parts.empty()) return;
+  const int ncf = (int)off.size() - 1;
+  DevBuf<int> doff;
+  DevBuf<double> dA, dB, dz, dO((size_t)nz * ncf * 2);
+  doff.upload(off);
+  dA.upload(A.empty() ? std::vector<double>{0.0} : A);
+  dB.upload(B.empty() ? std::vector<double>{0.0} : B);
+  dz.upload(z, (size_t)2 * nz);
+  cf_eval(ncf, doff.p, dA.p, dB.p, nz, dz.p, dO.p, stream());
+  const std::vector<double> h = dO.to_host();
+  for (const Part &pt : parts) assemble_cf(h.data(), ncf, pt.frac0, pt.n, nz, out + pt.at);
+}
+
+void Instance::green_function(int nz, const double *z, bool spin_down, double *out) {
+  if (!gf_solved) green_function_solve();
+  const int spin = gf_spin(spin_down);
   const int n = gf_dim();
   std::vector<double> acc((size_t)nz * n * n * 2, 0.0);
   DevBuf<double> dz;
@@ -579,18 +669,8 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
       // pack every fraction (electron, hole per pair) into one launch
       std::vector<int> off{0};
       std::vector<double> A, B;
-      std::vector<int> which;  // fraction index -> (pair * 2 + eh)
-      const int np = n * (n + 1) / 2;
-      for (int q = 0; q < np; ++q)
-        for (int eh = 0; eh < 2; ++eh) {
-          const auto &a = eh ? G.ha[q] : G.ea[q];
-          const auto &b = eh ? G.hb[q] : G.eb[q];
-          A.insert(A.end(), a.begin(), a.end());
-          B.insert(B.end(), b.begin(), b.begin() + a.size());
-          off.push_back((int)A.size());
-          which.push_back(q * 2 + eh);
-        }
-      const int ncf = (int)which.size();
+      pack_cf(G, n, off, A, B);
+      const int ncf = (int)off.size() - 1;
       DevBuf<int> doff;
       DevBuf<double> dA, dB, dO((size_t)nz * ncf * 2);
       doff.upload(off);
@@ -598,30 +678,7 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
       dB.upload(B.empty() ? std::vector<double>{0.0} : B);
       cf_eval(ncf, doff.p, dA.p, dB.p, nz, dz.p, dO.p, stream());
       std::vector<double> h = dO.to_host();
-      // continued_fraction_set::Green_function (src_ed/continued_fraction_set.cpp:63-98)
-      for (int iz = 0; iz < nz; ++iz) {
-        auto F = [&](int q) {
-          const double *p0 = &h[2 * ((size_t)iz * ncf + 2 * q)];
-          return cplx(p0[0] + p0[2], p0[1] + p0[3]);
-        };
-        std::vector<cplx> Gd(n);
-        int q = 0;
-        for (int o1 = 0; o1 < n; ++o1)
-          for (int o2 = 0; o2 <= o1; ++o2, ++q)
-            if (o1 == o2) Gd[o1] = F(q);
-        double *g = &acc[(size_t)iz * n * n * 2];
-        q = 0;
-        for (int o1 = 0; o1 < n; ++o1)
-          for (int o2 = 0; o2 <= o1; ++o2, ++q) {
-            cplx v = (o1 == o2) ? Gd[o1] : 0.5 * (F(q) - Gd[o1] - Gd[o2]);
-            g[2 * (o1 + (size_t)o2 * n)] += v.real();
-            g[2 * (o1 + (size_t)o2 * n) + 1] += v.imag();
-            if (o1 != o2) {
-              g[2 * (o2 + (size_t)o1 * n)] += v.real();
-              g[2 * (o2 + (size_t)o1 * n) + 1] += v.imag();
-            }
-          }
-      }
+      assemble_cf(h.data(), ncf, 0, n, nz, acc.data());
     } else if (G.kind == 'M') {
       for (int eh = 0; eh < 2; ++eh) {
         const GFRep::MCF &mc = eh ? G.mh : G.me;

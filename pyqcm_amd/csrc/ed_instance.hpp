@@ -64,6 +64,8 @@ struct Instance {
   void hybridization(int nz, const double *z, bool spin_down, double *out);
   std::string gs_string() const;
   int gf_dim() const { return model->n_sites; }
+  // spin block of the Green function for spin_down (checks the mixing)
+  int gf_spin(bool spin_down) const;
   // one-body matrix of the spin block (L x L, row-major) from the instance's terms
   std::vector<double> one_body_matrix(bool spin_down) const;
 };
@@ -88,6 +90,11 @@ struct CdagMaps {
   std::vector<int8_t> sgn, upar;
 };
 CdagMaps cdag_maps(int L, int pm, int a, const Sector &src);
+// Green functions of many instances on one grid: out holds, in instance
+// order, nz blocks of d x d complex (column-major) per instance; every
+// continued-fraction representation is evaluated in a single launch
+void green_function_batch(const std::vector<Instance *> &insts, int nz, const double *z, bool spin_down,
+                          double *out);
 size_t sector_dim(int L, const Sector &s);
 bool sector_valid(int L, const Sector &s);
 Sector shift_sector(const Sector &s, int pm, int spin);

@@ -813,17 +813,23 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
   return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
 }
 
+// one thread per (fraction, frequency), frequency fastest: the lanes of a
+// wave share the fraction, hence the trip count and the (broadcast)
+// coefficient loads.  G <- b_i / (z - a_i - G), innermost level first.
 __global__ void k_cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z,
                           double *out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncf * nz) return;
-  const int f = t % ncf, iz = t / ncf;
-  const cplx zz{z[2 * iz], z[2 * iz + 1]};
-  cplx G{0.0, 0.0};
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ncf * nz) return;
+  const int iz = (int)(t % nz), f = (int)(t / nz);
+  const double zr = z[2 * iz], zi = z[2 * iz + 1];
+  double gr = 0.0, gi = 0.0;
   for (int i = off[f + 1] - 1; i >= off[f]; --i) {
-    const cplx den{zz.re - a[i] - G.re, zz.im - G.im};
-    G = cdiv(cplx{b[i], 0.0}, den);
+    const double dr = zr - a[i] - gr, di = zi - gi;
+    const double s = b[i] / (dr * dr + di * di);
+    gr = dr * s;
+    gi = -di * s;
   }
+  const cplx G{gr, gi};
   out[2 * ((size_t)iz * ncf + f)] = G.re;
   out[2 * ((size_t)iz * ncf + f) + 1] = G.im;
 }
@@ -1243,9 +1249,10 @@ void cdag_gather(const CdagArgs &A, const double *x, double *y, hipStream_t s) {
 
 void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out,
              hipStream_t s) {
-  const int n = ncf * nz;
+  const size_t n = (size_t)ncf * nz;
   if (!n) return;
-  hipLaunchKernelGGL(k_cf_eval, dim3((n + 255) / 256), dim3(256), 0, s, ncf, off, a, b, nz, z, out);
+  if ((n + 255) / 256 > 0x7fffffffull) fail(ED_ERR_INVALID, "cf_eval: too many fractions x frequencies");
+  hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ncf, off, a, b, nz, z, out);
   ED_HIP(hipGetLastError());
 }
 

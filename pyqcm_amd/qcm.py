@@ -24,7 +24,7 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "ground_state_solve", "Green_function_solveC", "Green_function", "Green_function_grid",
            "Green_function_dimensionC", "self_energyC", "hybridization_functionC", "cluster_averages",
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
-           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch"]
+           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -170,6 +170,22 @@ def _gf_call(fn, label, z, spin_down, *extra):
 def Green_function_grid(z, spin_down=0, label=0, blocks=0):
     """G(z) for a whole frequency grid: array (nz, d, d)."""
     return _gf_call(lib().ed_green_function, label, z, spin_down, int(blocks))
+
+
+def Green_function_batch(z, labels, spin_down=0):
+    """G(z) of many instances on one grid, evaluated together: list of arrays (nz, d, d)."""
+    zz, zp = _grid(z)
+    labels = [int(x) for x in labels]
+    dims = [Green_function_dimensionC(l) for l in labels]
+    out = np.zeros(len(zz) * sum(d * d for d in dims), dtype=np.complex128)
+    arr = (C.c_int * max(len(labels), 1))(*labels)
+    check(lib().ed_green_function_batch(arr, len(labels), len(zz), zp, int(spin_down),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+    res, at = [], 0
+    for d in dims:
+        res.append(out[at:at + len(zz) * d * d].reshape(len(zz), d, d))
+        at += len(zz) * d * d
+    return res
 
 
 def Green_function(z, spin_down=0, label=0, blocks=0):

@@ -75,6 +75,11 @@ def _compare(wl, points, gf_method, z, dense=True):
             Ga = qcm.Green_function_grid(z, spin_down=spin, label=la)
             Gb = qcm.Green_function_grid(z, spin_down=spin, label=lb)
             assert np.abs(Ga - Gb).max() < 1e-8, (vals, spin, np.abs(Ga - Gb).max())
+    # all batched instances evaluated together equal their one-by-one evaluation
+    for spin in (0, 1):
+        Gs = qcm.Green_function_batch(z, batch, spin_down=spin)
+        for lb, G in zip(batch, Gs):
+            assert np.abs(G - qcm.Green_function_grid(z, spin_down=spin, label=lb)).max() < 1e-12
     for la, lb in zip(single, batch):
         qcm.erase_model_instance(la)
         qcm.erase_model_instance(lb)
