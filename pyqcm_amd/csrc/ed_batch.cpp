@@ -16,14 +16,11 @@
 namespace ed {
 namespace {
 
-// items whose sector has at most this many states run in a one-wave workgroup
-constexpr uint32_t kOneWaveDim = 4096;
-
 // packed device copy of many FactorizedHost tables
 struct TablePack {
   std::vector<uint32_t> u32;
   std::vector<double> f64;
-  struct Off { size_t up_ptr, up_col, dn_ptr, dn_col, up_val, dn_val, diag; uint32_t du, dd; };
+  struct Off { size_t up_ptr, up_col, dn_ptr, dn_col, up_val, dn_val, diag; uint32_t du, dd, nu, nd; };
   std::vector<Off> offs;
   DevBuf<uint32_t> du32;
   DevBuf<double> df64;
@@ -50,12 +47,19 @@ struct TablePack {
     o.diag = put64(F.diag);
     o.du = F.du;
     o.dd = F.dd;
+    o.nu = (uint32_t)F.up_col.size();
+    o.nd = (uint32_t)F.dn_col.size();
     offs.push_back(o);
     return (int)offs.size() - 1;
   }
   void upload() {
     du32.upload(u32);
     df64.upload(f64);
+  }
+  // LDS bytes of an item of this sector held in LDS
+  size_t lds(int k, int nvec, size_t ntri) const {
+    const Off &o = offs[k];
+    return batch_lds_bytes(nvec, ntri, o.du, o.dd, o.nu, o.nd);
   }
   BatchSector view(int k) const {
     const Off &o = offs[k];
@@ -73,22 +77,35 @@ struct TablePack {
   }
 };
 
-template <typename Item>
-void launch_split(std::vector<Item> &items, const std::vector<uint32_t> &dims,
-                  void (*launch)(const Item *, int, int, int, double, hipStream_t), int iters, double acc) {
-  // one-wave and four-wave groups, each one launch
-  std::vector<Item> small, large;
-  for (size_t k = 0; k < items.size(); ++k) (dims[k] <= kOneWaveDim ? small : large).push_back(items[k]);
-  DevBuf<Item> ds, dl;
-  if (!small.empty()) {
-    ds.alloc(small.size());
-    ED_HIP(hipMemcpyAsync(ds.p, small.data(), small.size() * sizeof(Item), hipMemcpyHostToDevice, stream()));
-    launch(ds.p, (int)small.size(), 64, iters, acc, stream());
-  }
-  if (!large.empty()) {
-    dl.alloc(large.size());
-    ED_HIP(hipMemcpyAsync(dl.p, large.data(), large.size() * sizeof(Item), hipMemcpyHostToDevice, stream()));
-    launch(dl.p, (int)large.size(), 256, iters, acc, stream());
+// execution class of an item: from LDS with one or four waves, global
+// vectors with one or four waves
+enum Cls { kLds = 0, kLdsWide = 1, kWave = 2, kBlock = 3 };
+constexpr int kClasses = 4;
+
+// items held in LDS with at least this many states get four waves (the
+// steps are latency-bound; more waves per item hide more of it)
+constexpr uint32_t kWideDim = 256;
+
+// items whose sector has at most this many states and does not fit in LDS
+// run in a one-wave workgroup with global vectors
+constexpr uint32_t kOneWaveDim = 4096;
+
+Cls classify(size_t lds, uint32_t dim) {
+  if (lds <= kBatchLdsCap) return dim >= kWideDim ? kLdsWide : kLds;
+  return dim <= kOneWaveDim ? kWave : kBlock;
+}
+bool in_lds(Cls c) { return c == kLds || c == kLdsWide; }
+
+// one launch per non-empty class
+template <typename Item, typename Launch>
+void launch_classes(const std::vector<Item> &items, const std::vector<Cls> &cls, size_t lds_max, Launch launch) {
+  std::vector<Item> part[kClasses];
+  for (size_t k = 0; k < items.size(); ++k) part[cls[k]].push_back(items[k]);
+  DevBuf<Item> d[kClasses];
+  for (int c = 0; c < kClasses; ++c) {
+    if (part[c].empty()) continue;
+    d[c].upload(part[c]);
+    launch(d[c].p, (int)part[c].size(), (c == kBlock || c == kLdsWide) ? 256 : 64, in_lds((Cls)c) ? lds_max : 0);
   }
   sync();
 }
@@ -127,7 +144,7 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
     TablePack tabs;
     struct GSRef { Instance *I; Sector s; int tab; uint32_t dim; };
     std::vector<GSRef> refs;
-    size_t ws_len = 0, gs_len = 0;
+    size_t gs_len = 0;
     for (Instance *I : todo) {
       const TermList T = I->terms();
       for (const auto &s : I->targets) {
@@ -135,34 +152,54 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
         if (dim == 0) continue;
         if (dim > 0xffffffffull) fail(ED_ERR_UNSUPPORTED, "batched sector too large");
         refs.push_back({I, s, tabs.add(factorized_tables(T, s)), (uint32_t)dim});
-        ws_len += 3 * dim;
         gs_len += dim;
       }
     }
     tabs.upload();
-    const size_t nab = 2 * (size_t)(max_iter + 2), ntri = 3 * (size_t)(max_iter + 2);
+    vlog("batch: %zu ground-state tables built and uploaded", refs.size());
+    const size_t ntri = (size_t)kTriArrays * (max_iter + 2);
+    std::vector<Cls> cls(refs.size());
+    size_t lds_max = 0, ws_len = 0, ntri_glob = 0;
+    for (size_t k = 0; k < refs.size(); ++k) {
+      const size_t b = tabs.lds(refs[k].tab, kGsVectors, ntri);
+      cls[k] = classify(b, refs[k].dim);
+      if (in_lds(cls[k])) {
+        lds_max = std::max(lds_max, b);
+      } else {
+        ws_len += 2 * (size_t)refs[k].dim;
+        ++ntri_glob;
+      }
+    }
     DevBuf<double> ws(std::max<size_t>(ws_len, 1)), gsv(std::max<size_t>(gs_len, 1)),
-        ab(std::max<size_t>(nab * refs.size(), 1)), tri(std::max<size_t>(ntri * refs.size(), 1)),
-        out(std::max<size_t>(3 * refs.size(), 1));
+        tri(std::max<size_t>(ntri * ntri_glob, 1)), out(std::max<size_t>(3 * refs.size(), 1));
     std::vector<BatchGSItem> items(refs.size());
-    std::vector<uint32_t> dims(refs.size());
-    size_t wo = 0, go = 0;
+    size_t wo = 0, go = 0, to = 0;
     for (size_t k = 0; k < refs.size(); ++k) {
       BatchGSItem &it = items[k];
       it.S = tabs.view(refs[k].tab);
       it.seed = seed;
-      it.ws = ws.p + wo;
       it.gs = gsv.p + go;
-      it.ab = ab.p + nab * k;
-      it.tri = tri.p + ntri * k;
       it.out = out.p + 3 * k;
-      dims[k] = refs[k].dim;
-      wo += 3 * (size_t)refs[k].dim;
       go += refs[k].dim;
+      if (!in_lds(cls[k])) {
+        it.ws = ws.p + wo;
+        it.tri = tri.p + ntri * to++;
+        wo += 2 * (size_t)refs[k].dim;
+      }
     }
-    launch_split<BatchGSItem>(items, dims, batch_gs, max_iter, accur);
+    launch_classes(items, cls, lds_max, [&](const BatchGSItem *d, int n, int block, size_t lds) {
+      batch_gs(d, n, block, lds, max_iter, accur, stream());
+    });
     const std::vector<double> res = out.to_host();
-    vlog("batch: %zu ground-state items solved", refs.size());
+    if (verbose()) {
+      double sum = 0.0, mx = 0.0;
+      for (size_t k = 0; k < refs.size(); ++k) {
+        sum += res[3 * k + 1];
+        mx = std::max(mx, res[3 * k + 1]);
+      }
+      vlog("batch: %zu ground-state items solved (Lanczos steps: mean %.1f, max %.0f)", refs.size(),
+           refs.empty() ? 0.0 : sum / refs.size(), mx);
+    }
     // states per instance, then the instance's own weighting
     std::map<Instance *, std::vector<std::unique_ptr<State>>> cand;
     for (size_t k = 0; k < refs.size(); ++k) {
@@ -177,10 +214,12 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
       cand[refs[k].I].push_back(std::move(st));
     }
     sync();
+    vlog("batch: ground states copied out");
     for (Instance *I : todo) {
       I->states = std::move(cand[I]);
       I->finalize_states();
     }
+    vlog("batch: states finalized");
   }
   if (!want_gf) return;
 
@@ -207,12 +246,13 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
     M->upar.upload(C.upar.empty() ? std::vector<int8_t>{1} : C.upar);
     return *(maps[key] = std::move(M));
   };
-  struct CFRef { Instance *I; State *st; int spin, pm, idx; uint32_t dim; int tab; };
+  struct CFRef { Instance *I; State *st; int spin, pm, idx; uint32_t dim; int tab; int fid; };
   std::vector<CFRef> refs;
   std::vector<BatchCFItem> items;
-  size_t ws_len = 0;
+  auto store = std::make_shared<CfStore>();
+  int nfrac = 0;
   for (Instance *I : cf) {
-    const int L = I->model->L, ns = I->model->n_sites, np = ns * (ns + 1) / 2;
+    const int L = I->model->L, ns = I->model->n_sites;
     const TermList T = I->terms();
     for (auto &stp : I->states) {
       State &st = *stp;
@@ -221,10 +261,9 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
         G = GFRep();
         G.kind = 'F';
         G.n = ns;
-        G.ea.assign(np, {});
-        G.eb.assign(np, {});
-        G.ha.assign(np, {});
-        G.hb.assign(np, {});
+        G.store = store;
+        G.frac0 = nfrac;
+        nfrac += ns * (ns + 1);
         for (int pm = -1; pm <= 1; pm += 2) {
           const Sector tgt = shift_sector(st.sec, pm, spin);
           if (!sector_valid(L, tgt)) continue;
@@ -252,51 +291,78 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
                 it.sgn[1] = m2.sgn.p;
               }
               items.push_back(it);
-              refs.push_back({I, &st, spin, pm, idx, (uint32_t)dt, tab});
-              ws_len += 3 * dt;
+              refs.push_back({I, &st, spin, pm, idx, (uint32_t)dt, tab, G.frac0 + 2 * idx + (pm == 1 ? 0 : 1)});
             }
         }
       }
     }
   }
   tabs.upload();
+  vlog("batch: %zu continued-fraction items, tables and maps built", refs.size());
+  std::vector<Cls> cls(refs.size());
+  size_t lds_max = 0, ws_len = 0;
+  for (size_t k = 0; k < refs.size(); ++k) {
+    const size_t b = tabs.lds(refs[k].tab, kCfVectors, 0);
+    cls[k] = classify(b, refs[k].dim);
+    if (in_lds(cls[k])) lds_max = std::max(lds_max, b);
+    else ws_len += 2 * (size_t)refs[k].dim;
+  }
   DevBuf<double> ws(std::max<size_t>(ws_len, 1)), av(std::max<size_t>((size_t)max_cf * refs.size(), 1)),
       bv(std::max<size_t>((size_t)(max_cf + 1) * refs.size(), 1)), out(std::max<size_t>(2 * refs.size(), 1));
-  std::vector<uint32_t> dims(refs.size());
   size_t wo = 0;
   for (size_t k = 0; k < refs.size(); ++k) {
     BatchCFItem &it = items[k];
     it.S = tabs.view(refs[k].tab);
-    it.ws = ws.p + wo;
     it.a = av.p + (size_t)max_cf * k;
     it.b = bv.p + (size_t)(max_cf + 1) * k;
     it.out = out.p + 2 * k;
-    dims[k] = refs[k].dim;
-    wo += 3 * (size_t)refs[k].dim;
-  }
-  launch_split<BatchCFItem>(items, dims, batch_cf, max_cf, acc_cf);
-  const std::vector<double> res = out.to_host(), ah = av.to_host(), bh = bv.to_host();
-  vlog("batch: %zu continued fractions built", refs.size());
-  // continued_fraction transform (src_ed/continued_fraction.cpp:33-42), as build_cf
-  for (size_t k = 0; k < refs.size(); ++k) {
-    const int n = (int)res[2 * k];
-    if (n < 0) continue;
-    const CFRef &r = refs[k];
-    std::vector<double> a(ah.begin() + (size_t)max_cf * k, ah.begin() + (size_t)max_cf * k + n);
-    std::vector<double> b(bh.begin() + (size_t)(max_cf + 1) * k, bh.begin() + (size_t)(max_cf + 1) * k + n + 1);
-    for (auto &x : b) x *= x;
-    b[0] = res[2 * k + 1] * r.st->weight;
-    if (r.pm == 1) {
-      for (auto &x : a) x -= r.st->energy;
-      r.st->gf[r.spin].ea[r.idx] = std::move(a);
-      r.st->gf[r.spin].eb[r.idx] = std::move(b);
-    } else {
-      for (auto &x : a) x = r.st->energy - x;
-      r.st->gf[r.spin].ha[r.idx] = std::move(a);
-      r.st->gf[r.spin].hb[r.idx] = std::move(b);
+    if (!in_lds(cls[k])) {
+      it.ws = ws.p + wo;
+      wo += 2 * (size_t)refs[k].dim;
     }
   }
+  launch_classes(items, cls, lds_max, [&](const BatchCFItem *d, int n, int block, size_t lds) {
+    batch_cf(d, n, block, lds, max_cf, acc_cf, stream());
+  });
+  const std::vector<double> res = out.to_host();
+  // the store: fraction f holds n_f coefficients (0 where psi vanished or no
+  // target sector exists)
+  std::vector<int> len(nfrac + 1, 0), fid(refs.size());
+  std::vector<double> e0(refs.size()), wt(refs.size());
+  std::vector<int8_t> pmv(refs.size());
+  for (size_t k = 0; k < refs.size(); ++k) {
+    const CFRef &r = refs[k];
+    len[r.fid] = std::max(0, (int)res[2 * k]);
+    fid[k] = r.fid;
+    e0[k] = r.st->energy;
+    wt[k] = r.st->weight;
+    pmv[k] = (int8_t)r.pm;
+  }
+  std::vector<int> off(nfrac + 1, 0);
+  size_t total = 0;
+  for (int f = 0; f < nfrac; ++f) {
+    off[f] = (int)total;
+    total += (size_t)len[f];
+    if (total > 0x7fffffffull) fail(ED_ERR_UNSUPPORTED, "batched continued fractions: too many coefficients");
+  }
+  off[nfrac] = (int)total;
+  store->nfrac = nfrac;
+  store->off.upload(off);
+  store->A.alloc(std::max<size_t>(total, 1));
+  store->B.alloc(std::max<size_t>(total, 1));
+  DevBuf<int> dfid;
+  DevBuf<double> de0, dwt;
+  DevBuf<int8_t> dpm;
+  dfid.upload(fid);
+  de0.upload(e0);
+  dwt.upload(wt);
+  dpm.upload(pmv);
+  batch_cf_store((int)refs.size(), max_cf, out.p, av.p, bv.p, dfid.p, store->off.p, de0.p, dwt.p, dpm.p,
+                 store->A.p, store->B.p, stream());
+  sync();
+  vlog("batch: %zu continued fractions built and stored on the device (%zu coefficients)", refs.size(), total);
   for (Instance *I : cf) I->gf_solved = true;
+  vlog("batch: continued fractions stored");
 }
 
 }  // namespace ed

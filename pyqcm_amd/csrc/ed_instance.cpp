@@ -560,32 +560,6 @@ void Instance::green_function_solve() {
 // the representation's fractions start at frac0, two per orbital pair
 // (electron, hole), pairs o1 >= o2 in order.  G_aa = F_aa, G_ab =
 // (F_ab - G_aa - G_bb) / 2, added to acc (nz blocks n x n, column-major).
-static void assemble_cf(const double *h, int ncf, int frac0, int n, int nz, double *acc) {
-  for (int iz = 0; iz < nz; ++iz) {
-    auto F = [&](int q) {
-      const double *p0 = &h[2 * ((size_t)iz * ncf + frac0 + 2 * q)];
-      return cplx(p0[0] + p0[2], p0[1] + p0[3]);
-    };
-    std::vector<cplx> Gd(n);
-    int q = 0;
-    for (int o1 = 0; o1 < n; ++o1)
-      for (int o2 = 0; o2 <= o1; ++o2, ++q)
-        if (o1 == o2) Gd[o1] = F(q);
-    double *g = &acc[(size_t)iz * n * n * 2];
-    q = 0;
-    for (int o1 = 0; o1 < n; ++o1)
-      for (int o2 = 0; o2 <= o1; ++o2, ++q) {
-        cplx v = (o1 == o2) ? Gd[o1] : 0.5 * (F(q) - Gd[o1] - Gd[o2]);
-        g[2 * (o1 + (size_t)o2 * n)] += v.real();
-        g[2 * (o1 + (size_t)o2 * n) + 1] += v.imag();
-        if (o1 != o2) {
-          g[2 * (o2 + (size_t)o1 * n)] += v.real();
-          g[2 * (o2 + (size_t)o1 * n) + 1] += v.imag();
-        }
-      }
-  }
-}
-
 // the fractions of one representation, appended to a packed list
 static void pack_cf(const GFRep &G, int n, std::vector<int> &off, std::vector<double> &A, std::vector<double> &B) {
   const int np = n * (n + 1) / 2;
@@ -597,6 +571,96 @@ static void pack_cf(const GFRep &G, int n, std::vector<int> &off, std::vector<do
       B.insert(B.end(), b.begin(), b.begin() + a.size());
       off.push_back((int)A.size());
     }
+}
+
+// continued-fraction representations summed into one G block of n x n per
+// frequency, at offset `at` (doubles) of the output
+struct CfGroup {
+  std::vector<const GFRep *> reps;
+  int n = 0;
+  size_t at = 0;
+};
+
+// the groups' fractions evaluated in one launch and assembled on the device
+// in a second one, into dG (device memory); host-held fractions are packed
+// and uploaded first, store-held ones are evaluated where they lie (one
+// evaluation per store over the range of fractions the groups use)
+static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const double *dz, double *dG) {
+  std::map<const CfStore *, std::vector<const CfGroup *>> by_store;
+  for (const CfGroup &g : groups) {
+    const CfStore *st = g.reps.empty() ? nullptr : g.reps[0]->store.get();
+    for (const GFRep *r : g.reps)
+      if (r->store.get() != st) fail(ED_ERR_INTERNAL, "continued fractions of one Green function held apart");
+    by_store[st].push_back(&g);
+  }
+  for (auto &kv : by_store) {
+    const CfStore *st = kv.first;
+    std::vector<int> nn, pbeg{0}, frac0;
+    std::vector<uint64_t> at;
+    DevBuf<int> doff;
+    DevBuf<double> dA, dB;
+    const int *off = nullptr;
+    const double *A = nullptr, *B = nullptr;
+    int ncf = 0;
+    if (st) {
+      int fb = st->nfrac, fe = 0;
+      for (const CfGroup *g : kv.second)
+        for (const GFRep *r : g->reps) {
+          fb = std::min(fb, r->frac0);
+          fe = std::max(fe, r->frac0 + g->n * (g->n + 1));
+        }
+      if (fe <= fb) fb = fe = 0;
+      for (const CfGroup *g : kv.second) {
+        for (const GFRep *r : g->reps) frac0.push_back(r->frac0 - fb);
+        pbeg.push_back((int)frac0.size());
+        nn.push_back(g->n);
+        at.push_back(g->at);
+      }
+      ncf = fe - fb;
+      off = st->off.p + fb;
+      A = st->A.p;
+      B = st->B.p;
+    } else {
+      size_t na = 0, nf = 0;
+      for (const CfGroup *g : kv.second)
+        for (const GFRep *r : g->reps)
+          for (size_t q = 0; q < r->ea.size(); ++q) {
+            na += r->ea[q].size() + r->ha[q].size();
+            nf += 2;
+          }
+      std::vector<int> hoff;
+      std::vector<double> hA, hB;
+      hoff.reserve(nf + 1);
+      hoff.push_back(0);
+      hA.reserve(na);
+      hB.reserve(na);
+      for (const CfGroup *g : kv.second) {
+        for (const GFRep *r : g->reps) {
+          frac0.push_back((int)hoff.size() - 1);
+          pack_cf(*r, g->n, hoff, hA, hB);
+        }
+        pbeg.push_back((int)frac0.size());
+        nn.push_back(g->n);
+        at.push_back(g->at);
+      }
+      ncf = (int)hoff.size() - 1;
+      doff.upload(hoff);
+      dA.upload(hA);
+      dB.upload(hB);
+      off = doff.p;
+      A = dA.p;
+      B = dB.p;
+    }
+    DevBuf<int> dn, dpb, dfr;
+    DevBuf<uint64_t> dat;
+    DevBuf<double> dV((size_t)nz * ncf * 2);
+    dn.upload(nn);
+    dpb.upload(pbeg);
+    dfr.upload(frac0);
+    dat.upload(at);
+    cf_eval(ncf, off, A, B, nz, dz, dV.p, stream());
+    cf_assemble((int)kv.second.size(), dn.p, dat.p, dpb.p, dfr.p, ncf, nz, dV.p, dG, stream());
+  }
 }
 
 int Instance::gf_spin(bool spin_down) const {
@@ -614,37 +678,34 @@ void green_function_batch(const std::vector<Instance *> &insts, int nz, const do
     base.push_back(tot);
     tot += (size_t)nz * I->gf_dim() * I->gf_dim() * 2;
   }
-  std::fill(out, out + tot, 0.0);
-  // every continued-fraction representation goes into one evaluation launch
-  struct Part { size_t at; int frac0, n; };
-  std::vector<Part> parts;
-  std::vector<int> off{0};
-  std::vector<double> A, B;
+  // instances whose states are all continued fractions: evaluated and
+  // assembled together on the device, copied out once
+  std::vector<CfGroup> groups;
+  std::vector<size_t> other;
   for (size_t k = 0; k < insts.size(); ++k) {
     Instance &I = *insts[k];
-    const int spin = I.gf_spin(spin_down), n = I.gf_dim();
+    const int spin = I.gf_spin(spin_down);
+    CfGroup g;
+    g.n = I.gf_dim();
+    g.at = base[k];
     bool all_cf = true;
-    for (auto &s : I.states) all_cf = all_cf && s->gf[spin].kind == 'F';
-    if (!all_cf) {
-      I.green_function(nz, z, spin_down, out + base[k]);
-      continue;
-    }
     for (auto &s : I.states) {
-      parts.push_back({base[k], (int)off.size() - 1, n});
-      pack_cf(s->gf[spin], n, off, A, B);
+      all_cf = all_cf && s->gf[spin].kind == 'F';
+      g.reps.push_back(&s->gf[spin]);
     }
+    if (all_cf) groups.push_back(std::move(g));
+    else other.push_back(k);
   }
-  if (parts.empty()) return;
-  const int ncf = (int)off.size() - 1;
-  DevBuf<int> doff;
-  DevBuf<double> dA, dB, dz, dO((size_t)nz * ncf * 2);
-  doff.upload(off);
-  dA.upload(A.empty() ? std::vector<double>{0.0} : A);
-  dB.upload(B.empty() ? std::vector<double>{0.0} : B);
-  dz.upload(z, (size_t)2 * nz);
-  cf_eval(ncf, doff.p, dA.p, dB.p, nz, dz.p, dO.p, stream());
-  const std::vector<double> h = dO.to_host();
-  for (const Part &pt : parts) assemble_cf(h.data(), ncf, pt.frac0, pt.n, nz, out + pt.at);
+  if (!groups.empty()) {
+    DevBuf<double> dz, dG(tot);
+    dz.upload(z, (size_t)2 * nz);
+    eval_cf_groups(groups, nz, dz.p, dG.p);
+    vlog("gf batch: %zu instances evaluated", groups.size());
+    dG.download(out, tot);
+  }
+  // the rest (written after the copy, which leaves their ranges undefined)
+  for (size_t k : other) insts[k]->green_function(nz, z, spin_down, out + base[k]);
+  vlog("gf batch: assembled");
 }
 
 void Instance::green_function(int nz, const double *z, bool spin_down, double *out) {
@@ -666,19 +727,13 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
       std::vector<double> h = dG.to_host();
       for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
     } else if (G.kind == 'F') {
-      // pack every fraction (electron, hole per pair) into one launch
-      std::vector<int> off{0};
-      std::vector<double> A, B;
-      pack_cf(G, n, off, A, B);
-      const int ncf = (int)off.size() - 1;
-      DevBuf<int> doff;
-      DevBuf<double> dA, dB, dO((size_t)nz * ncf * 2);
-      doff.upload(off);
-      dA.upload(A.empty() ? std::vector<double>{0.0} : A);
-      dB.upload(B.empty() ? std::vector<double>{0.0} : B);
-      cf_eval(ncf, doff.p, dA.p, dB.p, nz, dz.p, dO.p, stream());
-      std::vector<double> h = dO.to_host();
-      assemble_cf(h.data(), ncf, 0, n, nz, acc.data());
+      CfGroup g;
+      g.n = n;
+      g.reps.push_back(&G);
+      DevBuf<double> dG((size_t)nz * n * n * 2);
+      eval_cf_groups({g}, nz, dz.p, dG.p);
+      std::vector<double> h = dG.to_host();
+      for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
     } else if (G.kind == 'M') {
       for (int eh = 0; eh < 2; ++eh) {
         const GFRep::MCF &mc = eh ? G.mh : G.me;

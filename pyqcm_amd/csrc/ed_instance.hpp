@@ -14,6 +14,15 @@
 namespace ed {
 
 // Green-function representation of one state and one spin
+// continued fractions packed on the device, already transformed: fraction
+// f's a at A[off[f], off[f+1]) and its b_0 .. b_{n-1} at the same positions
+// of B (the evaluation never reads b_n)
+struct CfStore {
+  DevBuf<double> A, B;
+  DevBuf<int> off;  // nfrac + 1, device
+  int nfrac = 0;
+};
+
 struct GFRep {
   char kind = 0;  // 'L' Lehmann (Q-matrix), 'F' continued fractions, 'M' matrix continued fractions
   int n = 0;      // GF dimension (cluster orbitals)
@@ -22,6 +31,10 @@ struct GFRep {
   // continued fractions per ordered pair (o1 >= o2) for electrons / holes; a, b
   // already transformed (continued_fraction ctor, src_ed/continued_fraction.cpp:33-42)
   std::vector<std::vector<double>> ea, eb, ha, hb;
+  // or, when `store` is set, fractions frac0 + 2 q (electron) and frac0 + 2 q + 1
+  // (hole) of that device store, with ea .. hb empty
+  std::shared_ptr<const CfStore> store;
+  int frac0 = 0;
   // matrix continued fractions: floors A, B (p x p), weight W (p x n)
   struct MCF { int p = 0, nfl = 0; std::vector<double> A, B, W; };
   MCF me, mh;

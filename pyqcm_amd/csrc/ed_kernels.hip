@@ -1247,12 +1247,52 @@ void cdag_gather(const CdagArgs &A, const double *x, double *y, hipStream_t s) {
   ED_HIP(hipGetLastError());
 }
 
+// one thread per (group, frequency, matrix element)
+__global__ void k_cf_assemble(int ngroups, const int *nn, const uint64_t *at, const int *pbeg, const int *frac0,
+                              int ncf, int nz, const double *vals, double *G) {
+  for (int g = blockIdx.y; g < ngroups; g += gridDim.y) {
+    const int n = nn[g];
+    const size_t nsq = (size_t)n * n, m = nsq * nz;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += (size_t)gridDim.x * blockDim.x) {
+      const int iz = (int)(t / nsq), e = (int)(t - (size_t)iz * nsq);
+      const int o1 = e % n, o2 = e / n;
+      const int a = o1 > o2 ? o1 : o2, b = o1 > o2 ? o2 : o1;
+      const int qab = a * (a + 1) / 2 + b, qaa = a * (a + 1) / 2 + a, qbb = b * (b + 1) / 2 + b;
+      const double *row = vals + 2 * (size_t)iz * ncf;
+      double gr = 0.0, gi = 0.0;
+      for (int p = pbeg[g]; p < pbeg[g + 1]; ++p) {
+        const double *f = row + 2 * (size_t)frac0[p];
+        const double ar = f[4 * qaa] + f[4 * qaa + 2], ai = f[4 * qaa + 1] + f[4 * qaa + 3];
+        if (a == b) {
+          gr += ar;
+          gi += ai;
+        } else {
+          const double br = f[4 * qbb] + f[4 * qbb + 2], bi = f[4 * qbb + 1] + f[4 * qbb + 3];
+          const double cr = f[4 * qab] + f[4 * qab + 2], ci = f[4 * qab + 1] + f[4 * qab + 3];
+          gr += 0.5 * (cr - ar - br);
+          gi += 0.5 * (ci - ai - bi);
+        }
+      }
+      G[at[g] + 2 * t] = gr;
+      G[at[g] + 2 * t + 1] = gi;
+    }
+  }
+}
+
 void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out,
              hipStream_t s) {
   const size_t n = (size_t)ncf * nz;
   if (!n) return;
   if ((n + 255) / 256 > 0x7fffffffull) fail(ED_ERR_INVALID, "cf_eval: too many fractions x frequencies");
   hipLaunchKernelGGL(k_cf_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ncf, off, a, b, nz, z, out);
+  ED_HIP(hipGetLastError());
+}
+
+void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, int ncf, int nz,
+                 const double *vals, double *G, hipStream_t s) {
+  if (ngroups <= 0 || nz <= 0) return;
+  const dim3 grid(64, (unsigned)(ngroups < 65535 ? ngroups : 65535));
+  hipLaunchKernelGGL(k_cf_assemble, grid, dim3(256), 0, s, ngroups, n, at, pbeg, frac0, ncf, nz, vals, G);
   ED_HIP(hipGetLastError());
 }
 

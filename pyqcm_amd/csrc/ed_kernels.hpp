@@ -143,6 +143,14 @@ void cdag_gather(const CdagArgs &a, const double *x, double *y, hipStream_t s);
 // continued fractions: out[iz * ncf + f] = G_f(z_iz)
 void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, const double *z, double *out,
              hipStream_t s);
+// G matrices of groups of states from their evaluated fractions (vals, as
+// cf_eval writes them): group g owns states [pbeg[g], pbeg[g+1]); state p's
+// fractions start at frac0[p], two per orbital pair q = o1 (o1 + 1) / 2 + o2
+// (o1 >= o2), electron then hole.  G_aa = F_aa, G_ab = (F_ab - F_aa - F_bb) / 2
+// with F = electron + hole, summed over the group's states, written at
+// G + at[g] (doubles) as nz column-major n[g] x n[g] complex matrices
+void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, int ncf, int nz,
+                 const double *vals, double *G, hipStream_t s);
 // Lehmann sum per z: G[iz][b + a L] = sum_i V[a,i] V[b,i] / (z - e_i)
 void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s);
 // matrix continued fraction per z (p <= 16)
