@@ -84,7 +84,7 @@ constexpr int kClasses = 4;
 
 // items held in LDS with at least this many states get four waves (the
 // steps are latency-bound; more waves per item hide more of it)
-constexpr uint32_t kWideDim = 256;
+constexpr uint32_t kWideDim = 128;
 
 // items whose sector has at most this many states and does not fit in LDS
 // run in a one-wave workgroup with global vectors
@@ -109,6 +109,93 @@ void launch_classes(const std::vector<Item> &items, const std::vector<Cls> &cls,
   }
   sync();
 }
+
+// The one-body tables and the diagonal are linear in the term values: the
+// tables of every operator of a term list at unit value, merged on the union
+// of their patterns, give each instance's tables as a weighted sum.
+struct UnitTables {
+  FactorizedHost pat;                             // union pattern (values unused)
+  std::vector<std::vector<double>> up, dn, diag;  // per operator, on pat
+};
+
+UnitTables unit_tables(const TermList &T, const Sector &sec) {
+  UnitTables U;
+  std::vector<FactorizedHost> F;
+  for (const auto &tc : T.terms) {
+    TermList t1;
+    t1.L = T.L;
+    t1.terms = {{tc.first, 1.0}};
+    F.push_back(factorized_tables(t1, sec));
+  }
+  U.pat.du = F[0].du;
+  U.pat.dd = F[0].dd;
+  // union of the rows of one spin, then every operator's values on it
+  auto merge = [&](bool up, std::vector<uint32_t> &ptr, std::vector<uint32_t> &col,
+                   std::vector<std::vector<double>> &val) {
+    const uint32_t nrow = up ? U.pat.du : U.pat.dd;
+    ptr.assign(1, 0);
+    std::vector<uint32_t> row;
+    for (uint32_t r = 0; r < nrow; ++r) {
+      row.clear();
+      for (const FactorizedHost &f : F) {
+        const auto &fp = up ? f.up_ptr : f.dn_ptr;
+        const auto &fc = up ? f.up_col : f.dn_col;
+        row.insert(row.end(), fc.begin() + fp[r], fc.begin() + fp[r + 1]);
+      }
+      std::sort(row.begin(), row.end());
+      row.erase(std::unique(row.begin(), row.end()), row.end());
+      col.insert(col.end(), row.begin(), row.end());
+      ptr.push_back((uint32_t)col.size());
+    }
+    val.assign(F.size(), std::vector<double>(col.size(), 0.0));
+    for (size_t k = 0; k < F.size(); ++k) {
+      const auto &fp = up ? F[k].up_ptr : F[k].dn_ptr;
+      const auto &fc = up ? F[k].up_col : F[k].dn_col;
+      const auto &fv = up ? F[k].up_val : F[k].dn_val;
+      for (uint32_t r = 0; r < nrow; ++r)
+        for (uint32_t e = fp[r]; e < fp[r + 1]; ++e) {
+          const auto b = col.begin() + ptr[r], en = col.begin() + ptr[r + 1];
+          val[k][std::lower_bound(b, en, fc[e]) - col.begin()] += fv[e];
+        }
+    }
+  };
+  merge(true, U.pat.up_ptr, U.pat.up_col, U.up);
+  merge(false, U.pat.dn_ptr, U.pat.dn_col, U.dn);
+  for (const FactorizedHost &f : F) U.diag.push_back(f.diag);
+  return U;
+}
+
+// the tables of term list T in sector sec (same operators, in the same
+// order, as the list the unit tables were built from)
+FactorizedHost combine_tables(const UnitTables &U, const TermList &T) {
+  FactorizedHost F = U.pat;
+  auto sum = [&](const std::vector<std::vector<double>> &u, std::vector<double> &out) {
+    out.assign(u[0].size(), 0.0);
+    for (size_t k = 0; k < u.size(); ++k) {
+      const double v = T.terms[k].second;
+      if (v == 0.0) continue;
+      for (size_t i = 0; i < out.size(); ++i) out[i] += v * u[k][i];
+    }
+  };
+  sum(U.up, F.up_val);
+  sum(U.dn, F.dn_val);
+  sum(U.diag, F.diag);
+  return F;
+}
+
+// unit tables per (operator list, sector), built once per batched solve
+struct TableCache {
+  std::map<std::pair<std::vector<const Operator *>, Sector>, UnitTables> m;
+  FactorizedHost get(const TermList &T, const Sector &sec) {
+    if (T.terms.empty()) return factorized_tables(T, sec);
+    std::vector<const Operator *> ops;
+    for (const auto &tc : T.terms) ops.push_back(tc.first);
+    auto key = std::make_pair(std::move(ops), sec);
+    auto it = m.find(key);
+    if (it == m.end()) it = m.emplace(key, unit_tables(T, sec)).first;
+    return combine_tables(it->second, T);
+  }
+};
 
 bool batchable(const Instance &I) {
   const Params &P = params();
@@ -139,6 +226,7 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
   }
   vlog("batch: %zu instances batched, %zu solved one by one", todo.size(), insts.size() - todo.size());
 
+  TableCache cache;
   // ---- ground states: one item per (instance, target sector) ----
   if (!todo.empty()) {
     TablePack tabs;
@@ -151,7 +239,7 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
         const size_t dim = sector_dim(I->model->L, s);
         if (dim == 0) continue;
         if (dim > 0xffffffffull) fail(ED_ERR_UNSUPPORTED, "batched sector too large");
-        refs.push_back({I, s, tabs.add(factorized_tables(T, s)), (uint32_t)dim});
+        refs.push_back({I, s, tabs.add(cache.get(T, s)), (uint32_t)dim});
         gs_len += dim;
       }
     }
@@ -170,15 +258,17 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
         ++ntri_glob;
       }
     }
-    DevBuf<double> ws(std::max<size_t>(ws_len, 1)), gsv(std::max<size_t>(gs_len, 1)),
-        tri(std::max<size_t>(ntri * ntri_glob, 1)), out(std::max<size_t>(3 * refs.size(), 1));
+    // the ground states stay here; each state's vector is a view kept alive by it
+    auto gsv = std::make_shared<DevBuf<double>>(std::max<size_t>(gs_len, 1));
+    DevBuf<double> ws(std::max<size_t>(ws_len, 1)), tri(std::max<size_t>(ntri * ntri_glob, 1)),
+        out(std::max<size_t>(3 * refs.size(), 1));
     std::vector<BatchGSItem> items(refs.size());
     size_t wo = 0, go = 0, to = 0;
     for (size_t k = 0; k < refs.size(); ++k) {
       BatchGSItem &it = items[k];
       it.S = tabs.view(refs[k].tab);
       it.seed = seed;
-      it.gs = gsv.p + go;
+      it.gs = gsv->p + go;
       it.out = out.p + 3 * k;
       go += refs[k].dim;
       if (!in_lds(cls[k])) {
@@ -209,8 +299,7 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
       auto st = std::make_unique<State>();
       st->sec = refs[k].s;
       st->energy = res[3 * k];
-      st->psi.alloc(refs[k].dim);
-      ED_HIP(hipMemcpyAsync(st->psi.p, items[k].gs, sizeof(double) * refs[k].dim, hipMemcpyDeviceToDevice, stream()));
+      st->psi = DevBuf<double>::view(items[k].gs, refs[k].dim, gsv);
       cand[refs[k].I].push_back(std::move(st));
     }
     sync();
@@ -271,8 +360,10 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
           if (dt == 0) continue;
           auto key = std::make_pair(I, tgt);
           auto ti = tab_of.find(key);
-          const int tab = ti != tab_of.end() ? ti->second : (tab_of[key] = tabs.add(factorized_tables(T, tgt)));
+          const int tab = ti != tab_of.end() ? ti->second : (tab_of[key] = tabs.add(cache.get(T, tgt)));
           const uint32_t du_s = half_basis(L, st.sec.Nup())->dim;
+          std::vector<MapSet *> mo(ns);
+          for (int o = 0; o < ns; ++o) mo[o] = &get_maps(L, st.sec, pm, o + spin * L);
           int idx = 0;
           for (int o1 = 0; o1 < ns; ++o1)
             for (int o2 = 0; o2 <= o1; ++o2, ++idx) {
@@ -281,12 +372,12 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
               it.du_s = du_s;
               it.spin = spin;
               it.norb = o2 == o1 ? 1 : 2;
-              MapSet &m1 = get_maps(L, st.sec, pm, o1 + spin * L);
+              MapSet &m1 = *mo[o1];
               it.map[0] = m1.map.p;
               it.sgn[0] = m1.sgn.p;
               it.upar = m1.upar.p;
               if (o2 != o1) {
-                MapSet &m2 = get_maps(L, st.sec, pm, o2 + spin * L);
+                MapSet &m2 = *mo[o2];
                 it.map[1] = m2.map.p;
                 it.sgn[1] = m2.sgn.p;
               }

@@ -1,6 +1,8 @@
 // ed_common.cpp -- global parameters, caching device allocator, library stream.
 #include "ed_common.hpp"
 
+#include <cstring>
+
 #include <chrono>
 #include <cstdarg>
 #include <unordered_map>
@@ -141,6 +143,45 @@ void dev_free(void *p, size_t bytes) {
   std::lock_guard<std::mutex> l(g_alloc_mu);
   g_free.emplace(b, p);
   g_cached_bytes += b;
+}
+
+void d2h_pageable(void *dst, const void *src, size_t bytes) {
+  constexpr size_t kChunk = size_t(8) << 20;
+  if (bytes <= kChunk) {
+    if (bytes) ED_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    ED_HIP(hipStreamSynchronize(stream()));
+    return;
+  }
+  static std::mutex mu;
+  static char *pin[2] = {nullptr, nullptr};
+  static hipEvent_t ev[2];
+  std::lock_guard<std::mutex> lock(mu);
+  if (!pin[0])
+    for (int i = 0; i < 2; ++i) {
+      ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&pin[i]), kChunk, hipHostMallocDefault));
+      ED_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+  const size_t nch = (bytes + kChunk - 1) / kChunk;
+  auto len = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
+  auto issue = [&](size_t c) {
+    ED_HIP(hipMemcpyAsync(pin[c % 2], static_cast<const char *>(src) + c * kChunk, len(c), hipMemcpyDeviceToHost,
+                          stream()));
+    ED_HIP(hipEventRecord(ev[c % 2], stream()));
+  };
+  auto drain = [&](size_t c) {
+    ED_HIP(hipEventSynchronize(ev[c % 2]));
+    char *d = static_cast<char *>(dst) + c * kChunk;
+    const char *h = pin[c % 2];
+    const size_t m = len(c), blk = size_t(1) << 20, nb = (m + blk - 1) / blk;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nb; ++i) std::memcpy(d + i * blk, h + i * blk, std::min(blk, m - i * blk));
+  };
+  issue(0);
+  for (size_t c = 1; c < nch; ++c) {
+    issue(c);
+    drain(c - 1);
+  }
+  drain(nch - 1);
 }
 
 hipStream_t stream() {

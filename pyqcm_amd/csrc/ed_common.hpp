@@ -61,27 +61,50 @@ void dev_free(void *p, size_t bytes);
 void dev_release_cache();
 hipStream_t stream();  // the library's compute stream (default priority)
 
+// device-to-host copy into pageable memory through pinned staging buffers
+// (transfers of one chunk overlap the host copy of the previous one);
+// synchronizes the library stream
+void d2h_pageable(void *dst, const void *src, size_t bytes);
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  // set for a view: p lies in memory that `owner` keeps alive, and is not freed here
+  std::shared_ptr<void> owner;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), owner(std::move(o.owner)) { o.p = nullptr; o.n = 0; }
   DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      owner = std::move(o.owner);
+      o.p = nullptr;
+      o.n = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
+  // count elements at q inside memory kept alive by keep
+  static DevBuf view(T *q, size_t count, std::shared_ptr<void> keep) {
+    DevBuf b;
+    b.p = q;
+    b.n = count;
+    b.owner = std::move(keep);
+    return b;
+  }
   void alloc(size_t count) {
     release();
     n = count;
     if (count) p = static_cast<T *>(dev_alloc(count * sizeof(T)));
   }
   void release() {
-    if (p) dev_free(p, n * sizeof(T));
+    if (p && !owner) dev_free(p, n * sizeof(T));
+    owner.reset();
     p = nullptr;
     n = 0;
   }
@@ -90,10 +113,7 @@ struct DevBuf {
     if (count) ED_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
   }
   void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
-  void download(T *h, size_t count) const {
-    if (count) ED_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
-    ED_HIP(hipStreamSynchronize(stream()));
-  }
+  void download(T *h, size_t count) const { d2h_pageable(h, p, count * sizeof(T)); }
   std::vector<T> to_host() const {
     std::vector<T> h(n);
     download(h.data(), n);

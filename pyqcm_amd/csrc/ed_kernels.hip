@@ -824,8 +824,11 @@ __global__ void k_cf_eval(int ncf, const int *off, const double *a, const double
   const double zr = z[2 * iz], zi = z[2 * iz + 1];
   double gr = 0.0, gi = 0.0;
   for (int i = off[f + 1] - 1; i >= off[f]; --i) {
-    const double dr = zr - a[i] - gr, di = zi - gi;
-    const double s = b[i] / (dr * dr + di * di);
+    const double dr = zr - a[i] - gr, di = zi - gi, den = dr * dr + di * di;
+    // 1 / den: hardware estimate, one Newton step
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    const double s = b[i] * r;
     gr = dr * s;
     gi = -di * s;
   }
