@@ -94,8 +94,13 @@ int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, 
 int ed_states(int label, int cap, double *energy, double *weight, int *n);
 int ed_mixing(int label, int *mixing);
 
-/* ---- batched solve (north-star batching of sectors / parameter points) ----
-   Solves ground state + Green function of n instances, sharing the device. */
+/* ---- batched solve of many instances (parameter points of a VCA / CDMFT sweep) ----
+   Ground states of n instances (flags & 1: also their Green functions).  The
+   reference solves one instance per call (ED::ground_state_solve,
+   src_ed/qcm_ED.cpp; model_instance::low_energy_states,
+   src_ed/model_instance.hpp:245-417); instances whose operators factorize by
+   spin, with Ground_state_method 'L', n_states <= 1 and GF_method 'F' or 'L',
+   run as one launch per stage, the others through their own solve. */
 int ed_solve_batch(const int *labels, int n, int flags);
 
 /* ---- inner plug-in: Hamiltonian<T> (src_ed/Hamiltonian/Hamiltonian_base.hpp:43-82),

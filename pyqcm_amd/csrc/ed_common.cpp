@@ -47,9 +47,9 @@ Params::Params() {
   c["GF_method"] = 'L';
   // device-side knobs (no reference counterpart)
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
-  i["sell_sort_window"] = 256;
+  i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
   i["reduction_blocks_per_cu"] = 4;  // grid of the Krylov vector kernels (multi-dot, axpy, combine)
-  i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads    // two-pass up table: rows sorted by length within windows of this many states
+  i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
