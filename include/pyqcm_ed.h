@@ -160,6 +160,46 @@ int ed_vdvh(int L, int M, const double *e, const double *V, int nz, const double
 int ed_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz,
                 const double *z, double *G);
 
+/* ---- lattice sums over the superlattice Brillouin zone (CPT, VCA, CDMFT host) ----
+   One spin block of the superlattice one-body part of a lattice model:
+   V(k) = V0 + sum_e val_e exp(2 pi i k . R[nbr_e]) at (row_e, col_e), k in reduced
+   coordinates of the superlattice, i.e. V = t(k) - t' as lattice_model_instance::set_V
+   builds it (src_qcm/CPT.cpp:22-60; V0 = H - Hc, the elements are the operators'
+   IGF_elem times their values).  Matrices are d x d complex, column-major, (re, im)
+   pairs; d is the instance's Green-function dimension (at most 8). */
+typedef struct {
+  int d;
+  const double *V0;  /* d x d complex */
+  int ne;
+  const int *row, *col, *nbr; /* ne each, 0-based */
+  const double *val;          /* ne complex */
+  int nnbr;                   /* at most 64 */
+  const double *R;            /* nnbr x 3 */
+} ed_superlattice;
+/* wavevector grid: `dim` (0..3) axes of nk[a] points at (i + offset) / nk[a]: offset 0.5
+   is projected_Green_function's grid (src_qcm/CPT.cpp:283-343), 0 is k_integral_grid's
+   (src_util/integrate.cpp:177-235); dim 0 is the single point k = 0 */
+typedef struct {
+  int dim;
+  int nk[3];
+  double offset;
+} ed_kgrid;
+/* lattice_model_instance::projected_Green_function (src_qcm/CPT.cpp:283-343) for one
+   cluster: for each of the nz frequencies, (1/nk) sum_k G (1 - V(k) G)^{-1} with G the
+   instance's Green function; out as ed_green_function. */
+int ed_cpt_projected_gf(int label, int nz, const double *z, int spin_down, const ed_superlattice *V,
+                        const ed_kgrid *K, double *out);
+/* the (frequency, k) grid path of lattice_model_instance::Potthoff_functional
+   (src_qcm/lattice_model_instance.cpp:503-584) for one cluster: *integral = sum_n
+   weights[n] (1/nk) sum_k I(i w[n], k), I = log|det(V(k) G - 1)| summed over the spin
+   blocks (SEF_integrand_k, :624-650: twice the up block without spin mixing; Vdn is
+   required with up/down mixing and ignored otherwise).  per_freq (may be NULL)
+   receives the nw k-averages. */
+int ed_sef_grid(int label, int nw, const double *w, const double *weights, const ed_superlattice *Vup,
+                const ed_superlattice *Vdn, const ed_kgrid *K, double *integral, double *per_freq);
+/* ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term */
+int ed_potthoff_functional(int label, double *omega);
+
 /* ---- device memory helpers for callers that keep vectors resident ---- */
 int ed_device_alloc(int64_t bytes, void **ptr);
 int ed_device_free(void *ptr, int64_t bytes);

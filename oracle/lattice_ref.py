@@ -1,0 +1,85 @@
+"""numpy restatement of the lattice sums built on a cluster Green function.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ and the cpu_baseline leg of
+bench.py, as the checker / CPU baseline.  The product package (pyqcm_amd)
+never imports this module.
+
+Follows:
+  lattice_model_instance::set_V       src_qcm/CPT.cpp:22-60
+      V(k) = t(k) - t'; t(k) = H + sum_elements v pv exp(2 pi i k . R_n)
+  lattice_model_instance::set_Gcpt    src_qcm/CPT.cpp:76-86
+      G_cpt = G (1 - V G)^{-1}
+  projected_Green_function            src_qcm/CPT.cpp:283-343
+      midpoint grid (i + 0.5) / side, average over the grid
+  SEF_integrand_k                     src_qcm/lattice_model_instance.cpp:624-650
+      log|det(V G - 1)|, twice for spin-degenerate blocks
+  QCM::k_integral_grid                src_util/integrate.cpp:177-235
+      corner grid i / nk, average over the grid
+  Potthoff_functional (grid path)     src_qcm/lattice_model_instance.cpp:503-584
+
+Matrices are in natural orientation: G[iz, a, b] = G_ab(z_iz).  A superlattice
+spec is the dict documented in pyqcm_amd/lattice.py.
+
+Pinned by (tests/test_oracle_lattice.py): at U = 0 the cluster Green function
+is (z - h_c)^{-1}, so G_cpt(k) = (z - h(k))^{-1} with h(k) = h_c + V(k), whose
+diagonal average is a closed-form Bloch sum, and log|det(V G - 1)| =
+log|det(z - h(k))| - log|det(z - h_c)|; the whole grid functional then equals
+the lattice's exact grand potential (2 / (nk L)) sum_k,n min(eps_n(k), 0).
+"""
+import numpy as np
+
+
+def superlattice_V(sl, k):
+    """V(k) (src_qcm/CPT.cpp:22-60) at reduced wavevector k (3-vector)."""
+    V = np.array(sl["V0"], dtype=np.complex128)
+    R = np.asarray(sl.get("R", np.zeros((0, 3))), dtype=np.float64).reshape(-1, 3)
+    k = np.asarray(list(k) + [0.0] * (3 - len(k)), dtype=np.float64)
+    for r, c, n, v in sl.get("elements", []):
+        V[r, c] += complex(v) * np.exp(2j * np.pi * np.dot(k, R[n]))
+    return V
+
+
+def kgrid_points(kgrid):
+    """Reduced wavevectors of (dim, (nkx, nky, nkz), offset), x fastest."""
+    dim, nk, off = kgrid
+    nk = list(nk) + [1] * (3 - len(nk))
+    axes = [(np.arange(nk[a]) + off) / nk[a] if a < dim else np.zeros(1) for a in range(3)]
+    kz, ky, kx = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.stack([kx.ravel(), ky.ravel(), kz.ravel()], axis=1)
+
+
+def cpt_projected_gf(G, sl, kgrid):
+    """(1/nk) sum_k G (1 - V(k) G)^{-1} per frequency (set_Gcpt + projected_Green_function)."""
+    G = np.asarray(G, dtype=np.complex128)
+    d = G.shape[1]
+    out = np.zeros_like(G)
+    ks = kgrid_points(kgrid)
+    one = np.eye(d)
+    for k in ks:
+        V = superlattice_V(sl, k)
+        out += G @ np.linalg.inv(one - V @ G)
+    return out / len(ks)
+
+
+def sef_logdet(G, sl, kgrid):
+    """(1/nk) sum_k log|det(V(k) G - 1)| per frequency (SEF_integrand_k, one spin block)."""
+    G = np.asarray(G, dtype=np.complex128)
+    d = G.shape[1]
+    acc = np.zeros(G.shape[0])
+    ks = kgrid_points(kgrid)
+    one = np.eye(d)
+    for k in ks:
+        V = superlattice_V(sl, k)
+        acc += np.linalg.slogdet(V @ G - one)[1]
+    return acc / len(ks)
+
+
+def potthoff_functional(I, omega_cluster, V0_up, V0_down, n_sites, spin_mixing):
+    """Grid-path assembly (src_qcm/lattice_model_instance.cpp:561-584), no Nambu
+    doubling: (-I - Omega_diag + Omega_cluster) / n_sites."""
+    diag = -np.sum(np.real(np.diag(V0_up)))
+    if spin_mixing:
+        diag -= np.sum(np.real(np.diag(V0_down)))
+    else:
+        diag *= 2.0
+    return (-I - 0.5 * diag + omega_cluster) / n_sites

@@ -33,6 +33,21 @@ _u32p = C.POINTER(C.c_uint32)
 _i32p = C.POINTER(C.c_int32)
 
 
+class Superlattice(C.Structure):
+    """ed_superlattice: one spin block of V(k) = V0 + sum_e val_e exp(2 pi i k.R[nbr_e])."""
+    _fields_ = [("d", C.c_int), ("V0", _dp), ("ne", C.c_int), ("row", _ip), ("col", _ip), ("nbr", _ip),
+                ("val", _dp), ("nnbr", C.c_int), ("R", _dp)]
+
+
+class KGridC(C.Structure):
+    """ed_kgrid: dim axes of nk[a] points at (i + offset) / nk[a]."""
+    _fields_ = [("dim", C.c_int), ("nk", C.c_int * 3), ("offset", C.c_double)]
+
+
+_slp = C.POINTER(Superlattice)
+_kgp = C.POINTER(KGridC)
+
+
 SIGNATURES = {
     "ed_last_error": (C.c_char_p, []),
     "ed_version": (C.c_int, [C.c_char_p, C.c_int]),
@@ -60,6 +75,9 @@ SIGNATURES = {
     "ed_mixing": (C.c_int, [C.c_int, _ip]),
     "ed_solve_batch": (C.c_int, [_ip, C.c_int, C.c_int]),
     "ed_green_function_batch": (C.c_int, [_ip, C.c_int, C.c_int, _dp, C.c_int, _dp]),
+    "ed_cpt_projected_gf": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _slp, _kgp, _dp]),
+    "ed_sef_grid": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _slp, _slp, _kgp, _dp, _dp]),
+    "ed_potthoff_functional": (C.c_int, [C.c_int, _dp]),
     "ed_hamiltonian_create": (C.c_int, [C.c_int, C.c_char_p, C.c_char, _i64p]),
     "ed_hamiltonian_destroy": (C.c_int, [C.c_int64]),
     "ed_hamiltonian_dim": (C.c_int, [C.c_int64, _i64p]),

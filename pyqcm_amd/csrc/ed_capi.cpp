@@ -11,6 +11,7 @@
 #include "ed_common.hpp"
 #include "ed_batch.hpp"
 #include "ed_instance.hpp"
+#include "ed_lattice.hpp"
 #include "ed_model.hpp"
 #include "ed_solvers.hpp"
 
@@ -249,6 +250,97 @@ int ed_solve_batch(const int *labels, int n, int flags) {
     std::vector<Instance *> insts;
     for (int k = 0; k < n; ++k) insts.push_back(&get_instance(labels[k]));
     solve_batch(insts, (flags & 1) != 0);
+  });
+}
+
+// ---- lattice sums ----
+static Superlattice superlattice_from(const ed_superlattice *V) {
+  if (!V || V->d <= 0 || !V->V0 || V->ne < 0 || V->nnbr < 0 || (V->ne > 0 && (!V->row || !V->col || !V->nbr || !V->val)) ||
+      (V->nnbr > 0 && !V->R))
+    fail(ED_ERR_INVALID, "superlattice: bad arguments");
+  Superlattice S;
+  S.d = V->d;
+  S.V0.assign(V->V0, V->V0 + 2 * (size_t)V->d * V->d);
+  for (int e = 0; e < V->ne; ++e) {
+    if (V->row[e] < 0 || V->row[e] >= V->d || V->col[e] < 0 || V->col[e] >= V->d || V->nbr[e] < 0 ||
+        V->nbr[e] >= V->nnbr)
+      fail(ED_ERR_INVALID, "superlattice: element " + std::to_string(e) + " out of range");
+    S.row.push_back(V->row[e]);
+    S.col.push_back(V->col[e]);
+    S.nbr.push_back(V->nbr[e]);
+    S.val.push_back(V->val[2 * e]);
+    S.val.push_back(V->val[2 * e + 1]);
+  }
+  S.R.assign(V->R, V->R + 3 * (size_t)V->nnbr);
+  return S;
+}
+
+static KGrid kgrid_from(const ed_kgrid *K) {
+  if (!K) fail(ED_ERR_INVALID, "wavevector grid: null");
+  KGrid G;
+  G.dim = K->dim;
+  for (int a = 0; a < 3; ++a) G.nk[a] = K->nk[a];
+  G.offset = K->offset;
+  return G;
+}
+
+int ed_cpt_projected_gf(int label, int nz, const double *z, int spin_down, const ed_superlattice *V,
+                        const ed_kgrid *K, double *out) {
+  return guard([&] {
+    if (nz < 0 || (nz > 0 && (!z || !out))) fail(ED_ERR_INVALID, "ed_cpt_projected_gf: bad arguments");
+    Instance &I = get_instance(label);
+    const Superlattice S = superlattice_from(V);
+    const KGrid G = kgrid_from(K);
+    const int d = I.gf_dim();
+    if (S.d != d) fail(ED_ERR_INVALID, "superlattice dimension differs from the Green-function dimension");
+    std::vector<double> g((size_t)nz * d * d * 2);
+    I.green_function(nz, z, spin_down != 0, g.data());
+    DevBuf<double> dg;
+    dg.upload(g);
+    SuperlatticeDev SD(S);
+    lattice_cpt_average(SD, G, nz, dg.p, out);
+  });
+}
+
+int ed_sef_grid(int label, int nw, const double *w, const double *weights, const ed_superlattice *Vup,
+                const ed_superlattice *Vdn, const ed_kgrid *K, double *integral, double *per_freq) {
+  return guard([&] {
+    if (nw < 0 || (nw > 0 && (!w || !weights)) || !integral) fail(ED_ERR_INVALID, "ed_sef_grid: bad arguments");
+    Instance &I = get_instance(label);
+    if (!I.gf_solved) I.green_function_solve();
+    const bool two = (I.mixing & 4) != 0;
+    if (two && !Vdn) fail(ED_ERR_INVALID, "ed_sef_grid: the spin-down superlattice is required with up/down mixing");
+    const KGrid G = kgrid_from(K);
+    const int d = I.gf_dim();
+    std::vector<double> z(2 * (size_t)nw), acc(nw, 0.0), part(nw);
+    for (int n = 0; n < nw; ++n) {
+      z[2 * n] = 0.0;
+      z[2 * n + 1] = w[n];
+    }
+    for (int spin = 0; spin < (two ? 2 : 1); ++spin) {
+      const Superlattice S = superlattice_from(spin ? Vdn : Vup);
+      if (S.d != d) fail(ED_ERR_INVALID, "superlattice dimension differs from the Green-function dimension");
+      std::vector<double> g((size_t)nw * d * d * 2);
+      I.green_function(nw, z.data(), spin == 1, g.data());
+      DevBuf<double> dg;
+      dg.upload(g);
+      SuperlatticeDev SD(S);
+      lattice_logdet(SD, G, nw, dg.p, part.data());
+      for (int n = 0; n < nw; ++n) acc[n] += two ? part[n] : 2.0 * part[n];
+    }
+    double s = 0.0;
+    for (int n = 0; n < nw; ++n) s += weights[n] * acc[n];
+    *integral = s;
+    if (per_freq) std::copy(acc.begin(), acc.end(), per_freq);
+  });
+}
+
+int ed_potthoff_functional(int label, double *omega) {
+  return guard([&] {
+    if (!omega) fail(ED_ERR_INVALID, "ed_potthoff_functional: null output");
+    Instance &I = get_instance(label);
+    I.ground_state_solve();
+    *omega = I.E0 + I.sef_bath();
   });
 }
 

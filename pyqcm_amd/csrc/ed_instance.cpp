@@ -759,10 +759,31 @@ std::vector<double> Instance::one_body_matrix(bool spin_down) const {
     if (op.kind != OP_ONE_BODY) continue;
     for (const auto &x : op.el) {
       if ((x.r >= L) != spin_down || (x.c >= L) != spin_down) continue;
+      // the operator holds both triangles (Hermitian completion at
+      // new_operator, src_ed/Operators/one_body_operator.hpp:67-76)
       h[(size_t)(x.r - off) * L + (x.c - off)] += kv.second * x.v;
     }
   }
   return h;
+}
+
+// bath part of ED::Potthoff_functional (model_instance::set_hopping_matrix,
+// src_ed/model_instance.hpp:447-473): minus the negative eigenvalues of the
+// bath one-body block, both spins (twice the up block without spin mixing)
+double Instance::sef_bath() const {
+  const int L = model->L, ns = model->n_sites, nb = model->n_bath;
+  if (!nb) return 0.0;
+  double s = 0.0;
+  for (int spin = 0; spin < ((mixing & 4) ? 2 : 1); ++spin) {
+    const std::vector<double> h = one_body_matrix(spin == 1);
+    std::vector<double> tb((size_t)nb * nb), eb(nb);
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+    symeig(nb, tb.data(), eb.data(), nullptr);
+    for (int i = 0; i < nb; ++i)
+      if (eb[i] < 0.0) s -= eb[i];
+  }
+  return (mixing & 4) ? s : 2.0 * s;
 }
 
 // hybridization_function (src_ed/model_instance.hpp:1004-1018): bath block

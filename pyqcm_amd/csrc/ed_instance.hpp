@@ -80,7 +80,8 @@ struct Instance {
   // spin block of the Green function for spin_down (checks the mixing)
   int gf_spin(bool spin_down) const;
   // one-body matrix of the spin block (L x L, row-major) from the instance's terms
-  std::vector<double> one_body_matrix(bool spin_down) const;
+  std::vector<double> one_body_matrix(bool spin_down) const;  // L x L, row-major, full matrix
+  double sef_bath() const;
 };
 
 Instance &get_instance(int label);

@@ -18,6 +18,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _lib
 from ._lib import EDError, check, lib
 
 __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_model_instance",
@@ -199,6 +200,73 @@ def self_energyC(z, spin_down=0, label=0):
 
 def hybridization_functionC(z, spin_down=0, label=0):
     return _gf_call(lib().ed_hybridization_function, label, [z], spin_down)[0]
+
+
+def _superlattice_struct(sl):
+    """ctypes ed_superlattice from a dict with V0 ((d, d) complex, V0[a, b] the
+    (a, b) element), elements (row, col, neighbor, complex value) and R
+    ((nnbr, 3) neighbor vectors); the arrays are kept on the struct."""
+    V0 = np.asarray(sl["V0"], dtype=np.complex128)
+    d = V0.shape[0]
+    els = list(sl.get("elements", []))
+    R = np.ascontiguousarray(np.asarray(sl.get("R", np.zeros((0, 3))), dtype=np.float64).reshape(-1, 3))
+    v0 = np.ascontiguousarray(V0.T).reshape(-1)  # column-major
+    row = np.array([e[0] for e in els] or [0], dtype=np.int32)
+    col = np.array([e[1] for e in els] or [0], dtype=np.int32)
+    nbr = np.array([e[2] for e in els] or [0], dtype=np.int32)
+    val = np.array([complex(e[3]) for e in els] or [0j], dtype=np.complex128)
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    st = _lib.Superlattice(d, v0.ctypes.data_as(dp), len(els), row.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                           nbr.ctypes.data_as(ip), val.ctypes.data_as(dp), R.shape[0], R.ctypes.data_as(dp))
+    st._keep = (v0, row, col, nbr, val, R)
+    return st
+
+
+def _kgrid_struct(kgrid):
+    """ctypes ed_kgrid from (dim, (nkx, nky, nkz), offset)."""
+    dim, nk, off = kgrid
+    nk = list(nk) + [1] * (3 - len(nk))
+    return _lib.KGridC(int(dim), (C.c_int * 3)(*[int(x) for x in nk[:3]]), float(off))
+
+
+def projected_Green_function(z, superlattice, kgrid, spin_down=0, label=0):
+    """lattice_model_instance::projected_Green_function (src_qcm/CPT.cpp:283-343)
+    for a lattice tiled by one cluster: (1/nk) sum_k G (1 - V(k) G)^{-1} at every
+    frequency of z; array (nz, d, d) laid out as Green_function_grid."""
+    zz, zp = _grid(z)
+    d = Green_function_dimensionC(label)
+    out = np.zeros(len(zz) * d * d, dtype=np.complex128)
+    st, kg = _superlattice_struct(superlattice), _kgrid_struct(kgrid)
+    check(lib().ed_cpt_projected_gf(int(label), len(zz), zp, int(spin_down), C.byref(st), C.byref(kg),
+                                    out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(len(zz), d, d)
+
+
+def SEF_grid(w, weights, superlattice, kgrid, superlattice_down=None, label=0):
+    """The (frequency, k) grid integral of lattice_model_instance::Potthoff_functional
+    (src_qcm/lattice_model_instance.cpp:503-584): sum_n weights[n] (1/nk) sum_k
+    log|det(V(k) G(i w_n) - 1)|, spin blocks summed (SEF_integrand_k, :624-650).
+    Returns (integral, per-frequency k-averages)."""
+    w = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    p = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    per = np.zeros(len(w))
+    I = C.c_double()
+    up = _superlattice_struct(superlattice)
+    dn = _superlattice_struct(superlattice_down) if superlattice_down is not None else None
+    kg = _kgrid_struct(kgrid)
+    dp = C.POINTER(C.c_double)
+    check(lib().ed_sef_grid(int(label), len(w), w.ctypes.data_as(dp), p.ctypes.data_as(dp), C.byref(up),
+                            C.byref(dn) if dn is not None else None, C.byref(kg), C.byref(I),
+                            per.ctypes.data_as(dp)))
+    return I.value, per
+
+
+def Potthoff_functionalC(label=0):
+    """ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term."""
+    x = C.c_double()
+    check(lib().ed_potthoff_functional(int(label), C.byref(x)))
+    return x.value
 
 
 def cluster_averages(label=0):
