@@ -230,6 +230,85 @@ def _ed_solve_cpu_once(wl, nz, beta):
     return E0, time.perf_counter() - t0
 
 
+def _neel_2x2():
+    """2x2 Hubbard cluster with a Neel Weiss field tiling the square lattice:
+    the instance (t=1 U=4 mu=0.4 M=0.15, ground state solved) and its two
+    spin blocks of V(k)."""
+    import numpy as np
+    from pyqcm_amd import lattice as LT, qcm, workloads as W
+    wl = W.vca_cluster(2, 2)
+    name = "bench_lat_" + wl.name
+    qcm.new_model(name, wl.n_sites, wl.n_bath)
+    for op, (typ, el) in wl.operators.items():
+        qcm.new_operator(name, op, typ, [(r + 1, c + 1, v) for (r, c, v) in el])
+    M = 0.15
+    qcm.new_model_instanceC(name, {"t": 1.0, "U": 4.0, "mu": 0.4, "M": M}, wl.sector, 9900)
+    qcm.ground_state_solve(9900)
+    s = np.array([1.0 if (x + y) % 2 == 0 else -1.0 for y in range(2) for x in range(2)])
+    return 9900, LT.square_superlattice(2, 2, 1.0, weiss=M * s), LT.square_superlattice(2, 2, 1.0, weiss=-M * s)
+
+
+def lattice_gpu(reps=5, nk_sef=128, nk_cpt=64):
+    """Potthoff functional on the (frequency, k) grid (Gauss-Legendre 3 x 32
+    frequencies, nk_sef^2 corner k-points, both spin blocks) and the CPT
+    average (256 real frequencies, eta 0.1, nk_cpt^2 midpoint k-points, spin
+    up); median wall time of reps calls after a warm-up."""
+    import numpy as np
+    from pyqcm_amd import lattice as LT, qcm
+    lab, up, dn = _neel_2x2()
+    w, p = LT.legendre_frequency_grid(1.0, 20.0, 32, 32, 32)
+    kg = (2, (nk_sef, nk_sef), 0.0)
+    z = np.linspace(-6.0, 6.0, 256) + 0.1j
+    kc = (2, (nk_cpt, nk_cpt), 0.5)
+    LT.potthoff_functional(lab, w, p, up, kg, dn)
+    ts, tc = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        omega = LT.potthoff_functional(lab, w, p, up, kg, dn)
+        ts.append(time.perf_counter() - t0)
+    qcm.projected_Green_function(z, up, kc, 0, lab)
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        qcm.projected_Green_function(z, up, kc, 0, lab)
+        tc.append(time.perf_counter() - t0)
+    return lab, up, dn, omega, float(np.median(ts)), float(np.median(tc)), len(w)
+
+
+def lattice_cpu(lab, up, dn, nk=12):
+    """The numpy oracle on a sample: the same Potthoff functional on an
+    nk^2 corner grid (both spins), timed, and the GPU's value on the same
+    grid for the parity check."""
+    import numpy as np
+    from oracle import lattice_ref as LR
+    from pyqcm_amd import lattice as LT, qcm
+    w, p = LT.legendre_frequency_grid(1.0, 20.0, 32, 32, 32)
+    kg = (2, (nk, nk), 0.0)
+    G = [np.transpose(qcm.Green_function_grid(1j * w, spin_down=s, label=lab), (0, 2, 1)) for s in (0, 1)]
+    t0 = time.perf_counter()
+    I = np.dot(p, LR.sef_logdet(G[0], up, kg) + LR.sef_logdet(G[1], dn, kg))
+    tcpu = time.perf_counter() - t0
+    om_cpu = LR.potthoff_functional(I, qcm.Potthoff_functionalC(lab), up["V0"], dn["V0"], 4, True)
+    om_gpu = LT.potthoff_functional(lab, w, p, up, kg, dn)
+    return om_cpu, om_gpu, tcpu, 2 * len(w) * nk * nk
+
+
+def _lattice_leg(args):
+    lab, up, dn, omega, ts, tc, nw = lattice_gpu()
+    pairs_sef, pairs_cpt = 2 * nw * 128 * 128, 256 * 64 * 64
+    r = {"workload": "2x2 Hubbard cluster t=1 U=4 mu=0.4 with Neel Weiss field M=0.15 tiling the square lattice: "
+                     "Potthoff functional on the grid (Gauss-Legendre 3x32 imaginary frequencies x 128^2 k, "
+                     "both spins) and CPT Green function averaged over 64^2 k at 256 real frequencies",
+         "omega": omega, "gpu_sef_s": round(ts, 5), "gpu_sef_pairs_per_s": round(pairs_sef / ts, 1),
+         "gpu_cpt_s": round(tc, 5), "gpu_cpt_pairs_per_s": round(pairs_cpt / tc, 1),
+         "timing": "median of 5 calls after a warm-up, whole call (G evaluation, k-sums, copies)"}
+    if not args.no_cpu:
+        om_cpu, om_gpu, tcpu, npairs = lattice_cpu(lab, up, dn)
+        r.update({"cpu_sef_pairs_per_s": round(npairs / tcpu, 1), "cpu_cores": 1, "cpu_kind": "port (numpy oracle)",
+                  "cpu_sample": "same functional on a 12^2 k grid", "dOmega_sample": abs(om_gpu - om_cpu),
+                  "speedup": round((pairs_sef / ts) / (npairs / tcpu), 1)})
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +319,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-hv", action="store_true", help="skip the H.v leg (profiling of the solve leg alone)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched VCA-sweep leg")
+    ap.add_argument("--no-lattice", action="store_true", help="skip the CPT / Potthoff-functional leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
@@ -259,6 +339,8 @@ def main():
             out["ed_solves"] = _solve_leg(args)
         if not args.no_batch:
             out["batch"] = _batch_leg(args)
+        if not args.no_lattice:
+            out["lattice"] = _lattice_leg(args)
         print(json.dumps(out))
         return
     H, wl, ms, wall = bench_hv(args.steps, args.warmup, args.variant)
@@ -302,6 +384,8 @@ def main():
         out["ed_solves"] = _solve_leg(args)
     if not args.no_batch:
         out["batch"] = _batch_leg(args)
+    if not args.no_lattice:
+        out["lattice"] = _lattice_leg(args)
     print(json.dumps(out))
 
 
