@@ -50,6 +50,7 @@ Params::Params() {
   i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
   i["reduction_blocks_per_cu"] = 4;  // grid of the Krylov vector kernels (multi-dot, axpy, combine)
   i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads
+  i["hv_row_order"] = 1;          // fused row kernel: 0 label order, 1 breadth-first order of the dn rows
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one

@@ -347,7 +347,8 @@ __global__ __launch_bounds__(BS) void k_hv_rows_sell(FusedParams P) {
   constexpr uint32_t NW = BS / kWave;
   const SellTable &T = a.up.sell[DOWN ? 0 : 1];
   double dot = 0.0;
-  for (uint32_t d = blockIdx.x; d < dd; d += gridDim.x) {
+  for (uint32_t di = blockIdx.x; di < dd; di += gridDim.x) {
+    const uint32_t d = a.row_order ? a.dn.order[di] : di;
     const size_t base = (size_t)d * du;
     const uint32_t g0 = a.dn.ptr[d], nnb = a.dn.ptr[d + 1] - g0;
     const uint32_t nst = min(nnb, (uint32_t)kMaxNb);

@@ -45,6 +45,8 @@ struct DevHalf {
   // entries col | sign << 15 in SellTable::e16
   int mode16 = 0;
   double t0 = 0.0;
+  // the states in breadth-first order of the operator's hopping graph
+  const uint32_t *order = nullptr;
 };
 
 // slab configuration of the two-pass down kernel (autotuned per plan)
@@ -61,6 +63,7 @@ struct FactArgs {
   const double *dg = nullptr;  // full diagonal (diag_mode 0)
   double Uc[32];          // cross (up-down same-site) coefficients (diag_mode 1, 2)
   int block = 0;          // row-kernel block size: 0 auto (256 below 2048 up states, else 1024), 256, 1024
+  int row_order = 0;      // fused row kernel: 0 rows in label order, 1 in dn.order (neighbour rows reused sooner)
 };
 
 enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };

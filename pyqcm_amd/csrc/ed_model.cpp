@@ -556,8 +556,35 @@ struct DevSell {
   }
 };
 
+// breadth-first order of the states over the operator's hopping graph (each
+// connected component from its lowest state): rows processed in this order
+// meet their neighbour rows again sooner than in label order
+static std::vector<uint32_t> bfs_order(const HalfHost &H) {
+  std::vector<uint32_t> order;
+  order.reserve(H.dim);
+  std::vector<char> seen(H.dim, 0);
+  const uint32_t mask = H.packed ? ((1u << H.colbits) - 1u) : 0xffffffffu;
+  for (uint32_t s0 = 0; s0 < H.dim; ++s0) {
+    if (seen[s0]) continue;
+    seen[s0] = 1;
+    size_t head = order.size();
+    order.push_back(s0);
+    while (head < order.size()) {
+      const uint32_t u = order[head++];
+      for (uint32_t e = H.ptr[u]; e < H.ptr[u + 1]; ++e) {
+        const uint32_t c = H.ent[e] & mask;
+        if (c < H.dim && !seen[c]) {
+          seen[c] = 1;
+          order.push_back(c);
+        }
+      }
+    }
+  }
+  return order;
+}
+
 struct DevHalfStore {
-  DevBuf<uint32_t> ptr, ent, states;
+  DevBuf<uint32_t> ptr, ent, states, order;
   DevBuf<double> val, diag;
   DevSell sell_nat, sell_sorted;
   DevHalf view;
@@ -596,6 +623,8 @@ struct DevHalfStore {
     view.sell[1] = sell_sorted.view(w <= 1);
     view.mode16 = t0 != 0.0 ? 1 : 0;
     view.t0 = t0;
+    order.upload(H.dim ? bfs_order(H) : std::vector<uint32_t>{0u});
+    view.order = order.p;
   }
 };
 
@@ -687,6 +716,7 @@ struct HamFactorized : Hamiltonian {
     }
     variant = (int)params().geti("hv_variant");
     args.block = (int)params().geti("hv_block");
+    args.row_order = (int)params().geti("hv_row_order");
   }
   // candidate schedules, in the order reported by ed_hamiltonian_autotune
   static std::vector<std::pair<int, SlabConfig>> candidates() {

@@ -95,26 +95,30 @@ def test_factorized_variants_agree(oracle, variant):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("row_order", [0, 1])
 @pytest.mark.parametrize("block", [0, 1024])
 @pytest.mark.parametrize("sell16", [True, False])
 @pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("kind", ["chain12", "plaquette8b"])
-def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16, block):
+def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16, block, row_order):
     """Spin-up rows of >= 512 states take the sliced-ELL row kernel; a single
     hopping magnitude (chain) allows the 16-bit sign-only table, several
-    magnitudes (plaquette + bath) the packed 32-bit one."""
+    magnitudes (plaquette + bath) the packed 32-bit one; rows in label or
+    breadth-first order."""
     if kind == "chain12":
         wl, sec, Nup, Ndw = W.hubbard_chain(12, U=4.0, mu=2.0), "R0:N11:S-1", 5, 6   # du = 792
     else:
         wl, sec, Nup, Ndw = W.plaquette_bath(seed=2025), "R0:N12:S0", 6, 6        # du = 924
     qcm.set_global_parameterC("sell16", sell16)
     qcm.set_global_parameterC("hv_block", block)
+    qcm.set_global_parameterC("hv_row_order", row_order)
     try:
         lab = _instance(wl, sec)
         H = qcm.Hamiltonian(lab, sec, "G")
     finally:
         qcm.set_global_parameterC("sell16", True)
         qcm.set_global_parameterC("hv_block", 0)
+        qcm.set_global_parameterC("hv_row_order", 1)
     H.set_variant(variant)
     x = np.random.default_rng(5).normal(size=H.dim)
     got = H.apply(x)
