@@ -672,28 +672,29 @@ struct PtrPack { const double *p[kMaxVec]; };
 struct MPtrPack { double *p[kMaxVec]; };
 struct CoefPack { double c[kMaxVec]; };
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, int m, PtrPack b, double *part) {
-  __shared__ double red[kMaxVec][BS / kWave];
-  double acc[kMaxVec];
+// The vector count M is a template parameter: with a run-time count the
+// guarded loads of the unrolled loop were each followed by a wait, so a
+// thread's loads ran one after another at full memory latency.
+template <int BS, int M>
+__global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, PtrPack b, double *part) {
+  __shared__ double red[M > 0 ? M : 1][BS / kWave];
+  double acc[M > 0 ? M : 1];
 #pragma unroll
-  for (int k = 0; k < kMaxVec; ++k) acc[k] = 0.0;
+  for (int k = 0; k < M; ++k) acc[k] = 0.0;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
     const double av = a[i];
 #pragma unroll
-    for (int k = 0; k < kMaxVec; ++k)
-      if (k < m) acc[k] += av * b.p[k][i];
+    for (int k = 0; k < M; ++k) acc[k] += av * b.p[k][i];
   }
   // one shuffle reduction per vector, one barrier per block
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
-  for (int k = 0; k < kMaxVec; ++k)
-    if (k < m) {
-      const double s = wave_sum(acc[k]);
-      if (lane == 0) red[k][w] = s;
-    }
+  for (int k = 0; k < M; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[k][w] = s;
+  }
   __syncthreads();
-  if (threadIdx.x < m) {
+  if (threadIdx.x < M) {
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < BS / kWave; ++q) s += red[threadIdx.x][q];
@@ -701,9 +702,9 @@ __global__ __launch_bounds__(BS) void k_multi_dot(size_t n, const double *a, int
   }
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, double *x, int m, MPtrPack y, CoefPack c,
-                                                       double *part, double xs) {
+template <int BS, int M>
+__global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, double *x, MPtrPack y, CoefPack c, double *part,
+                                                       double xs) {
   __shared__ double red[BS / kWave];
   double nn = 0.0;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
@@ -713,12 +714,11 @@ __global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, double *x, int 
       x[i] = xv;
     }
 #pragma unroll
-    for (int k = 0; k < kMaxVec; ++k)
-      if (k < m) {
-        const double v = y.p[k][i] + c.c[k] * xv;
-        y.p[k][i] = v;
-        if (k == 0) nn += v * v;
-      }
+    for (int k = 0; k < M; ++k) {
+      const double v = y.p[k][i] + c.c[k] * xv;
+      y.p[k][i] = v;
+      if (k == 0) nn += v * v;
+    }
   }
   if (part) {
     const double s = block_sum<BS>(nn, red);
@@ -726,16 +726,15 @@ __global__ __launch_bounds__(BS) void k_multi_axpy_src(size_t n, double *x, int 
   }
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, int m, PtrPack b, CoefPack c,
+template <int BS, int M>
+__global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, PtrPack b, CoefPack c,
                                                 double *npart, const double *w, double *dpart) {
   __shared__ double red[BS / kWave];
   double nn = 0.0, dd = 0.0;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
     double v = (st == 0.0) ? 0.0 : st * t[i];
 #pragma unroll
-    for (int k = 0; k < kMaxVec; ++k)
-      if (k < m) v += c.c[k] * b.p[k][i];
+    for (int k = 0; k < M; ++k) v += c.c[k] * b.p[k][i];
     t[i] = v;
     nn += v * v;
     if (w) dd += v * w[i];
@@ -1170,13 +1169,24 @@ int hv_otf(const OtfArgs &A, const double *x, double *y, double ax, double beta,
   return g;
 }
 
+// case list 0 .. kMaxVec of the vector-count templates
+#define ED_VEC_COUNTS(X) \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+static_assert(kMaxVec == 16, "ED_VEC_COUNTS lists 0 .. kMaxVec");
+
 int multi_dot(size_t n, const double *a, int m, const double *const *b, double *part, hipStream_t s) {
   if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_dot: too many vectors");
   PtrPack pk{};
   for (int k = 0; k < m; ++k) pk.p[k] = b[k];
   constexpr int BS = 256;
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
-  hipLaunchKernelGGL((k_multi_dot<BS>), dim3(g), dim3(BS), 0, s, n, a, m, pk, part);
+  switch (m) {
+#define ED_MD(M) \
+  case M: hipLaunchKernelGGL((k_multi_dot<BS, M>), dim3(g), dim3(BS), 0, s, n, a, pk, part); break;
+    ED_VEC_COUNTS(ED_MD)
+#undef ED_MD
+    default: fail(ED_ERR_INTERNAL, "multi_dot: bad vector count");
+  }
   ED_HIP(hipGetLastError());
   return g;
 }
@@ -1189,7 +1199,13 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
   for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
   constexpr int BS = 256;
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
-  hipLaunchKernelGGL((k_multi_axpy_src<BS>), dim3(g), dim3(BS), 0, s, n, x, m, pk, cp, part, xs);
+  switch (m) {
+#define ED_MA(M) \
+  case M: hipLaunchKernelGGL((k_multi_axpy_src<BS, M>), dim3(g), dim3(BS), 0, s, n, x, pk, cp, part, xs); break;
+    ED_VEC_COUNTS(ED_MA)
+#undef ED_MA
+    default: fail(ED_ERR_INTERNAL, "multi_axpy: bad vector count");
+  }
   ED_HIP(hipGetLastError());
   return g;
 }
@@ -1202,7 +1218,13 @@ int combine(size_t n, double *t, double st, int m, const double *const *b, const
   for (int k = 0; k < m; ++k) { pk.p[k] = b[k]; cp.c[k] = c[k]; }
   constexpr int BS = 256;
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
-  hipLaunchKernelGGL((k_combine<BS>), dim3(g), dim3(BS), 0, s, n, t, st, m, pk, cp, npart, w, dpart);
+  switch (m) {
+#define ED_CB(M) \
+  case M: hipLaunchKernelGGL((k_combine<BS, M>), dim3(g), dim3(BS), 0, s, n, t, st, pk, cp, npart, w, dpart); break;
+    ED_VEC_COUNTS(ED_CB)
+#undef ED_CB
+    default: fail(ED_ERR_INTERNAL, "combine: bad vector count");
+  }
   ED_HIP(hipGetLastError());
   return g;
 }
