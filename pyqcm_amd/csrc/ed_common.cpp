@@ -48,7 +48,8 @@ Params::Params() {
   // device-side knobs (no reference counterpart)
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
   i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
-  i["reduction_blocks_per_cu"] = 2;  // grid of the Krylov vector kernels (multi-dot, axpy, combine)
+  i["reduction_blocks_per_cu"] = 1;
+  i["vector_block"] = 1024;        // threads per workgroup of the Krylov vector kernels: 256, 512 or 1024  // grid of the Krylov vector kernels (multi-dot, axpy, combine)
   i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads
   i["hv_row_order"] = 1;          // fused row kernel: 0 label order, 1 breadth-first order of the dn rows
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence

@@ -1174,11 +1174,14 @@ int hv_otf(const OtfArgs &A, const double *x, double *y, double ax, double beta,
   X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 static_assert(kMaxVec == 16, "ED_VEC_COUNTS lists 0 .. kMaxVec");
 
-int multi_dot(size_t n, const double *a, int m, const double *const *b, double *part, hipStream_t s) {
-  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_dot: too many vectors");
-  PtrPack pk{};
-  for (int k = 0; k < m; ++k) pk.p[k] = b[k];
-  constexpr int BS = 256;
+// threads per workgroup of the Krylov vector kernels (256, 512 or 1024)
+static int vector_block() {
+  const long v = params().geti("vector_block");
+  return v >= 1024 ? 1024 : (v >= 512 ? 512 : 256);
+}
+
+template <int BS>
+static int multi_dot_bs(size_t n, const double *a, int m, const PtrPack &pk, double *part, hipStream_t s) {
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
   switch (m) {
 #define ED_MD(M) \
@@ -1191,13 +1194,20 @@ int multi_dot(size_t n, const double *a, int m, const double *const *b, double *
   return g;
 }
 
-int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c, double *part, hipStream_t s,
-                   double xs) {
-  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_axpy: too many vectors");
-  MPtrPack pk{};
-  CoefPack cp{};
-  for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
-  constexpr int BS = 256;
+int multi_dot(size_t n, const double *a, int m, const double *const *b, double *part, hipStream_t s) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_dot: too many vectors");
+  PtrPack pk{};
+  for (int k = 0; k < m; ++k) pk.p[k] = b[k];
+  switch (vector_block()) {
+    case 1024: return multi_dot_bs<1024>(n, a, m, pk, part, s);
+    case 512: return multi_dot_bs<512>(n, a, m, pk, part, s);
+    default: return multi_dot_bs<256>(n, a, m, pk, part, s);
+  }
+}
+
+template <int BS>
+static int multi_axpy_bs(size_t n, double *x, int m, const MPtrPack &pk, const CoefPack &cp, double *part,
+                         hipStream_t s, double xs) {
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
   switch (m) {
 #define ED_MA(M) \
@@ -1210,13 +1220,22 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
   return g;
 }
 
-int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *npart,
-            const double *w, double *dpart, hipStream_t s) {
-  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "combine: too many vectors");
-  PtrPack pk{};
+int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c, double *part, hipStream_t s,
+                   double xs) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "multi_axpy: too many vectors");
+  MPtrPack pk{};
   CoefPack cp{};
-  for (int k = 0; k < m; ++k) { pk.p[k] = b[k]; cp.c[k] = c[k]; }
-  constexpr int BS = 256;
+  for (int k = 0; k < m; ++k) { pk.p[k] = y[k]; cp.c[k] = c[k]; }
+  switch (vector_block()) {
+    case 1024: return multi_axpy_bs<1024>(n, x, m, pk, cp, part, s, xs);
+    case 512: return multi_axpy_bs<512>(n, x, m, pk, cp, part, s, xs);
+    default: return multi_axpy_bs<256>(n, x, m, pk, cp, part, s, xs);
+  }
+}
+
+template <int BS>
+static int combine_bs(size_t n, double *t, double st, int m, const PtrPack &pk, const CoefPack &cp, double *npart,
+                      const double *w, double *dpart, hipStream_t s) {
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
   switch (m) {
 #define ED_CB(M) \
@@ -1227,6 +1246,19 @@ int combine(size_t n, double *t, double st, int m, const double *const *b, const
   }
   ED_HIP(hipGetLastError());
   return g;
+}
+
+int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *npart,
+            const double *w, double *dpart, hipStream_t s) {
+  if (m > kMaxVec) fail(ED_ERR_INTERNAL, "combine: too many vectors");
+  PtrPack pk{};
+  CoefPack cp{};
+  for (int k = 0; k < m; ++k) { pk.p[k] = b[k]; cp.c[k] = c[k]; }
+  switch (vector_block()) {
+    case 1024: return combine_bs<1024>(n, t, st, m, pk, cp, npart, w, dpart, s);
+    case 512: return combine_bs<512>(n, t, st, m, pk, cp, npart, w, dpart, s);
+    default: return combine_bs<256>(n, t, st, m, pk, cp, npart, w, dpart, s);
+  }
 }
 
 void scale(size_t n, double *x, double a, hipStream_t s) {

@@ -44,7 +44,7 @@ _PARAM_TYPES = {
     "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
     "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
-    "sell_sort_window": _INT, "hv_block": _INT, "hv_row_order": _INT, "reduction_blocks_per_cu": _INT,
+    "sell_sort_window": _INT, "hv_block": _INT, "hv_row_order": _INT, "vector_block": _INT, "reduction_blocks_per_cu": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 
