@@ -423,3 +423,32 @@ def test_mcf_green_function_matches_band(oracle):
     qcm.set_global_parameterC("accur_band_lanczos", 1e-12)
     qcm.set_global_parameterC("GF_method", "L")
     assert np.abs(G1 - G2).max() < 1e-7
+
+
+@pytest.mark.parametrize("vector_block,blocks_per_cu", [(256, 4), (512, 2), (1024, 1)])
+def test_krylov_vector_kernel_shapes(vector_block, blocks_per_cu):
+    """Ground state (dense E0) and band-Lanczos G(i w) of a plaquette + 4 bath
+    sector with the Krylov vector kernels (multi-dot, multi-axpy, combine)
+    at several workgroup sizes and grids: the same results as the default."""
+    wl = W.plaquette_bath(n_bath=4)
+    z = W.matsubara(8, 20.0)
+
+    def solve():
+        lab = _instance(wl, "R0:N8:S0")
+        E0, _ = qcm.ground_state_solve(lab)
+        G = qcm.Green_function_grid(z, label=lab)
+        qcm.erase_model_instance(lab)
+        return E0, G
+
+    E_ref, G_ref = solve()
+    qcm.set_global_parameterC("vector_block", vector_block)
+    qcm.set_global_parameterC("reduction_blocks_per_cu", blocks_per_cu)
+    try:
+        E0, G = solve()
+    finally:
+        qcm.set_global_parameterC("vector_block", 1024)
+        qcm.set_global_parameterC("reduction_blocks_per_cu", 1)
+    Ed = np.linalg.eigvalsh(dense_hamiltonian(wl.L, wl.terms(), 4, 4))[0]
+    assert abs(E0 - Ed) <= 1e-10 * abs(Ed)
+    assert abs(E0 - E_ref) <= 1e-12 * abs(Ed)
+    assert np.abs(G - G_ref).max() < 1e-10 * np.abs(G_ref).max()
