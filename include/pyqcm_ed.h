@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-/* status codes (the reference's qcm_ED_throw triggers, SURVEY of failure paths) */
+/* status codes (the failure paths where the reference calls qcm_ED_throw) */
 enum {
   ED_OK = 0,
   ED_ERR_INVALID = 1,        /* bad argument, unknown name, malformed elements */
@@ -36,7 +36,7 @@ enum {
   ED_ERR_OOM = 5,            /* device or host allocation failed */
   ED_ERR_HIP = 6,            /* HIP runtime error */
   ED_ERR_NOT_FOUND = 7,      /* unknown model / instance label */
-  ED_ERR_UNSUPPORTED = 8,    /* feature outside this build (point groups, complex operators) */
+  ED_ERR_UNSUPPORTED = 8,    /* feature this library does not implement (point groups, complex operators) */
   ED_ERR_INTERNAL = 9,
   ED_ERR_DEGENERATE = 10     /* band Lanczos degenerate gap with no_degenerate_BL (Lanczos.hpp:368-370) */
 };

@@ -146,3 +146,32 @@ def test_bath_model_self_energy_hybridization_and_potthoff():
     eb = np.linalg.eigvalsh(Tbb)
     assert abs(qcm.Potthoff_functionalC(lab) - (E0 - 2.0 * np.sum(eb[eb < 0]))) < 1e-10
     qcm.erase_model_instance(lab)
+
+
+def test_lattice_sums_reject_bad_input():
+    """Errors come back as EDError with the library's status codes: a
+    superlattice of the wrong dimension or with an element outside it, a
+    missing spin-down superlattice under spin mixing, and d > 8."""
+    from pyqcm_amd._lib import EDError
+    lab, up, dn = _neel_cluster()
+    kg = (2, (4, 4), 0.5)
+    bad_dim = LT.square_superlattice(2, 1, 1.0)
+    with pytest.raises(EDError) as e:
+        qcm.projected_Green_function([0.5j], bad_dim, kg, label=lab)
+    assert e.value.code == 1
+    bad_el = dict(up)
+    bad_el["elements"] = list(up["elements"]) + [(0, 7, 0, -1.0)]
+    with pytest.raises(EDError) as e:
+        qcm.projected_Green_function([0.5j], bad_el, kg, label=lab)
+    assert e.value.code == 1
+    w, p = LT.legendre_frequency_grid(1.0, 20.0, 4, 4, 4)
+    with pytest.raises(EDError) as e:
+        qcm.SEF_grid(w, p, up, kg, None, label=lab)
+    assert e.value.code == 1
+    qcm.erase_model_instance(lab)
+    wl = W.hubbard_chain(9, U=4.0, mu=2.0, N=8)
+    big = _instance("lat_chain9", wl.n_sites, wl.n_bath, wl.operators, dict(wl.values), wl.sector)
+    with pytest.raises(EDError) as e:
+        qcm.projected_Green_function([0.5j], LT.chain_superlattice(9), (1, (4,), 0.5), label=big)
+    assert e.value.code == 8
+    qcm.erase_model_instance(big)
