@@ -1,11 +1,13 @@
 // ed_lattice.hip -- (frequency, wavevector) grid sums for CPT and the
-// self-energy functional.  One wave per (frequency, chunk of k points); lane
-// e < d^2 holds matrix element (e mod d, e / d) of the column-major d x d
-// matrices, which live in LDS.  Per k point the wave builds V(k) from the
-// Fourier elements, forms V G, and factors it with partial pivoting: LU for
-// log|det(V G - 1)|, Gauss-Jordan for (1 - V G)^{-1}.  Each wave accumulates
-// its chunk in registers; the chunks are summed in a fixed order by a second
-// kernel, so results do not depend on scheduling.
+// self-energy functional.  One wave per (frequency, chunk of k points), which
+// works on 64 / G k points at a time: each group of G lanes (G the smallest
+// power of two >= d^2, at least 16) holds one column-major d x d matrix in
+// LDS, lane e < d^2 of the group element (e mod d, e / d).  Per k point the
+// group builds V(k) from the Fourier elements, forms V G, and factors it with
+// partial pivoting: LU for log|det(V G - 1)|, Gauss-Jordan for
+// (1 - V G)^{-1}.  Each wave accumulates its chunk in registers; the chunks
+// are summed in a fixed order by a second kernel, so results do not depend
+// on scheduling.
 #include "ed_lattice.hpp"
 
 #include <cmath>
@@ -53,10 +55,16 @@ __device__ __forceinline__ void cmul(double ar, double ai, double br, double bi,
   i = ar * bi + ai * br;
 }
 
-// index of the largest value over the wave (ties: the smaller index)
-__device__ __forceinline__ int wave_argmax(double v, int who) {
+// lanes per matrix: the smallest power of two >= d^2 (at least 16); matrices per wave
+__host__ __device__ constexpr int lattice_group(int d) {
+  return d * d <= 16 ? 16 : (d * d <= 32 ? 32 : 64);
+}
+
+// index of the largest value over aligned groups of G lanes (ties: the smaller index)
+template <int G>
+__device__ __forceinline__ int group_argmax(double v, int who) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int o = G / 2; o > 0; o >>= 1) {
     const double v2 = __shfl_xor(v, o, 64);
     const int w2 = __shfl_xor(who, o, 64);
     if (v2 > v || (v2 == v && w2 < who)) {
@@ -67,96 +75,108 @@ __device__ __forceinline__ int wave_argmax(double v, int who) {
   return who;
 }
 
+// One wave per (frequency, chunk of k points).  The wave holds 64 / G
+// matrices at once (G lanes each, lane e < d^2 of a group holding element
+// (e mod d, e / d)), i.e. that many k points of the chunk per pass.
 // MODE 0: part[iz][chunk] = sum over the chunk of log|det(V G - 1)|
 // MODE 1: part[iz][chunk][e] = sum over the chunk of (G (1 - V G)^{-1})_e
-template <int MODE>
-__global__ __launch_bounds__(64) void k_lattice(Arg A, const double *G, double *part) {
-  __shared__ double sG[2 * kLatticeMaxDim * kLatticeMaxDim], sV[2 * kLatticeMaxDim * kLatticeMaxDim];
-  __shared__ double sM[2 * kLatticeMaxDim * kLatticeMaxDim], sB[2 * kLatticeMaxDim * kLatticeMaxDim];
-  __shared__ double sph[2 * kLatticeMaxNbr];
+template <int MODE, int G>
+__global__ __launch_bounds__(64) void k_lattice(Arg A, const double *Gz, double *part) {
+  constexpr int NQ = 64 / G;  // matrices per wave
+  __shared__ double sG[2 * kLatticeMaxDim * kLatticeMaxDim];
+  __shared__ double sV[2 * 64], sM[2 * 64], sB[2 * 64];
+  __shared__ double sph[2 * NQ * kLatticeMaxNbr];
   const int iz = blockIdx.y, chunk = blockIdx.x, lane = threadIdx.x, d = A.d, dd = d * d;
-  const bool act = lane < dd;
-  const int i = act ? lane % d : 0, j = act ? lane / d : 0;
+  const int q = lane / G, e = lane - q * G;
+  const bool act = e < dd;
+  const int i = act ? e % d : 0, j = act ? e / d : 0;
+  const int base = q * G;  // this group's offset in sV / sM / sB
   const long nchunk = gridDim.x;
-  if (act) {
-    sG[2 * lane] = G[2 * ((size_t)iz * dd + lane)];
-    sG[2 * lane + 1] = G[2 * ((size_t)iz * dd + lane) + 1];
+  for (int t = lane; t < dd; t += 64) {
+    sG[2 * t] = Gz[2 * ((size_t)iz * dd + t)];
+    sG[2 * t + 1] = Gz[2 * ((size_t)iz * dd + t) + 1];
   }
   double acc = 0.0, accr = 0.0, acci = 0.0;
   const long k0 = (long)chunk * A.kchunk, k1 = k0 + A.kchunk < A.nk ? k0 + A.kchunk : A.nk;
-  for (long kk = k0; kk < k1; ++kk) {
-    // reduced wavevector and the neighbor phases exp(2 pi i k . R)
-    const long a0 = kk % A.nk0, a1 = (kk / A.nk0) % A.nk1, a2 = kk / ((long)A.nk0 * A.nk1);
-    const double kx = A.dim >= 1 ? ((double)a0 + A.off) / A.nk0 : 0.0;
-    const double ky = A.dim >= 2 ? ((double)a1 + A.off) / A.nk1 : 0.0;
-    const double kz = A.dim >= 3 ? ((double)a2 + A.off) / A.nk2 : 0.0;
-    for (int n = lane; n < A.nnbr; n += 64) {
+  for (long kb = k0; kb < k1; kb += NQ) {
+    // the wavevectors of the NQ groups and their phases exp(2 pi i k . R)
+    for (int t = lane; t < NQ * A.nnbr; t += 64) {
+      const int g = t / A.nnbr, n = t - g * A.nnbr;
+      const long kk = kb + g < k1 ? kb + g : k1 - 1;
+      const long a0 = kk % A.nk0, a1 = (kk / A.nk0) % A.nk1, a2 = kk / ((long)A.nk0 * A.nk1);
+      const double kx = A.dim >= 1 ? ((double)a0 + A.off) / A.nk0 : 0.0;
+      const double ky = A.dim >= 2 ? ((double)a1 + A.off) / A.nk1 : 0.0;
+      const double kz = A.dim >= 3 ? ((double)a2 + A.off) / A.nk2 : 0.0;
       const double ang = 2.0 * M_PI * (kx * A.R[3 * n] + ky * A.R[3 * n + 1] + kz * A.R[3 * n + 2]);
-      double s, c;
-      sincos(ang, &s, &c);
-      sph[2 * n] = c;
-      sph[2 * n + 1] = s;
+      double sn, cs;
+      sincos(ang, &sn, &cs);
+      sph[2 * t] = cs;
+      sph[2 * t + 1] = sn;
     }
     __syncthreads();
+    const bool valid = kb + q < k1;
     if (act) {
-      double vr = A.V0[2 * lane], vi = A.V0[2 * lane + 1];
-      for (int q = A.ptr[lane]; q < A.ptr[lane + 1]; ++q) {
+      double vr = A.V0[2 * e], vi = A.V0[2 * e + 1];
+      for (int r = A.ptr[e]; r < A.ptr[e + 1]; ++r) {
+        const int ph = q * A.nnbr + A.nbr[r];
         double tr, ti;
-        cmul(A.val[2 * q], A.val[2 * q + 1], sph[2 * A.nbr[q]], sph[2 * A.nbr[q] + 1], tr, ti);
+        cmul(A.val[2 * r], A.val[2 * r + 1], sph[2 * ph], sph[2 * ph + 1], tr, ti);
         vr += tr;
         vi += ti;
       }
-      sV[2 * lane] = vr;
-      sV[2 * lane + 1] = vi;
+      sV[2 * (base + e)] = vr;
+      sV[2 * (base + e) + 1] = vi;
     }
     __syncthreads();
     if (act) {
       double pr = 0.0, pi = 0.0;
       for (int l = 0; l < d; ++l) {
         double tr, ti;
-        cmul(sV[2 * (i + l * d)], sV[2 * (i + l * d) + 1], sG[2 * (l + j * d)], sG[2 * (l + j * d) + 1], tr, ti);
+        cmul(sV[2 * (base + i + l * d)], sV[2 * (base + i + l * d) + 1], sG[2 * (l + j * d)], sG[2 * (l + j * d) + 1],
+             tr, ti);
         pr += tr;
         pi += ti;
       }
       const double one = i == j ? 1.0 : 0.0;
-      sM[2 * lane] = MODE == 0 ? pr - one : one - pr;
-      sM[2 * lane + 1] = MODE == 0 ? pi : -pi;
+      sM[2 * (base + e)] = MODE == 0 ? pr - one : one - pr;
+      sM[2 * (base + e) + 1] = MODE == 0 ? pi : -pi;
       if (MODE == 1) {
-        sB[2 * lane] = one;
-        sB[2 * lane + 1] = 0.0;
+        sB[2 * (base + e)] = one;
+        sB[2 * (base + e) + 1] = 0.0;
       }
     }
     __syncthreads();
     double logdet = 0.0;
     for (int c = 0; c < d; ++c) {
-      // pivot row: the largest |M(r, c)|, r >= c
+      // pivot row of this group's matrix: the largest |M(r, c)|, r >= c
       double mag = -1.0;
-      if (act && j == c && i >= c) mag = sM[2 * lane] * sM[2 * lane] + sM[2 * lane + 1] * sM[2 * lane + 1];
-      const int p = wave_argmax(mag, act ? i : d);
-      if (p != c) {
-        double mr = 0.0, mi = 0.0, br = 0.0, bi = 0.0;
-        const bool sw = act && (i == c || i == p);
-        if (sw) {
-          const int o = (i == c ? p : c) + j * d;
-          mr = sM[2 * o];
-          mi = sM[2 * o + 1];
-          if (MODE == 1) {
-            br = sB[2 * o];
-            bi = sB[2 * o + 1];
-          }
+      if (act && j == c && i >= c)
+        mag = sM[2 * (base + e)] * sM[2 * (base + e)] + sM[2 * (base + e) + 1] * sM[2 * (base + e) + 1];
+      const int p = group_argmax<G>(mag, act ? i : d);
+      // swap rows c and p (nothing to do where p == c)
+      double mr = 0.0, mi = 0.0, br = 0.0, bi = 0.0;
+      const bool sw = act && p != c && (i == c || i == p);
+      if (sw) {
+        const int o = base + (i == c ? p : c) + j * d;
+        mr = sM[2 * o];
+        mi = sM[2 * o + 1];
+        if (MODE == 1) {
+          br = sB[2 * o];
+          bi = sB[2 * o + 1];
         }
-        __syncthreads();
-        if (sw) {
-          sM[2 * lane] = mr;
-          sM[2 * lane + 1] = mi;
-          if (MODE == 1) {
-            sB[2 * lane] = br;
-            sB[2 * lane + 1] = bi;
-          }
-        }
-        __syncthreads();
       }
-      const double qr = sM[2 * (c + c * d)], qi = sM[2 * (c + c * d) + 1], q2 = qr * qr + qi * qi;
+      __syncthreads();
+      if (sw) {
+        sM[2 * (base + e)] = mr;
+        sM[2 * (base + e) + 1] = mi;
+        if (MODE == 1) {
+          sB[2 * (base + e)] = br;
+          sB[2 * (base + e) + 1] = bi;
+        }
+      }
+      __syncthreads();
+      const int pc = base + c + c * d;
+      const double qr = sM[2 * pc], qi = sM[2 * pc + 1], q2 = qr * qr + qi * qi;
       logdet += 0.5 * log(q2);
       // 1 / pivot
       const double ir = qr / q2, ii = -qi / q2;
@@ -166,58 +186,59 @@ __global__ __launch_bounds__(64) void k_lattice(Arg A, const double *G, double *
         const bool up = act && i > c && j > c;
         if (up) {
           double fr, fi, tr, ti;
-          cmul(sM[2 * (i + c * d)], sM[2 * (i + c * d) + 1], ir, ii, fr, fi);
-          cmul(fr, fi, sM[2 * (c + j * d)], sM[2 * (c + j * d) + 1], tr, ti);
-          nr = sM[2 * lane] - tr;
-          ni = sM[2 * lane + 1] - ti;
+          cmul(sM[2 * (base + i + c * d)], sM[2 * (base + i + c * d) + 1], ir, ii, fr, fi);
+          cmul(fr, fi, sM[2 * (base + c + j * d)], sM[2 * (base + c + j * d) + 1], tr, ti);
+          nr = sM[2 * (base + e)] - tr;
+          ni = sM[2 * (base + e) + 1] - ti;
         }
         __syncthreads();
         if (up) {
-          sM[2 * lane] = nr;
-          sM[2 * lane + 1] = ni;
+          sM[2 * (base + e)] = nr;
+          sM[2 * (base + e) + 1] = ni;
         }
         __syncthreads();
       } else {
         // Gauss-Jordan on [M | B]: row c scaled by 1 / pivot, eliminated from every other row
-        double mr = 0.0, mi = 0.0, br = 0.0, bi = 0.0;
+        double nmr = 0.0, nmi = 0.0, nbr = 0.0, nbi = 0.0;
         if (act) {
           double cr, ci, dr, di;
-          cmul(sM[2 * (c + j * d)], sM[2 * (c + j * d) + 1], ir, ii, cr, ci);  // M(c, j) / pivot
-          cmul(sB[2 * (c + j * d)], sB[2 * (c + j * d) + 1], ir, ii, dr, di);  // B(c, j) / pivot
+          cmul(sM[2 * (base + c + j * d)], sM[2 * (base + c + j * d) + 1], ir, ii, cr, ci);
+          cmul(sB[2 * (base + c + j * d)], sB[2 * (base + c + j * d) + 1], ir, ii, dr, di);
           if (i == c) {
-            mr = cr;
-            mi = ci;
-            br = dr;
-            bi = di;
+            nmr = cr;
+            nmi = ci;
+            nbr = dr;
+            nbi = di;
           } else {
-            const double fr = sM[2 * (i + c * d)], fi = sM[2 * (i + c * d) + 1];
+            const double fr = sM[2 * (base + i + c * d)], fi = sM[2 * (base + i + c * d) + 1];
             double tr, ti;
             cmul(fr, fi, cr, ci, tr, ti);
-            mr = sM[2 * lane] - tr;
-            mi = sM[2 * lane + 1] - ti;
+            nmr = sM[2 * (base + e)] - tr;
+            nmi = sM[2 * (base + e) + 1] - ti;
             cmul(fr, fi, dr, di, tr, ti);
-            br = sB[2 * lane] - tr;
-            bi = sB[2 * lane + 1] - ti;
+            nbr = sB[2 * (base + e)] - tr;
+            nbi = sB[2 * (base + e) + 1] - ti;
           }
         }
         __syncthreads();
         if (act) {
-          sM[2 * lane] = mr;
-          sM[2 * lane + 1] = mi;
-          sB[2 * lane] = br;
-          sB[2 * lane + 1] = bi;
+          sM[2 * (base + e)] = nmr;
+          sM[2 * (base + e) + 1] = nmi;
+          sB[2 * (base + e)] = nbr;
+          sB[2 * (base + e) + 1] = nbi;
         }
         __syncthreads();
       }
     }
     if (MODE == 0) {
-      acc += logdet;
-    } else if (act) {
+      if (valid) acc += logdet;
+    } else if (act && valid) {
       // G (1 - V G)^{-1}
       double xr = 0.0, xi = 0.0;
       for (int l = 0; l < d; ++l) {
         double tr, ti;
-        cmul(sG[2 * (i + l * d)], sG[2 * (i + l * d) + 1], sB[2 * (l + j * d)], sB[2 * (l + j * d) + 1], tr, ti);
+        cmul(sG[2 * (i + l * d)], sG[2 * (i + l * d) + 1], sB[2 * (base + l + j * d)], sB[2 * (base + l + j * d) + 1],
+             tr, ti);
         xr += tr;
         xi += ti;
       }
@@ -226,12 +247,24 @@ __global__ __launch_bounds__(64) void k_lattice(Arg A, const double *G, double *
     }
     __syncthreads();
   }
+  // the groups' sums, in group order
   if (MODE == 0) {
-    if (lane == 0) part[(size_t)iz * nchunk + chunk] = acc;
-  } else if (act) {
-    const size_t o = ((size_t)iz * nchunk + chunk) * dd + lane;
-    part[2 * o] = accr;
-    part[2 * o + 1] = acci;
+    double s = 0.0;
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) s += __shfl(acc, g * G, 64);
+    if (lane == 0) part[(size_t)iz * nchunk + chunk] = s;
+  } else {
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int g = 0; g < NQ; ++g) {
+      sr += __shfl(accr, g * G + e, 64);
+      si += __shfl(acci, g * G + e, 64);
+    }
+    if (q == 0 && act) {
+      const size_t o = ((size_t)iz * nchunk + chunk) * dd + e;
+      part[2 * o] = sr;
+      part[2 * o + 1] = si;
+    }
   }
 }
 
@@ -276,7 +309,13 @@ void run(const SuperlatticeDev &S, const KGrid &K, int nz, const double *dG, dou
   A.nbr = S.nbr.p;
   const int width = MODE == 0 ? 1 : 2 * S.d * S.d;
   DevBuf<double> part((size_t)nz * nchunk * width), dout((size_t)nz * width);
-  hipLaunchKernelGGL(k_lattice<MODE>, dim3((unsigned)nchunk, (unsigned)nz), dim3(64), 0, stream(), A, dG, part.p);
+  const int G = lattice_group(S.d);
+  if (G == 16)
+    hipLaunchKernelGGL((k_lattice<MODE, 16>), dim3((unsigned)nchunk, (unsigned)nz), dim3(64), 0, stream(), A, dG, part.p);
+  else if (G == 32)
+    hipLaunchKernelGGL((k_lattice<MODE, 32>), dim3((unsigned)nchunk, (unsigned)nz), dim3(64), 0, stream(), A, dG, part.p);
+  else
+    hipLaunchKernelGGL((k_lattice<MODE, 64>), dim3((unsigned)nchunk, (unsigned)nz), dim3(64), 0, stream(), A, dG, part.p);
   ED_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_lattice_reduce, dim3((nz * width + 255) / 256), dim3(256), 0, stream(), nz, (int)nchunk, width,
                      1.0 / (double)A.nk, part.p, dout.p);
