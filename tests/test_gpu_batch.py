@@ -104,3 +104,35 @@ def test_batch_large_sector_four_wave_kernel():
     wl = W.hubbard_chain(10, U=4.0, mu=2.0)
     _compare(wl, [{"t": 1.0, "U": 4.0, "mu": 2.0}, {"t": 1.0, "U": 6.0, "mu": 3.1}], "F", W.matsubara(16, 20.0),
              dense=False)
+
+
+def test_batch_mixed_formats_and_empty():
+    """An empty batch is a no-op; instances built with the CSR format ('S')
+    are solved one by one inside the same call, next to batched 'G' ones, and
+    all agree with their own solves."""
+    qcm.solve_batch([])
+    qcm.set_global_parameterC("GF_method", "F")
+    wl = W.vca_cluster(2, 2)
+    name = _model(wl)
+    pts = W.vca_points(4, seed=9)
+    z = W.matsubara(16, 20.0)
+    labs, refs = [], []
+    for k, vals in enumerate(pts):
+        qcm.set_global_parameterC("Hamiltonian_format", "S" if k % 2 else "G")
+        la, lb = _new_label(), _new_label()
+        qcm.new_model_instanceC(name, vals, wl.sector, la)
+        qcm.new_model_instanceC(name, vals, wl.sector, lb)
+        labs.append(la)
+        refs.append(lb)
+    qcm.set_global_parameterC("Hamiltonian_format", "G")
+    qcm.solve_batch(labs)
+    for la, lb in zip(labs, refs):
+        Ea, _ = qcm.ground_state_solve(la)
+        Eb, _ = qcm.ground_state_solve(lb)
+        assert abs(Ea - Eb) <= 1e-10 * abs(Eb)
+        for spin in (0, 1):
+            Ga = qcm.Green_function_grid(z, spin_down=spin, label=la)
+            Gb = qcm.Green_function_grid(z, spin_down=spin, label=lb)
+            assert np.abs(Ga - Gb).max() < 1e-8
+        qcm.erase_model_instance(la)
+        qcm.erase_model_instance(lb)
