@@ -200,6 +200,27 @@ int ed_sef_grid(int label, int nw, const double *w, const double *weights, const
 /* ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term */
 int ed_potthoff_functional(int label, double *omega);
 
+/* ---- frequency integrals and responses of one instance ---- */
+/* ED::Green_function_average (src_ed/qcm_ED.cpp:240-248; model_instance.hpp:641-652):
+   the frequency-integrated Green function of the spin block, <c^dagger_b c_a>; out
+   receives d x d complex, column-major, as one block of ed_green_function. */
+int ed_green_function_average(int label, int spin_down, double *out);
+/* ED::Green_function_density (src_ed/qcm_ED.cpp:251-257; model_instance.hpp:660-675):
+   electrons per site from the trace of the average. */
+int ed_green_function_density(int label, double *density);
+/* ED::tr_sigma_inf (src_ed/qcm_ED.cpp:345-348; model_instance.hpp:1278-1312): the
+   trace of the self-energy extrapolated to infinite frequency, both spins. */
+int ed_tr_sigma_inf(int label, double *value);
+/* ED::susceptibility (src_ed/qcm_ED.cpp:307-315; model_instance.hpp:1022-1058): the
+   dynamic response sum_n |<n|O|Omega>|^2 / (w - E_n + E_Omega), weighted over the
+   low-energy states, of instance operator `op` at nw complex frequencies (re, im
+   pairs); chi receives nw complex values. */
+int ed_susceptibility(int label, const char *op, int nw, const double *w, double *chi);
+/* ED::susceptibility_poles (src_ed/qcm_ED.cpp:319-326; model_instance.hpp:1066-1108):
+   the same response as *n (pole, residue) pairs, written to poles[2k], poles[2k+1]
+   for k < cap: poles above 1e-8, residues above 1e-6, poles closer than 1e-4 merged. */
+int ed_susceptibility_poles(int label, const char *op, int cap, double *poles, int *n);
+
 /* ---- device memory helpers for callers that keep vectors resident ---- */
 int ed_device_alloc(int64_t bytes, void **ptr);
 int ed_device_free(void *ptr, int64_t bytes);

@@ -78,6 +78,11 @@ SIGNATURES = {
     "ed_cpt_projected_gf": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _slp, _kgp, _dp]),
     "ed_sef_grid": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _slp, _slp, _kgp, _dp, _dp]),
     "ed_potthoff_functional": (C.c_int, [C.c_int, _dp]),
+    "ed_green_function_average": (C.c_int, [C.c_int, C.c_int, _dp]),
+    "ed_green_function_density": (C.c_int, [C.c_int, _dp]),
+    "ed_tr_sigma_inf": (C.c_int, [C.c_int, _dp]),
+    "ed_susceptibility": (C.c_int, [C.c_int, C.c_char_p, C.c_int, _dp, _dp]),
+    "ed_susceptibility_poles": (C.c_int, [C.c_int, C.c_char_p, C.c_int, _dp, _ip]),
     "ed_hamiltonian_create": (C.c_int, [C.c_int, C.c_char_p, C.c_char, _i64p]),
     "ed_hamiltonian_destroy": (C.c_int, [C.c_int64]),
     "ed_hamiltonian_dim": (C.c_int, [C.c_int64, _i64p]),

@@ -344,6 +344,93 @@ int ed_potthoff_functional(int label, double *omega) {
   });
 }
 
+int ed_green_function_average(int label, int spin_down, double *out) {
+  return guard([&] {
+    need(out != nullptr, "ed_green_function_average: null output");
+    Instance &I = get_instance(label);
+    const std::vector<double> G = I.gf_average(spin_down != 0);
+    for (size_t k = 0; k < G.size(); ++k) {
+      out[2 * k] = G[k];
+      out[2 * k + 1] = 0.0;
+    }
+  });
+}
+
+int ed_green_function_density(int label, double *density) {
+  return guard([&] {
+    need(density != nullptr, "ed_green_function_density: null output");
+    *density = get_instance(label).gf_density();
+  });
+}
+
+int ed_tr_sigma_inf(int label, double *value) {
+  return guard([&] {
+    need(value != nullptr, "ed_tr_sigma_inf: null output");
+    *value = get_instance(label).tr_sigma_inf();
+  });
+}
+
+int ed_susceptibility(int label, const char *op, int nw, const double *w, double *chi) {
+  return guard([&] {
+    need(op != nullptr, "ed_susceptibility: null operator name");
+    need(nw >= 0 && (nw == 0 || (w && chi)), "ed_susceptibility: bad frequency arrays");
+    const auto parts = get_instance(label).susceptibility_parts(op);
+    // all states' poles in one Lehmann sum, residues weight * v^2, evaluated
+    // for every frequency in one launch (the 1 x 1 case of the Green-function
+    // Lehmann kernel)
+    std::vector<double> e, v;
+    for (const auto &c : parts) {
+      const double sw = std::sqrt(c.weight);
+      for (size_t i = 0; i < c.e.size(); ++i) {
+        e.push_back(c.e[i]);
+        v.push_back(c.v[i] * sw);
+      }
+    }
+    if (nw == 0) return;
+    if (e.empty()) {
+      std::memset(chi, 0, sizeof(double) * 2 * (size_t)nw);
+      return;
+    }
+    DevBuf<double> de, dv, dz, dchi((size_t)2 * nw);
+    de.upload(e);
+    dv.upload(v);
+    dz.upload(w, (size_t)2 * nw);
+    vdvh(1, (int)e.size(), de.p, dv.p, nw, dz.p, dchi.p, stream());
+    dchi.download(chi, (size_t)2 * nw);
+  });
+}
+
+int ed_susceptibility_poles(int label, const char *op, int cap, double *poles, int *n) {
+  return guard([&] {
+    need(op != nullptr && n != nullptr, "ed_susceptibility_poles: null argument");
+    need(cap >= 0 && (cap == 0 || poles), "ed_susceptibility_poles: bad output array");
+    const auto parts = get_instance(label).susceptibility_parts(op);
+    // poles above 1e-8 with residue weight v^2 above 1e-6, in the order found;
+    // a pole within 1e-4 of one already kept adds its residue to that one
+    std::vector<std::pair<double, double>> chi, kept;
+    for (const auto &c : parts)
+      for (size_t i = 0; i < c.e.size(); ++i) {
+        if (c.e[i] < 1e-8) continue;
+        const double r = c.v[i] * c.v[i] * c.weight;
+        if (r > 1e-6) chi.push_back({c.e[i], r});
+      }
+    for (const auto &x : chi) {
+      size_t j = 0;
+      for (; j < kept.size(); ++j)
+        if (std::fabs(kept[j].first - x.first) < 1e-4) {
+          kept[j].second += x.second;
+          break;
+        }
+      if (j == kept.size()) kept.push_back(x);
+    }
+    *n = (int)kept.size();
+    for (int k = 0; k < *n && k < cap; ++k) {
+      poles[2 * k] = kept[k].first;
+      poles[2 * k + 1] = kept[k].second;
+    }
+  });
+}
+
 // ---- Hamiltonian handles ----
 int ed_hamiltonian_create(int label, const char *sector, char format, int64_t *handle) {
   return guard([&] {

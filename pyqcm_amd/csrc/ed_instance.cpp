@@ -873,4 +873,164 @@ void Instance::self_energy(int nz, const double *z, bool spin_down, double *out)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Frequency-integrated Green function, self-energy at infinity, response
+// ---------------------------------------------------------------------------
+// b_0 of the hole fraction of every orbital pair q (o1 >= o2, row by row):
+// weight * <Omega| (c^dagger_o1 + c^dagger_o2)(c_o1 + c_o2) |Omega>, zero for a
+// pair whose start vector vanished (no fraction was built)
+static std::vector<double> hole_b0(const GFRep &R, int n) {
+  const int np = n * (n + 1) / 2;
+  std::vector<double> b0(np, 0.0);
+  if (R.store) {
+    const CfStore &S = *R.store;
+    if (R.frac0 + 2 * np > S.nfrac) fail(ED_ERR_INTERNAL, "continued-fraction store shorter than its Green function");
+    std::vector<int> off(2 * np + 1);
+    d2h_pageable(off.data(), S.off.p + R.frac0, off.size() * sizeof(int));
+    for (int q = 0; q < np; ++q) {
+      const int f = 2 * q + 1;  // hole fraction of pair q
+      if (off[f + 1] > off[f]) d2h_pageable(&b0[q], S.B.p + off[f], sizeof(double));
+    }
+  } else {
+    for (int q = 0; q < np && q < (int)R.hb.size(); ++q)
+      if (!R.hb[q].empty()) b0[q] = R.hb[q][0];
+  }
+  return b0;
+}
+
+// Green_function_average (src_ed/model_instance.hpp:641-652): the integrated
+// Green function of each state added into one matrix, G(b, a) += ... as in
+//  * Q_matrix::integrated_Green_function (src_ed/Q_matrix.hpp:155-168): the
+//    poles below zero, sum_i v_a v_b (the weight is in v);
+//  * continued_fraction_set::integrated_Green_function
+//    (src_ed/continued_fraction_set.cpp:104-125): the hole fractions' b_0 on
+//    the diagonal, and (b_0(o1, o2) - G_o1o1 - G_o2o2) / 2 off it, where G is the
+//    running sum over the states so far (with several states this subtracts
+//    the diagonals of the earlier states as well, as the reference does);
+//  * mcf_set::integrated_Green_function (src_ed/mcf_set.hpp:146-158): the
+//    hole weights' W_h^T W_h.
+std::vector<double> Instance::gf_average(bool spin_down) {
+  if (!gf_solved) green_function_solve();
+  const int spin = gf_spin(spin_down);
+  const int n = gf_dim();
+  std::vector<double> G((size_t)n * n, 0.0);
+  auto at = [&](int a, int b) -> double & { return G[a + (size_t)b * n]; };
+  for (auto &s : states) {
+    const GFRep &R = s->gf[spin];
+    if (R.kind == 'L') {
+      for (size_t i = 0; i < R.e.size(); ++i) {
+        if (R.e[i] >= 0.0) continue;
+        const double *v = R.V.data() + i * n;
+        for (int a = 0; a < n; ++a)
+          for (int b = 0; b < n; ++b) at(b, a) += v[a] * v[b];
+      }
+    } else if (R.kind == 'F') {
+      const std::vector<double> b0 = hole_b0(R, n);
+      auto q = [](int o1, int o2) { return o1 * (o1 + 1) / 2 + o2; };
+      for (int o1 = 0; o1 < n; ++o1) at(o1, o1) += b0[q(o1, o1)];
+      for (int o1 = 0; o1 < n; ++o1)
+        for (int o2 = 0; o2 < o1; ++o2) {
+          const double t = 0.5 * (b0[q(o1, o2)] - at(o1, o1) - at(o2, o2));
+          at(o1, o2) += t;
+          at(o2, o1) += t;
+        }
+    } else if (R.kind == 'M') {
+      const GFRep::MCF &mc = R.mh;
+      if (mc.nfl == 0 || mc.p == 0) continue;
+      // W is p x n column-major
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+          double t = 0.0;
+          for (int k = 0; k < mc.p; ++k) t += mc.W[k + (size_t)a * mc.p] * mc.W[k + (size_t)b * mc.p];
+          at(a, b) += t;
+        }
+    }
+  }
+  return G;
+}
+
+// Green_function_density (src_ed/model_instance.hpp:660-675): the trace of the
+// average (both spin blocks) per site, twice the up block without spin mixing
+double Instance::gf_density() {
+  double nG = 0.0;
+  const int n = gf_dim();
+  for (int sd = 0; sd < ((mixing & 4) ? 2 : 1); ++sd) {
+    const std::vector<double> G = gf_average(sd == 1);
+    for (int i = 0; i < n; ++i) nG += G[i + (size_t)i * n];
+  }
+  nG /= model->n_sites;
+  return mixing == 0 ? 2.0 * nG : nG;
+}
+
+// tr_sigma_inf (src_ed/model_instance.hpp:1278-1312): Tr Sigma(i / x) at x =
+// 1e-5, 2e-5, 3e-5 (both spin blocks; twice the up block without spin mixing),
+// extrapolated to x = 0 by the polynomial through the three points
+// (polynomial_fit, src_ed/model_instance.cpp:184-222, Neville's scheme for
+// that polynomial; written here in Lagrange form)
+double Instance::tr_sigma_inf() {
+  constexpr int nw = 3;
+  const int n = gf_dim();
+  double x[nw], z[2 * nw], S[nw] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < nw; ++i) {
+    x[i] = (i + 1) * 1.0e-5;
+    z[2 * i] = 0.0;
+    z[2 * i + 1] = 1.0 / x[i];
+  }
+  std::vector<double> sig((size_t)nw * n * n * 2);
+  for (int sd = 0; sd < ((mixing & 4) ? 2 : 1); ++sd) {
+    self_energy(nw, z, sd == 1, sig.data());
+    for (int i = 0; i < nw; ++i)
+      for (int j = 0; j < n; ++j) S[i] += sig[2 * ((size_t)i * n * n + j + (size_t)j * n)];
+  }
+  if (mixing == 0)
+    for (double &s : S) s *= 2.0;
+  double y = 0.0;
+  for (int i = 0; i < nw; ++i) {
+    double l = 1.0;
+    for (int j = 0; j < nw; ++j)
+      if (j != i) l *= (0.0 - x[j]) / (x[i] - x[j]);
+    y += S[i] * l;
+  }
+  return y;
+}
+
+// susceptibility / susceptibility_poles (src_ed/model_instance.hpp:1022-1058,
+// :1066-1108): for each target sector (in sector order) and each low-energy
+// state |Omega> in it, the band-Lanczos Lehmann representation (one start
+// vector, build_Q_matrix, src_ed/Hamiltonian/Hamiltonian_base.hpp:229-254) of
+// the Hamiltonian of that sector from O |Omega>, poles shifted by -E_Omega.
+// O is the operator as defined (its instance value is not applied).
+std::vector<Instance::ChiPart> Instance::susceptibility_parts(const std::string &op) {
+  if (!model->ops.count(op)) fail(ED_ERR_INVALID, "Operator " + op + " is not defined in the model.");
+  if (!value.count(op))
+    fail(ED_ERR_INVALID, "operator " + op + " is not activated in instance " + std::to_string(label));
+  if (!gs_solved) ground_state_solve();
+  TermList one;
+  one.L = model->L;
+  one.terms.push_back({&model->ops.at(op), 1.0});
+  const TermList T = terms();
+  std::vector<ChiPart> parts;
+  for (const Sector &sec : targets) {
+    bool any = false;
+    for (auto &s : states) any = any || s->sec == sec;
+    if (!any) continue;
+    auto H = make_hamiltonian(T, sec, format);
+    if (H->dim == 0) continue;
+    auto Ho = make_hamiltonian(one, sec, format);
+    for (auto &s : states) {
+      if (!(s->sec == sec)) continue;
+      DevBuf<double> b(H->dim);
+      Ho->apply(s->psi.p, b.p, 1.0, 0.0, false);
+      ChiPart c;
+      c.weight = s->weight;
+      std::vector<const double *> ph{b.p};
+      band_lanczos(*H, ph, c.e, c.v);
+      for (double &e : c.e) e -= s->energy;
+      vlog("susceptibility of %s in %s: %zu poles", op.c_str(), sec.name().c_str(), c.e.size());
+      parts.push_back(std::move(c));
+    }
+  }
+  return parts;
+}
+
 }  // namespace ed

@@ -82,6 +82,20 @@ struct Instance {
   // one-body matrix of the spin block (L x L, row-major) from the instance's terms
   std::vector<double> one_body_matrix(bool spin_down) const;  // L x L, row-major, full matrix
   double sef_bath() const;
+  // frequency integral of the Green function of the spin block, n x n
+  // column-major (real Hamiltonians: real and symmetric)
+  std::vector<double> gf_average(bool spin_down);
+  // electrons per site from the diagonal of gf_average
+  double gf_density();
+  // trace of the self-energy at infinite frequency, both spins
+  double tr_sigma_inf();
+  // Lehmann representation of the response of one instance operator, one
+  // part per low-energy state: poles e (excitation energies), residues v^2
+  struct ChiPart {
+    double weight = 0.0;
+    std::vector<double> e, v;
+  };
+  std::vector<ChiPart> susceptibility_parts(const std::string &op);
 };
 
 Instance &get_instance(int label);

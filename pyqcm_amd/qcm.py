@@ -9,7 +9,9 @@ code written against ``qcm`` ED functions runs unchanged:
     new_model, new_operator, new_model_instanceC, ground_state_solve,
     Green_function_solveC, Green_function, Green_function_dimensionC,
     self_energyC, hybridization_functionC, cluster_averages, qmatrix,
-    mixingC, model_sizeC, set_global_parameterC, erase_model_instance
+    mixingC, model_sizeC, set_global_parameterC, erase_model_instance,
+    Green_function_average, Green_function_density, susceptibility,
+    susceptibility_poles (and tr_sigma_inf, a C++-level function there)
 
 Vectorized variants (``Green_function_grid``) take whole frequency grids, so
 the device kernels evaluate every frequency in one launch.
@@ -25,7 +27,8 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "ground_state_solve", "Green_function_solveC", "Green_function", "Green_function_grid",
            "Green_function_dimensionC", "self_energyC", "hybridization_functionC", "cluster_averages",
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
-           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch"]
+           "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
+           "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -267,6 +270,51 @@ def Potthoff_functionalC(label=0):
     x = C.c_double()
     check(lib().ed_potthoff_functional(int(label), C.byref(x)))
     return x.value
+
+
+def Green_function_average(label=0, spin_down=0):
+    """qcm.Green_function_average (src_python/qcm_ED_wrap.hpp:44-50): the
+    frequency-integrated Green function, the matrix of <c^dagger_i c_j>, as a
+    (d, d) complex array (the reference's flat copy of the column-major block)."""
+    d = Green_function_dimensionC(label)
+    out = np.zeros(d * d, dtype=np.complex128)
+    check(lib().ed_green_function_average(int(label), int(spin_down), out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(d, d)
+
+
+def Green_function_density(label=0):
+    """qcm.Green_function_density (src_python/qcm_ED_wrap.hpp:52-54): electrons per site."""
+    x = C.c_double()
+    check(lib().ed_green_function_density(int(label), C.byref(x)))
+    return x.value
+
+
+def tr_sigma_inf(label=0):
+    """ED::tr_sigma_inf (src_ed/qcm_ED.cpp:345-348): trace of the self-energy at
+    infinite frequency, both spins."""
+    x = C.c_double()
+    check(lib().ed_tr_sigma_inf(int(label), C.byref(x)))
+    return x.value
+
+
+def susceptibility(op, w, label=0):
+    """qcm.susceptibility (src_python/qcm_ED_wrap.hpp:327-333): the dynamic
+    susceptibility of operator op at the complex frequencies w."""
+    ww, wp = _grid(w)
+    out = np.zeros(len(ww), dtype=np.complex128)
+    check(lib().ed_susceptibility(int(label), str(op).encode(), len(ww), wp, out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def susceptibility_poles(op, label=0):
+    """qcm.susceptibility_poles (src_python/qcm_ED_wrap.hpp:318-325): array (n, 2)
+    of (pole, residue)."""
+    n = C.c_int()
+    check(lib().ed_susceptibility_poles(int(label), str(op).encode(), 0, None, C.byref(n)))
+    out = np.zeros(2 * max(n.value, 1))
+    check(lib().ed_susceptibility_poles(int(label), str(op).encode(), n.value,
+                                        out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+    return out[:2 * n.value].reshape(n.value, 2)
 
 
 def cluster_averages(label=0):
