@@ -10,7 +10,9 @@ Also reported: ED solves per second on the 2x2-plaquette + 8-bath CDMFT
 impurity (sector N = 12, S = 0, dim 853 776; ground state + band-Lanczos G on
 1024 Matsubara frequencies, beta = 50), with |dE0|/|E0| against the CPU FP64
 oracle, and the CPU baselines (the oracle's OpenMP restatement) timed on this
-host in the same run.
+host in the same run; the Lanczos ground state of the 4x4 sector itself
+(E0 against the oracle's value in tests/golden/torus4x4_u8_gs.json); the
+batched VCA sweeps; the CPT and Potthoff-functional lattice sums.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve]
 """
@@ -173,6 +175,33 @@ def copy_calibration(nbytes=1 << 30):
     return nbytes
 
 
+def large_solve_gpu():
+    """Lanczos ground state (recurrence + eigenvector pass) of the 4x4 sector
+    as a fresh instance of the model the H.v leg registered."""
+    from pyqcm_amd import qcm, workloads as W
+    wl = W.torus(4, 4, U=8.0)
+    lab = 7002
+    qcm.new_model_instanceC("bench_torus4x4", dict(wl.values), "R0:N16:S0", lab)
+    t0 = time.perf_counter()
+    E0, _ = qcm.ground_state_solve(lab)
+    t1 = time.perf_counter()
+    qcm.erase_model_instance(lab)
+    return E0, t1 - t0
+
+
+def _large_leg():
+    E0, t = large_solve_gpu()
+    with open(os.path.join(ROOT, "tests", "golden", "torus4x4_u8_gs.json")) as f:
+        g = json.load(f)
+    return {"workload": "4x4 periodic Hubbard cluster t=1 U=8 mu=4, sector N=16 S=0 (dim 165 636 900): "
+                        "Lanczos ground state with eigenvector, one solve",
+            "gpu_s_per_solve": round(t, 3), "E0_gpu": E0, "E0_oracle": g["E0"],
+            "dE0_rel": abs(E0 - g["E0"]) / abs(g["E0"]),
+            "cpu_oracle_s_per_solve": g["seconds"], "cpu_oracle_cores": 8,
+            "cpu_note": "the oracle's OpenMP Lanczos on the same sector, timed once on the 8-core build "
+                        "host when the golden value was written (stop rule 1e-10), not in this run"}
+
+
 def ed_solve_gpu(nz=1024, beta=50.0, reps=5):
     import numpy as np
     from pyqcm_amd import qcm, workloads as W
@@ -320,6 +349,7 @@ def main():
     ap.add_argument("--no-hv", action="store_true", help="skip the H.v leg (profiling of the solve leg alone)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched VCA-sweep leg")
     ap.add_argument("--no-lattice", action="store_true", help="skip the CPT / Potthoff-functional leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 4x4 ground-state solve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
@@ -380,6 +410,8 @@ def main():
         out["cpu_baseline"] = {"value": round(cgbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
                                "sample": "oracle OpenMP factorized H.v on the same sector, 2 timed products "
                                          "(%.2f s each) after one untimed build" % per}
+    if not args.no_large:
+        out["large_solve"] = _large_leg()
     if not args.no_solve:
         out["ed_solves"] = _solve_leg(args)
     if not args.no_batch:

@@ -55,7 +55,7 @@ def _chain_ops(L, extra=None):
     return ops
 
 
-def _instance(name, n_sites, n_bath, ops, values, sector, method="L"):
+def _instance(name, n_sites, n_bath, ops, values, sector, method="L", solve=True):
     if name not in _models:
         qcm.new_model(name, n_sites, n_bath)
         for op, (typ, el) in ops.items():
@@ -68,7 +68,8 @@ def _instance(name, n_sites, n_bath, ops, values, sector, method="L"):
     for op in ops:
         vals.setdefault(op, 0.0)
     qcm.new_model_instanceC(name, vals, sector, lab)
-    qcm.ground_state_solve(lab)
+    if solve:
+        qcm.ground_state_solve(lab)
     return lab
 
 
@@ -150,6 +151,27 @@ def test_average_two_degenerate_states(method):
         assert abs(2.0 * Mu[1, 2].real - ave["h12u"][0]) < 1e-10
     assert abs(qcm.Green_function_density(lab) - 1.0) < 1e-10
     _finish(lab)
+
+
+def test_average_after_batched_solve():
+    """continued fractions held in the batched solver's device store: the
+    same averages as the per-instance solve, and as the operator averages"""
+    L = 4
+    points = [(4.0, 2.0), (2.5, 1.25), (6.0, 2.7)]
+    labs = [_instance("resp_chain4", L, 0, _chain_ops(L), {"t": 1.0, "U": U, "mu": mu}, "R0:N4:S0", "F", solve=False)
+            for (U, mu) in points]
+    qcm.set_global_parameterC("GF_method", "F")
+    qcm.solve_batch(labs)
+    for lab, (U, mu) in zip(labs, points):
+        M = qcm.Green_function_average(lab)
+        one = _instance("resp_chain4", L, 0, _chain_ops(L), {"t": 1.0, "U": U, "mu": mu}, "R0:N4:S0", "F")
+        ref = qcm.Green_function_average(one)
+        ave = qcm.cluster_averages(one)
+        _finish(one)
+        assert np.abs(M - ref).max() < 1e-10, (U, np.abs(M - ref).max())
+        assert abs(4.0 * M[0, 1].real - ave["h01"][0]) < 1e-10
+    for lab in labs:
+        _finish(lab)
 
 
 @pytest.mark.parametrize("method", METHODS)
