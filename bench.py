@@ -190,12 +190,15 @@ def large_solve_gpu():
 
 
 def _large_leg():
+    E0a, t_first = large_solve_gpu()
     E0, t = large_solve_gpu()
     with open(os.path.join(ROOT, "tests", "golden", "torus4x4_u8_gs.json")) as f:
         g = json.load(f)
     return {"workload": "4x4 periodic Hubbard cluster t=1 U=8 mu=4, sector N=16 S=0 (dim 165 636 900): "
-                        "Lanczos ground state with eigenvector, one solve",
-            "gpu_s_per_solve": round(t, 3), "E0_gpu": E0, "E0_oracle": g["E0"],
+                        "Lanczos ground state with eigenvector (Krylov basis kept in HBM)",
+            "gpu_s_per_solve": round(t, 3), "gpu_s_first_solve": round(t_first, 3),
+            "timing": "second of two solves; the first also maps the ~230 GB of device memory the basis takes",
+            "E0_gpu": E0, "E0_oracle": g["E0"],
             "dE0_rel": abs(E0 - g["E0"]) / abs(g["E0"]),
             "cpu_oracle_s_per_solve": g["seconds"], "cpu_oracle_cores": 8,
             "cpu_note": "the oracle's OpenMP Lanczos on the same sector, timed once on the 8-core build "

@@ -57,7 +57,7 @@ Params::Params() {
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
-  d["lanczos_basis_budget_GB"] = 96.0;
+  d["lanczos_basis_budget_GB"] = -1.0;  // negative: 90 % of the available device memory less 4 GB
 }
 
 bool Params::getb(const std::string &n) {
@@ -111,6 +111,13 @@ void dev_release_cache() {
   for (auto &kv : g_free) (void)hipFree(kv.second);
   g_free.clear();
   g_cached_bytes = 0;
+}
+
+size_t dev_available_bytes() {
+  size_t fr = 0, tot = 0;
+  ED_HIP(hipMemGetInfo(&fr, &tot));
+  std::lock_guard<std::mutex> l(g_alloc_mu);
+  return fr + g_cached_bytes;
 }
 
 void *dev_alloc(size_t bytes) {

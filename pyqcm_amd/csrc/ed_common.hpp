@@ -59,6 +59,8 @@ Params &params();
 void *dev_alloc(size_t bytes);
 void dev_free(void *p, size_t bytes);
 void dev_release_cache();
+// free device memory plus the blocks this library holds in its cache
+size_t dev_available_bytes();
 hipStream_t stream();  // the library's compute stream (default priority)
 
 // device-to-host copy into pageable memory through pinned staging buffers

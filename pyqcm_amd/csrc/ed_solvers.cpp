@@ -94,7 +94,8 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
   const int max_iter = (int)P.geti("max_iter_lanczos");
   const double accur = P.getd("accur_lanczos");
   const size_t n = H.dim;
-  const double budget = P.getd("lanczos_basis_budget_GB") * 1e9;
+  const double gb = P.getd("lanczos_basis_budget_GB");
+  const double budget = gb >= 0.0 ? gb * 1e9 : std::max(0.0, 0.9 * (double)dev_available_bytes() - 4e9);
   const bool reorth = P.getb("lanczos_reorth");
   bool store = P.getb("store_lanczos_basis") || reorth;
   LanczosGS R;
