@@ -226,6 +226,9 @@ int ed_device_alloc(int64_t bytes, void **ptr);
 int ed_device_free(void *ptr, int64_t bytes);
 int ed_device_copy(void *dst, const void *src, int64_t bytes, int kind /* 0 H2D, 1 D2H, 2 D2D */);
 int ed_device_sync(void);
+/* device blocks the library frees are kept for reuse (a solve's Krylov basis
+   included, up to ~235 GB on the 4x4 sector); this returns them to the driver,
+   e.g. before another library in the process allocates */
 int ed_release_cache(void);
 
 #ifdef __cplusplus
