@@ -38,10 +38,17 @@ enum {
   ED_ERR_NOT_FOUND = 7,      /* unknown model / instance label */
   ED_ERR_UNSUPPORTED = 8,    /* feature this library does not implement (point groups, complex operators) */
   ED_ERR_INTERNAL = 9,
-  ED_ERR_DEGENERATE = 10     /* band Lanczos degenerate gap with no_degenerate_BL (Lanczos.hpp:368-370) */
+  ED_ERR_DEGENERATE = 10,    /* band Lanczos degenerate gap with no_degenerate_BL (Lanczos.hpp:368-370) */
+  ED_ERR_CANCELLED = 11      /* ed_request_cancel was called (check_signals, src_python/common_Py.cpp:29-33) */
 };
 
 const char *ed_last_error(void);
+/* Interrupt: the running (or next) solve stops at its next Lanczos / band-Lanczos /
+   Davidson step or Green-function start vector with ED_ERR_CANCELLED, and the request
+   is consumed.  Replaces check_signals (src_python/common_Py.cpp:29-33, polled in
+   src_ed/Hamiltonian/Lanczos.hpp:66, :123, :172, :269, :473; model_instance.hpp:717,
+   :855).  Async-signal-safe and callable from any thread. */
+int ed_request_cancel(void);
 int ed_version(char *buf, int cap);
 int ed_device_count(int *n);
 

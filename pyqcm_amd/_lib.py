@@ -50,6 +50,7 @@ _kgp = C.POINTER(KGridC)
 
 SIGNATURES = {
     "ed_last_error": (C.c_char_p, []),
+    "ed_request_cancel": (C.c_int, []),
     "ed_version": (C.c_int, [C.c_char_p, C.c_int]),
     "ed_device_count": (C.c_int, [_ip]),
     "ed_set_global_bool": (C.c_int, [C.c_char_p, C.c_int]),

@@ -212,6 +212,7 @@ bool batchable(const Instance &I) {
 }  // namespace
 
 void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
+  check_cancel();
   Params &P = params();
   const int max_iter = (int)P.geti("max_iter_lanczos");
   const double accur = P.getd("accur_lanczos");
@@ -311,6 +312,7 @@ void solve_batch(const std::vector<Instance *> &insts, bool want_gf) {
     vlog("batch: states finalized");
   }
   if (!want_gf) return;
+  check_cancel();
 
   // ---- Green functions: continued fractions batched, other methods one by one ----
   std::vector<Instance *> cf;

@@ -60,6 +60,11 @@ extern "C" {
 
 const char *ed_last_error(void) { return g_last_error.c_str(); }
 
+int ed_request_cancel(void) {
+  request_cancel();
+  return ED_OK;
+}
+
 int ed_version(char *buf, int cap) {
   return guard([&] {
     const char *v = "pyqcm_amd ED 0.1 (gfx950, FP64)";

@@ -1,6 +1,7 @@
 // ed_common.cpp -- global parameters, caching device allocator, library stream.
 #include "ed_common.hpp"
 
+#include <atomic>
 #include <cstring>
 
 #include <chrono>
@@ -111,6 +112,16 @@ void dev_release_cache() {
   for (auto &kv : g_free) (void)hipFree(kv.second);
   g_free.clear();
   g_cached_bytes = 0;
+}
+
+namespace {
+std::atomic<int> g_cancel{0};
+}
+
+void request_cancel() { g_cancel.store(1, std::memory_order_relaxed); }
+
+void check_cancel() {
+  if (g_cancel.load(std::memory_order_relaxed) && g_cancel.exchange(0)) fail(ED_ERR_CANCELLED, "interrupted");
 }
 
 size_t dev_available_bytes() {

@@ -26,6 +26,11 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
 
+// interrupt requests (ed_request_cancel): check_cancel throws ED_ERR_CANCELLED
+// once per request; the solvers call it once per step
+void request_cancel();
+void check_cancel();
+
 #define ED_HIP(call)                                                                       \
   do {                                                                                     \
     hipError_t _e = (call);                                                                \

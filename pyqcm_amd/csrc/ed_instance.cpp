@@ -227,6 +227,7 @@ std::pair<double, std::string> Instance::ground_state_solve() {
   states.clear();
   GS_energy = 1.0e12;
   for (const auto &s : targets) {
+    check_cancel();
     const size_t dim = sector_dim(model->L, s);
     if (dim == 0) continue;
     auto H = make_hamiltonian(T, s, format);
@@ -431,6 +432,7 @@ static void build_gf(Instance &I, State &st, int spin) {
   if (G.kind == 'F') { G.ea.assign(np, {}); G.eb.assign(np, {}); G.ha.assign(np, {}); G.hb.assign(np, {}); }
   std::vector<double> eh, Vh, ee, Ve;  // Lehmann parts (hole, electron)
   for (int pm = -1; pm <= 1; pm += 2) {
+    check_cancel();
     Sector tgt = shift_sector(st.sec, pm, spin);
     if (!sector_valid(L, tgt)) continue;
     const size_t dt = sector_dim(L, tgt);

@@ -111,6 +111,7 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
   }
   double ritz = 2.0 * accur;
   while (ritz > accur) {
+    check_cancel();
     const int k = (int)R.alpha.size();
     double alpha, beta;
     if (store) {
@@ -197,6 +198,7 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
     d2d(d_gs, d_start, n);
     scale(n, d_gs, z[0], stream());
     for (int j = 1; j < kk; ++j) {
+      check_cancel();
       const double cb = (j > 1) ? -R.beta[j - 2] / bB : 0.0;
       H.apply(A.p, B.p, 1.0 / bA, cb, false);
       replay_step(n, B.p, A.p, d_gs, -R.alpha[j - 1] / bA, z[j] / R.beta[j - 1], stream());
@@ -223,6 +225,7 @@ void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, 
   d2d(A.p, d_psi, n);
   double bA = 1.0, bB = 1.0, prod = 1.0;
   while (prod > acc && std::fabs(b.back()) > acc && a.size() < max_cf) {
+    check_cancel();
     const size_t k = a.size();
     const double cb = k ? -b.back() / bB : 0.0;
     const int nb = H.apply(A.p, B.p, 1.0 / bA, cb, true);
@@ -269,6 +272,7 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
   int nd = 0, j;
   double ev_old = 1e12;
   for (j = 0; j < M0; ++j) {
+    check_cancel();
     // one multi-dot per step: ||v_j||^2, <v_j, phi_k> (P matrix), the step (6)
     // overlaps <v_j, v_k> (k = j+1 .. j+pc-1) and <v_j, v_k> for the vectors
     // step (8) subtracts (k0 <= k < j), all against the unnormalized v_j
@@ -497,6 +501,7 @@ int davidson(Hamiltonian &H, int K, double accuracy, std::vector<double> &evals,
   evals.assign(K, 0.0);
   int iter;
   for (iter = 0; iter < max_iter; ++iter) {
+    check_cancel();
     std::vector<double> mu(nb), a((size_t)nb * nb);
     symeig(nb, bhb.data(), mu.data(), a.data());
     std::vector<const double *> bp, hp;
@@ -598,6 +603,7 @@ int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q0, int M0,
   double ev_old = 1e12;
   int j;
   for (j = 0; j < M0; ++j) {
+    check_cancel();
     for (int k = 0; k < p; ++k) H.apply(Qc[k].p, Z[k].p, 1.0, 0.0, false);
     std::vector<double> Aj(pp);
     for (int l = 0; l < p; ++l) {

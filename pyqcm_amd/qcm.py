@@ -28,7 +28,8 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "Green_function_dimensionC", "self_energyC", "hybridization_functionC", "cluster_averages",
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
            "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
-           "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles"]
+           "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
+           "interrupt"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -348,6 +349,15 @@ def states(label=0):
     check(lib().ed_states(int(label), cap, e.ctypes.data_as(C.POINTER(C.c_double)),
                           w.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
     return e[:n.value], w[:n.value]
+
+
+def interrupt():
+    """Stop the running (or the next) solve at its next Lanczos / band-Lanczos /
+    Davidson step or Green-function start vector: that call raises EDError
+    with code 11.  The reference polls Python signals there (check_signals,
+    src_python/common_Py.cpp:29-33); here any thread, or a C signal handler
+    through ed_request_cancel, makes the request."""
+    check(lib().ed_request_cancel())
 
 
 def mixingC(label=0):

@@ -284,3 +284,58 @@ def test_response_errors():
     assert qcm.susceptibility("n0", [], lab).shape == (0,)
     qcm.erase_model_instance(lab2)
     _finish(lab)
+
+
+def test_interrupt_stops_the_next_step_once():
+    """a pending interrupt stops the ground-state and Green-function solves at
+    their first step with code 11 and is consumed: the calls then succeed"""
+    import threading
+    import time
+    L = 4
+    lab = _instance("resp_chain4", L, 0, _chain_ops(L), {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N4:S0", solve=False)
+    qcm.interrupt()
+    with pytest.raises(EDError) as e:
+        qcm.ground_state_solve(lab)
+    assert e.value.code == 11
+    E0, _ = qcm.ground_state_solve(lab)
+    qcm.interrupt()
+    with pytest.raises(EDError) as e:
+        qcm.Green_function_solveC(lab)
+    assert e.value.code == 11
+    G = qcm.Green_function(0.5j, label=lab)
+    assert np.all(np.isfinite(G))
+    _finish(lab)
+    # from another thread, during a longer solve (Davidson on 3 states of a
+    # 12-orbital sector, ~70 000 states)
+    qcm.new_model("resp_chain12", 12, 0)
+    for op, (typ, el) in _chain_ops(12).items():
+        qcm.new_operator("resp_chain12", op, typ, [(r + 1, c + 1, v) for (r, c, v) in el])
+    _label[0] += 1
+    lab = _label[0]
+    qcm.new_model_instanceC("resp_chain12", {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N12:S0", lab)
+    H = qcm.Hamiltonian(lab, "R0:N12:S0", "G")
+    out = {}
+
+    def run():
+        t0 = time.perf_counter()
+        try:
+            H.davidson(3)
+            out["code"] = 0
+        except EDError as err:
+            out["code"] = err.code
+        out["s"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run)
+    th.start()
+    time.sleep(0.05)
+    qcm.interrupt()
+    th.join(60)
+    assert not th.is_alive()
+    # either the request landed inside the iteration (code 11) or the solve
+    # had already finished; a request left pending is consumed by the next step
+    assert out["code"] in (0, 11), out
+    if out["code"] == 0:
+        with pytest.raises(EDError):
+            H.davidson(1)
+    del H
+    qcm.erase_model_instance(lab)
