@@ -19,9 +19,15 @@ using namespace ed;
 
 namespace {
 thread_local std::string g_last_error;
+// One caller at a time: the library's device stream, reduction buffers and
+// registries are shared, so concurrent calls (the reference's OpenMP loop over
+// clusters, src_qcm/lattice_model_instance.cpp:107, :160) queue here.
+// ed_request_cancel and ed_last_error do not take it.
+std::recursive_mutex g_api_mu;
 
 template <typename F>
 int guard(F &&f) {
+  std::lock_guard<std::recursive_mutex> lock(g_api_mu);
   try {
     f();
     return ED_OK;

@@ -339,3 +339,37 @@ def test_interrupt_stops_the_next_step_once():
             H.davidson(1)
     del H
     qcm.erase_model_instance(lab)
+
+
+def test_concurrent_callers_get_the_serial_results():
+    """four threads solving and evaluating different instances at once (the
+    reference's OpenMP loop over clusters) get the results of serial calls"""
+    import threading
+    L = 4
+    pts = [(3.0, 1.5), (4.0, 2.0), (5.0, 2.2), (6.0, 3.0)]
+    z = np.array([0.3 + 0.1j, 1.1j, -0.7 + 0.2j])
+    labs = [_instance("resp_chain4", L, 0, _chain_ops(L), {"t": 1.0, "U": U, "mu": mu}, "R0:N4:S0", solve=False)
+            for (U, mu) in pts]
+    got = {}
+
+    def run(lab):
+        E0, _ = qcm.ground_state_solve(lab)
+        got[lab] = (E0, qcm.Green_function_grid(z, label=lab), qcm.Green_function_average(lab))
+
+    ths = [threading.Thread(target=run, args=(lab,)) for lab in labs]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join(120)
+    assert all(not t.is_alive() for t in ths) and len(got) == len(labs)
+    for lab, (U, mu) in zip(labs, pts):
+        one = _instance("resp_chain4", L, 0, _chain_ops(L), {"t": 1.0, "U": U, "mu": mu}, "R0:N4:S0")
+        E0 = qcm.ground_state_solve(one)[0]
+        G = qcm.Green_function_grid(z, label=one)
+        M = qcm.Green_function_average(one)
+        _finish(one)
+        assert abs(got[lab][0] - E0) <= 1e-12 * abs(E0)
+        assert np.abs(got[lab][1] - G).max() < 1e-10
+        assert np.abs(got[lab][2] - M).max() < 1e-10
+    for lab in labs:
+        _finish(lab)
