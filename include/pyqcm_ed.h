@@ -207,6 +207,19 @@ int ed_sef_grid(int label, int nw, const double *w, const double *weights, const
 /* ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term */
 int ed_potthoff_functional(int label, double *omega);
 
+/* ---- solved instances in HDF5 files (the reference's layout) ---- */
+/* ED::write_instance_hdf5 (src_ed/qcm_ED.cpp:373-389; model_instance.hpp:1351-1375;
+   state.hpp:52-72): the solved instance (Green function solved first) into `group`
+   of `filename` (created if absent; an existing group of that name is replaced). */
+int ed_write_instance_hdf5(const char *filename, int label, const char *group);
+/* ED::read_instance_hdf5 (src_ed/qcm_ED.cpp:392-402; model_instance.hpp:1382-1420):
+   the instance takes the file's low-energy states and Green functions and counts as
+   solved (the format holds no wavefunction: cluster averages and susceptibilities of
+   such an instance return ED_ERR_INVALID). */
+int ed_read_instance_hdf5(const char *filename, int label, const char *group);
+/* version of the HDF5 C library found at run time (ED_ERR_UNSUPPORTED if none) */
+int ed_hdf5_version(char *buf, int cap);
+
 /* ---- frequency integrals and responses of one instance ---- */
 /* ED::Green_function_average (src_ed/qcm_ED.cpp:240-248; model_instance.hpp:641-652):
    the frequency-integrated Green function of the spin block, <c^dagger_b c_a>; out

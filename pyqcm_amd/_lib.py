@@ -79,6 +79,9 @@ SIGNATURES = {
     "ed_cpt_projected_gf": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _slp, _kgp, _dp]),
     "ed_sef_grid": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _slp, _slp, _kgp, _dp, _dp]),
     "ed_potthoff_functional": (C.c_int, [C.c_int, _dp]),
+    "ed_write_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),
+    "ed_read_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),
+    "ed_hdf5_version": (C.c_int, [C.c_char_p, C.c_int]),
     "ed_green_function_average": (C.c_int, [C.c_int, C.c_int, _dp]),
     "ed_green_function_density": (C.c_int, [C.c_int, _dp]),
     "ed_tr_sigma_inf": (C.c_int, [C.c_int, _dp]),

@@ -355,6 +355,28 @@ int ed_potthoff_functional(int label, double *omega) {
   });
 }
 
+int ed_write_instance_hdf5(const char *filename, int label, const char *group) {
+  return guard([&] {
+    need(filename && group, "ed_write_instance_hdf5: null argument");
+    write_instance_hdf5(filename, label, group);
+  });
+}
+
+int ed_read_instance_hdf5(const char *filename, int label, const char *group) {
+  return guard([&] {
+    need(filename && group, "ed_read_instance_hdf5: null argument");
+    read_instance_hdf5(filename, label, group);
+  });
+}
+
+int ed_hdf5_version(char *buf, int cap) {
+  return guard([&] {
+    const std::string v = hdf5_version();
+    need(buf && cap > (int)v.size(), "buffer too small");
+    std::strcpy(buf, v.c_str());
+  });
+}
+
 int ed_green_function_average(int label, int spin_down, double *out) {
   return guard([&] {
     need(out != nullptr, "ed_green_function_average: null output");

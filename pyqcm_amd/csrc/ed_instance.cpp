@@ -338,6 +338,8 @@ const std::vector<std::tuple<std::string, double, double>> &Instance::cluster_av
   if (!gs_solved) ground_state_solve();
   if (averages_done) return averages;
   // <O> and <O^2> - <O>^2 per operator (Hermitian_operator.hpp:67-83)
+  for (auto &x : states)
+    if (!x->psi.p) fail(ED_ERR_INVALID, "instance " + std::to_string(label) + " holds no ground-state vector (read from a file)");
   averages.clear();
   for (const auto &kv : value) {
     double ave = 0.0, var = 0.0;
@@ -1007,6 +1009,8 @@ std::vector<Instance::ChiPart> Instance::susceptibility_parts(const std::string 
   if (!value.count(op))
     fail(ED_ERR_INVALID, "operator " + op + " is not activated in instance " + std::to_string(label));
   if (!gs_solved) ground_state_solve();
+  for (auto &s : states)
+    if (!s->psi.p) fail(ED_ERR_INVALID, "instance " + std::to_string(label) + " holds no ground-state vector (read from a file)");
   TermList one;
   one.L = model->L;
   one.terms.push_back({&model->ops.at(op), 1.0});

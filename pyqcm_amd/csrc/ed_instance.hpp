@@ -124,6 +124,11 @@ CdagMaps cdag_maps(int L, int pm, int a, const Sector &src);
 void green_function_batch(const std::vector<Instance *> &insts, int nz, const double *z, bool spin_down,
                           double *out);
 size_t sector_dim(int L, const Sector &s);
+// solved instances to / from HDF5 files (ed_hdf5.cpp; the HDF5 C library is
+// loaded at run time)
+std::string hdf5_version();
+void write_instance_hdf5(const std::string &filename, int label, const std::string &group);
+void read_instance_hdf5(const std::string &filename, int label, const std::string &group);
 bool sector_valid(int L, const Sector &s);
 Sector shift_sector(const Sector &s, int pm, int spin);
 

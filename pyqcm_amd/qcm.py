@@ -29,7 +29,7 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
            "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
            "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
-           "interrupt"]
+           "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -271,6 +271,25 @@ def Potthoff_functionalC(label=0):
     x = C.c_double()
     check(lib().ed_potthoff_functional(int(label), C.byref(x)))
     return x.value
+
+
+def write_instance_to_hdf5(filename, group, label=0):
+    """qcm.write_instance_to_hdf5 (src_python/qcm_ED_wrap.hpp:364-369): the solved
+    instance into a group of an HDF5 file, in the reference's layout."""
+    check(lib().ed_write_instance_hdf5(str(filename).encode(), int(label), str(group).encode()))
+
+
+def read_instance_from_hdf5(filename, group, label=0):
+    """qcm.read_instance_from_hdf5 (src_python/qcm_ED_wrap.hpp:371-376): the
+    instance takes the states and Green functions stored in the group."""
+    check(lib().ed_read_instance_hdf5(str(filename).encode(), int(label), str(group).encode()))
+
+
+def hdf5_version():
+    """version of the HDF5 C library the solver found (EDError code 8 if none)"""
+    buf = C.create_string_buffer(64)
+    check(lib().ed_hdf5_version(buf, 64))
+    return buf.value.decode()
 
 
 def Green_function_average(label=0, spin_down=0):

@@ -115,3 +115,17 @@ def test_tridiag_lowest_matches_dense_eigh(case, seed):
         assert abs(abs(z @ V[:, 0]) - 1.0) <= 1e-10
         # the component the Lanczos stop rule uses, |z_last|
         assert abs(abs(z[-1]) - abs(V[-1, 0])) <= 1e-12
+
+
+def test_hdf5_library_found_at_run_time():
+    """the solver opens the HDF5 C library at run time (solved instances to
+    and from files); where no library exists the entry points report code 8"""
+    from pyqcm_amd import qcm
+    from pyqcm_amd._lib import EDError
+    try:
+        v = qcm.hdf5_version()
+    except EDError as e:
+        assert e.code == 8
+        pytest.skip("no HDF5 C library on this machine")
+    major, minor, _ = (int(x) for x in v.split("."))
+    assert (major, minor) >= (1, 10)
