@@ -207,6 +207,31 @@ int ed_sef_grid(int label, int nw, const double *w, const double *weights, const
 /* ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term */
 int ed_potthoff_functional(int label, double *omega);
 
+/* ---- one-body data, operators, overlaps ---- */
+/* ED::complex_HS (src_ed/qcm_ED.hpp:94): always 0 here (real operators only) */
+int ed_complex_hs(int label, int *is_complex);
+/* ED::hopping_matrix / hopping_matrix_full (src_ed/qcm_ED.cpp:269-284;
+   model_instance.cpp:13-16, model_instance.hpp:1491-1522): the one-body matrix of the
+   spin block; full = 0: the cluster block (n_sites); full = 1: cluster and bath
+   (n_sites + n_bath), with diag = 1 the bath block diagonalized (set_hopping_matrix,
+   model_instance.hpp:424-474: eigenvalues ascending unless the block is diagonal
+   already; eigenvector signs are the library's own).  out: d x d complex column-major. */
+int ed_hopping_matrix(int label, int spin_down, int diag, int full, double *out);
+/* ED::interactions (model_instance.hpp:1527-1540): (r, c, U v) of every element of every
+   interaction operator of the instance (instance parameters in name order); *n = count,
+   the first cap written. */
+int ed_interactions(int label, int cap, int *r, int *c, double *v, int *n);
+/* ED::hybridization (model_instance.hpp:1455-1466): the diagonalized bath energies
+   e[n_bath] and the cluster-bath matrix tcb (n_sites x n_bath complex column-major). */
+int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb);
+/* ED::matrix_elements (src_ed/qcm_ED.cpp:442-450): the operator's type ("one_body" or
+   "interaction") and its stored elements (0-based; one-body ones Hermitian-completed). */
+int ed_matrix_elements(const char *model, const char *op, char *type, int type_cap, int cap, int *r, int *c,
+                       double *v, int *n);
+/* ED::fidelity (src_ed/qcm_ED.cpp:360-368; model_instance.hpp:1261-1273): sum over pairs of
+   low-energy states of the two instances in the same sector of |<psi_1|psi_2>|^2 w_1 w_2. */
+int ed_fidelity(int label1, int label2, double *f);
+
 /* ---- solved instances in HDF5 files (the reference's layout) ---- */
 /* ED::write_instance_hdf5 (src_ed/qcm_ED.cpp:373-389; model_instance.hpp:1351-1375;
    state.hpp:52-72): the solved instance (Green function solved first) into `group`

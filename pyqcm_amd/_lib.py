@@ -79,6 +79,12 @@ SIGNATURES = {
     "ed_cpt_projected_gf": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _slp, _kgp, _dp]),
     "ed_sef_grid": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _slp, _slp, _kgp, _dp, _dp]),
     "ed_potthoff_functional": (C.c_int, [C.c_int, _dp]),
+    "ed_complex_hs": (C.c_int, [C.c_int, _ip]),
+    "ed_hopping_matrix": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "ed_interactions": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _dp, _ip]),
+    "ed_hybridization_lehmann": (C.c_int, [C.c_int, C.c_int, _dp, _dp]),
+    "ed_matrix_elements": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, _ip, _ip, _dp, _ip]),
+    "ed_fidelity": (C.c_int, [C.c_int, C.c_int, _dp]),
     "ed_write_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),
     "ed_read_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),
     "ed_hdf5_version": (C.c_int, [C.c_char_p, C.c_int]),

@@ -355,6 +355,106 @@ int ed_potthoff_functional(int label, double *omega) {
   });
 }
 
+int ed_complex_hs(int label, int *is_complex) {
+  return guard([&] {
+    need(is_complex != nullptr, "ed_complex_hs: null output");
+    get_instance(label);
+    *is_complex = 0;  // complex operators are rejected at new_operator
+  });
+}
+
+int ed_hopping_matrix(int label, int spin_down, int diag, int full, double *out) {
+  return guard([&] {
+    need(out != nullptr, "ed_hopping_matrix: null output");
+    Instance &I = get_instance(label);
+    const int ns = I.model->n_sites, nb = I.model->n_bath;
+    const Instance::Hopping H = I.hopping_blocks(spin_down != 0, full && diag);
+    const int d = full ? ns + nb : ns;
+    std::vector<double> m((size_t)d * d, 0.0);
+    for (int a = 0; a < ns; ++a)
+      for (int b = 0; b < ns; ++b) m[a + (size_t)b * d] = H.tc[a + (size_t)b * ns];
+    if (full) {
+      for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) m[ns + i + (size_t)(ns + j) * d] = H.tb[i + (size_t)j * nb];
+      for (int a = 0; a < ns; ++a)
+        for (int i = 0; i < nb; ++i) {
+          m[a + (size_t)(ns + i) * d] = H.tcb[a + (size_t)i * ns];
+          m[ns + i + (size_t)a * d] = H.tcb[a + (size_t)i * ns];
+        }
+    }
+    for (size_t k = 0; k < m.size(); ++k) {
+      out[2 * k] = m[k];
+      out[2 * k + 1] = 0.0;
+    }
+  });
+}
+
+int ed_interactions(int label, int cap, int *r, int *c, double *v, int *n) {
+  return guard([&] {
+    need(n != nullptr && (cap == 0 || (r && c && v)), "ed_interactions: bad output arrays");
+    Instance &I = get_instance(label);
+    int k = 0;
+    for (const auto &kv : I.value) {
+      const Operator &op = I.model->ops.at(kv.first);
+      if (op.kind != OP_INTERACTION) continue;
+      for (const auto &x : op.el) {
+        if (k < cap) {
+          r[k] = x.r;
+          c[k] = x.c;
+          v[k] = x.v * kv.second;
+        }
+        ++k;
+      }
+    }
+    *n = k;
+  });
+}
+
+int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb) {
+  return guard([&] {
+    Instance &I = get_instance(label);
+    const int ns = I.model->n_sites, nb = I.model->n_bath;
+    if (nb == 0) fail(ED_ERR_INVALID, "System has no bath. Hybridization not defined.");
+    need(e && tcb, "ed_hybridization_lehmann: null output");
+    const Instance::Hopping H = I.hopping_blocks(spin_down != 0, true);
+    for (int i = 0; i < nb; ++i) e[i] = H.tb[i + (size_t)i * nb];
+    for (size_t k = 0; k < (size_t)ns * nb; ++k) {
+      tcb[2 * k] = H.tcb[k];
+      tcb[2 * k + 1] = 0.0;
+    }
+  });
+}
+
+int ed_matrix_elements(const char *model, const char *op, char *type, int type_cap, int cap, int *r, int *c,
+                       double *v, int *n) {
+  return guard([&] {
+    need(model && op && n && (cap == 0 || (r && c && v)), "ed_matrix_elements: bad arguments");
+    Model &M = get_model(model);
+    auto it = M.ops.find(op);
+    if (it == M.ops.end())
+      fail(ED_ERR_INVALID, std::string("operator ") + op + " is not defined in model . Check spelling.");
+    const std::string t = it->second.kind == OP_INTERACTION ? "interaction" : "one_body";
+    if (type) {
+      need(type_cap > (int)t.size(), "type buffer too small");
+      std::strcpy(type, t.c_str());
+    }
+    const auto &el = it->second.el;
+    *n = (int)el.size();
+    for (int k = 0; k < *n && k < cap; ++k) {
+      r[k] = el[k].r;
+      c[k] = el[k].c;
+      v[k] = el[k].v;
+    }
+  });
+}
+
+int ed_fidelity(int label1, int label2, double *f) {
+  return guard([&] {
+    need(f != nullptr, "ed_fidelity: null output");
+    *f = get_instance(label1).fidelity(get_instance(label2));
+  });
+}
+
 int ed_write_instance_hdf5(const char *filename, int label, const char *group) {
   return guard([&] {
     need(filename && group, "ed_write_instance_hdf5: null argument");

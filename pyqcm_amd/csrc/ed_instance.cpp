@@ -877,6 +877,54 @@ void Instance::self_energy(int nz, const double *z, bool spin_down, double *out)
   }
 }
 
+Instance::Hopping Instance::hopping_blocks(bool spin_down, bool diag) const {
+  const int L = model->L, ns = model->n_sites, nb = model->n_bath;
+  const std::vector<double> h = one_body_matrix(spin_down && (mixing & 4));  // row-major L x L
+  Hopping H;
+  H.tc.assign((size_t)ns * ns, 0.0);
+  H.tb.assign((size_t)nb * nb, 0.0);
+  H.tcb.assign((size_t)ns * nb, 0.0);
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b) H.tc[a + (size_t)b * ns] = h[(size_t)a * L + b];
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) H.tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+  for (int a = 0; a < ns; ++a)
+    for (int i = 0; i < nb; ++i) H.tcb[a + (size_t)i * ns] = h[(size_t)a * L + ns + i];
+  if (!diag || nb == 0) return H;
+  bool is_diag = true;
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j)
+      if (i != j && H.tb[i + (size_t)j * nb] != 0.0) is_diag = false;
+  if (is_diag) return H;
+  std::vector<double> eb(nb), S((size_t)nb * nb);
+  symeig(nb, H.tb.data(), eb.data(), S.data());
+  std::vector<double> t((size_t)ns * nb, 0.0);
+  for (int a = 0; a < ns; ++a)
+    for (int i = 0; i < nb; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nb; ++k) s += H.tcb[a + (size_t)k * ns] * S[k + (size_t)i * nb];
+      t[a + (size_t)i * ns] = s;
+    }
+  H.tcb.swap(t);
+  H.tb.assign((size_t)nb * nb, 0.0);
+  for (int i = 0; i < nb; ++i) H.tb[i + (size_t)i * nb] = eb[i];
+  return H;
+}
+
+double Instance::fidelity(Instance &other) {
+  if (!gs_solved) ground_state_solve();
+  if (!other.gs_solved) other.ground_state_solve();
+  double z = 0.0;
+  for (auto &s1 : states)
+    for (auto &s2 : other.states) {
+      if (!(s1->sec == s2->sec)) continue;
+      if (!s1->psi.p || !s2->psi.p) fail(ED_ERR_INVALID, "fidelity needs the ground-state vectors (not read from a file)");
+      const double zz = ddot(s1->psi.n, s1->psi.p, s2->psi.p);
+      z += zz * zz * s1->weight * s2->weight;
+    }
+  return z;
+}
+
 // ---------------------------------------------------------------------------
 // Frequency-integrated Green function, self-energy at infinity, response
 // ---------------------------------------------------------------------------

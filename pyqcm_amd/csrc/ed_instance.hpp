@@ -96,6 +96,17 @@ struct Instance {
     std::vector<double> e, v;
   };
   std::vector<ChiPart> susceptibility_parts(const std::string &op);
+  // set_hopping_matrix (src_ed/model_instance.hpp:424-474) for the spin block
+  // of spin_down: cluster tc (ns x ns), bath tb (nb x nb), cluster-bath tcb
+  // (ns x nb), column-major; with `diag` the bath block is diagonalized when it
+  // is not diagonal already (eigenvalues ascending, tcb -> tcb S)
+  struct Hopping {
+    std::vector<double> tc, tb, tcb;
+  };
+  Hopping hopping_blocks(bool spin_down, bool diag) const;
+  // sum over pairs of low-energy states in the same sector of
+  // |<psi_1|psi_2>|^2 w_1 w_2 (model_instance<T>::fidelity, :1261-1273)
+  double fidelity(Instance &other);
 };
 
 Instance &get_instance(int label);

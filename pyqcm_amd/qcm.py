@@ -29,7 +29,8 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "qmatrix", "mixingC", "model_sizeC", "set_global_parameterC", "set_global_parameter",
            "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
            "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
-           "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version"]
+           "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version", "ED_complex_HS",
+           "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -270,6 +271,76 @@ def Potthoff_functionalC(label=0):
     """ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 plus the bath term."""
     x = C.c_double()
     check(lib().ed_potthoff_functional(int(label), C.byref(x)))
+    return x.value
+
+
+def ED_complex_HS(label=0):
+    """qcm.ED_complex_HS (src_python/qcm_ED_wrap.hpp:27-30): 0 (real Hilbert space)."""
+    x = C.c_int()
+    check(lib().ed_complex_hs(int(label), C.byref(x)))
+    return x.value
+
+
+def _model_sizes(label):
+    name = _instances.get(int(label), (None,))[0]
+    if name is None or name not in _models:
+        raise EDError(7, "unknown instance %d" % label)
+    return _models[name][0], _models[name][1]
+
+
+def hopping_matrix(spin_down=0, diag=0, label=0, full=0):
+    """qcm.hopping_matrix (src_python/qcm_ED_wrap.hpp:94-106): the one-body matrix of
+    the cluster (full=0) or of cluster and bath (full=1; diag=1: bath diagonalized),
+    with the reference's flat copy of the column-major matrix."""
+    ns, nb = _model_sizes(label)
+    d = ns + nb if full else ns
+    out = np.zeros(d * d, dtype=np.complex128)
+    check(lib().ed_hopping_matrix(int(label), int(spin_down), int(diag), int(full),
+                                  out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(d, d)
+
+
+def interactions(label=0):
+    """qcm.interactions (src_python/qcm_ED_wrap.hpp:108-111): [(r, c, U v)]"""
+    n = C.c_int()
+    check(lib().ed_interactions(int(label), 0, None, None, None, C.byref(n)))
+    m = max(n.value, 1)
+    r, c, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m)
+    check(lib().ed_interactions(int(label), n.value, r.ctypes.data_as(C.POINTER(C.c_int)),
+                                c.ctypes.data_as(C.POINTER(C.c_int)), v.ctypes.data_as(C.POINTER(C.c_double)),
+                                C.byref(n)))
+    return [(int(r[k]), int(c[k]), float(v[k])) for k in range(n.value)]
+
+
+def hybridization(label=0):
+    """qcm.hybridization (src_python/qcm_ED_wrap.hpp:351-362): (bath energies,
+    cluster-bath matrix) with the reference's (L, M) flat copy of the column-major
+    n_sites x n_bath block."""
+    ns, nb = _model_sizes(label)
+    e = np.zeros(max(nb, 1))
+    t = np.zeros(max(ns * nb, 1), dtype=np.complex128)
+    check(lib().ed_hybridization_lehmann(int(label), 0, e.ctypes.data_as(C.POINTER(C.c_double)),
+                                         t.ctypes.data_as(C.POINTER(C.c_double))))
+    return e[:nb], t[:ns * nb].reshape(ns, nb)
+
+
+def matrix_elements(model, op):
+    """qcm.matrix_elements (src_python/qcm_ED_wrap.hpp:121-130): (type, [(r, c, v)])"""
+    n = C.c_int()
+    tb = C.create_string_buffer(32)
+    check(lib().ed_matrix_elements(model.encode(), op.encode(), tb, 32, 0, None, None, None, C.byref(n)))
+    m = max(n.value, 1)
+    r, c, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m)
+    check(lib().ed_matrix_elements(model.encode(), op.encode(), tb, 32, n.value, r.ctypes.data_as(C.POINTER(C.c_int)),
+                                   c.ctypes.data_as(C.POINTER(C.c_int)), v.ctypes.data_as(C.POINTER(C.c_double)),
+                                   C.byref(n)))
+    return tb.value.decode(), [(int(r[k]), int(c[k]), complex(v[k])) for k in range(n.value)]
+
+
+def fidelity(label1, label2):
+    """qcm.fidelity (src_python/qcm_ED_wrap.hpp:378-381)"""
+    x = C.c_double()
+    check(lib().ed_fidelity(int(label1), int(label2), C.byref(x)))
     return x.value
 
 

@@ -129,3 +129,76 @@ def test_hdf5_library_found_at_run_time():
         pytest.skip("no HDF5 C library on this machine")
     major, minor, _ = (int(x) for x in v.split("."))
     assert (major, minor) >= (1, 10)
+
+
+def _bath_instance(label, diag_bath=False):
+    """2 cluster sites + 2 bath orbitals; the bath block coupled (tb) unless
+    diag_bath; elements 0-based (r, c, v) per spin"""
+    ns, nb = 2, 2
+    L = ns + nb
+    one = {"t": [(0, 1, -1.0)], "V": [(0, 2, 0.5), (1, 3, 0.4), (0, 3, 0.2)],
+           "eb": [(2, 2, 1.0), (3, 3, -0.7)], "mu": [(0, 0, -1.0), (1, 1, -1.0)]}
+    if not diag_bath:
+        one["tb"] = [(2, 3, 0.3)]
+    name = "capi_bath%d" % int(diag_bath)
+    if name not in qcm._models:
+        qcm.new_model(name, ns, nb)
+        for k, el in one.items():
+            qcm.new_operator(name, k, "one-body", [(r + 1, c + 1, v) for (r, c, v) in el + [(r + L, c + L, v) for (r, c, v) in el]])
+        qcm.new_operator(name, "U", "interaction", [(i + 1, i + L + 1, 1.0) for i in range(ns)])
+    vals = {k: 1.0 for k in one}
+    vals.update({"mu": 0.2, "U": 3.0})
+    qcm.new_model_instanceC(name, vals, "R0:N4:S0", label)
+    h = np.zeros((L, L))
+    for k, el in one.items():
+        for r, c, v in el:
+            h[r, c] += vals[k] * v
+            if r != c:
+                h[c, r] += vals[k] * v
+    return h
+
+
+def test_hopping_matrix_hybridization_interactions_matrix_elements():
+    """host-side one-body data of an instance (set_hopping_matrix,
+    hopping_matrix(_full), hybridization, interactions, matrix_elements)
+    against the one-body matrix assembled here from the same elements"""
+    h = _bath_instance(61)
+    ns, nb = 2, 2
+    assert qcm.ED_complex_HS(61) == 0
+    assert np.abs(qcm.hopping_matrix(label=61) - h[:ns, :ns]).max() == 0.0
+    assert np.abs(qcm.hopping_matrix(label=61, full=1) - h).max() == 0.0
+    Hd = qcm.hopping_matrix(label=61, full=1, diag=1).real
+    eb, S = np.linalg.eigh(h[ns:, ns:])
+    assert np.abs(np.diag(Hd[ns:, ns:]) - eb).max() < 1e-14
+    assert np.abs(Hd[ns:, ns:] - np.diag(np.diag(Hd[ns:, ns:]))).max() == 0.0
+    tcb = h[:ns, ns:] @ S
+    for i in range(nb):  # each bath eigenvector up to its sign
+        col = Hd[:ns, ns + i]
+        assert min(np.abs(col - tcb[:, i]).max(), np.abs(col + tcb[:, i]).max()) < 1e-14
+    assert np.abs(Hd[ns:, :ns] - Hd[:ns, ns:].T).max() == 0.0
+    e, t = qcm.hybridization(61)
+    assert np.abs(e - eb).max() < 1e-14
+    # the binding's flat copy of the column-major n_sites x n_bath block
+    assert np.abs(t.reshape(-1) - Hd[:ns, ns:].reshape(-1, order="F")).max() == 0.0
+    assert qcm.interactions(61) == [(0, 4, 3.0), (1, 5, 3.0)]
+    typ, el = qcm.matrix_elements("capi_bath0", "V")
+    assert typ == "one_body"
+    assert el[:4] == [(0, 2, 0.5 + 0j), (2, 0, 0.5 + 0j), (1, 3, 0.4 + 0j), (3, 1, 0.4 + 0j)]
+    assert qcm.matrix_elements("capi_bath0", "U") == ("interaction", [(0, 4, 1 + 0j), (1, 5, 1 + 0j)])
+    # a bath block that is diagonal already keeps its order
+    h2 = _bath_instance(62, diag_bath=True)
+    Hd2 = qcm.hopping_matrix(label=62, full=1, diag=1).real
+    assert np.abs(Hd2 - h2).max() == 0.0
+    e2, _ = qcm.hybridization(62)
+    assert list(e2) == [1.0, -0.7]
+    with pytest.raises(_lib.EDError) as err:
+        qcm.matrix_elements("capi_bath0", "nope")
+    assert err.value.code == 1
+    qcm.new_model("capi_nobath", 2, 0)
+    qcm.new_operator("capi_nobath", "t", "one-body", [(1, 2, -1.0), (3, 4, -1.0)])
+    qcm.new_model_instanceC("capi_nobath", {"t": 1.0}, "R0:N2:S0", 63)
+    with pytest.raises(_lib.EDError) as err:
+        qcm.hybridization(63)
+    assert err.value.code == 1
+    for lab in (61, 62, 63):
+        qcm.erase_model_instance(lab)

@@ -373,3 +373,27 @@ def test_concurrent_callers_get_the_serial_results():
         assert np.abs(got[lab][2] - M).max() < 1e-10
     for lab in labs:
         _finish(lab)
+
+
+def test_fidelity_matches_dense_overlaps():
+    """|<psi(U=4)|psi(U=4.6)>|^2 against the dense ground states; an instance
+    with itself gives the sum of the squared weights of its states"""
+    L = 4
+    ops = _chain_ops(L)
+    a = _instance("resp_chain4", L, 0, ops, {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N4:S0")
+    b = _instance("resp_chain4", L, 0, ops, {"t": 1.0, "U": 4.6, "mu": 2.3}, "R0:N4:S0")
+    gs = []
+    for U, mu in ((4.0, 2.0), (4.6, 2.3)):
+        vals = {"t": 1.0, "U": U, "mu": mu}
+        terms = [(typ, float(vals.get(name, 0.0)), el) for name, (typ, el) in ops.items()]
+        E, V = np.linalg.eigh(dense_hamiltonian(L, terms, 2, 2))
+        assert E[1] - E[0] > 1e-3
+        gs.append(V[:, 0])
+    ref = float(gs[0] @ gs[1]) ** 2
+    assert abs(qcm.fidelity(a, b) - ref) < 1e-10, (qcm.fidelity(a, b), ref)
+    assert abs(qcm.fidelity(a, a) - 1.0) < 1e-12
+    d = _instance("resp_chain3", 3, 0, _chain_ops(3), {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N3:S1/R0:N3:S-1")
+    assert abs(qcm.fidelity(d, d) - 0.5) < 1e-12
+    assert qcm.fidelity(a, d) == 0.0
+    for x in (a, b, d):
+        _finish(x)
