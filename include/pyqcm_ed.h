@@ -228,6 +228,14 @@ int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb);
    "interaction") and its stored elements (0-based; one-body ones Hermitian-completed). */
 int ed_matrix_elements(const char *model, const char *op, char *type, int type_cap, int cap, int *r, int *c,
                        double *v, int *n);
+/* ED::density_matrix (src_ed/qcm_ED.cpp:205-216; density_matrix_mixed,
+   model_instance.hpp:1547-1622): reduced density matrix of the orbitals sites[0..nsites)
+   in the first low-energy state; *dim_a = subsystem dimension; when cap >= *dim_a, rho
+   receives dim_a x dim_a complex (column-major) and basis the subsystem states (orbital
+   sites[i] at bit i for up, 32 + i for down), in order of first appearance along the basis.
+   (For factorized Hamiltonian formats the reference returns an empty matrix; here the
+   same result is returned for every format.) */
+int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *rho, uint64_t *basis, int *dim_a);
 /* ED::fidelity (src_ed/qcm_ED.cpp:360-368; model_instance.hpp:1261-1273): sum over pairs of
    low-energy states of the two instances in the same sector of |<psi_1|psi_2>|^2 w_1 w_2. */
 int ed_fidelity(int label1, int label2, double *f);

@@ -448,6 +448,24 @@ int ed_matrix_elements(const char *model, const char *op, char *type, int type_c
   });
 }
 
+int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *rho, uint64_t *basis, int *dim_a) {
+  return guard([&] {
+    need(nsites >= 0 && (nsites == 0 || sites) && dim_a, "ed_density_matrix: bad arguments");
+    std::vector<double> r;
+    std::vector<uint64_t> b;
+    get_instance(label).density_matrix(std::vector<int>(sites, sites + nsites), r, b);
+    *dim_a = (int)b.size();
+    if ((int)b.size() <= cap) {
+      need(rho && basis, "ed_density_matrix: null output");
+      for (size_t k = 0; k < r.size(); ++k) {
+        rho[2 * k] = r[k];
+        rho[2 * k + 1] = 0.0;
+      }
+      std::memcpy(basis, b.data(), b.size() * sizeof(uint64_t));
+    }
+  });
+}
+
 int ed_fidelity(int label1, int label2, double *f) {
   return guard([&] {
     need(f != nullptr, "ed_fidelity: null output");

@@ -911,6 +911,56 @@ Instance::Hopping Instance::hopping_blocks(bool spin_down, bool diag) const {
   return H;
 }
 
+void Instance::density_matrix(const std::vector<int> &sites, std::vector<double> &rho, std::vector<uint64_t> &basis) {
+  if (!gs_solved) ground_state_solve();
+  if (states.empty() || !states[0]->psi.p)
+    fail(ED_ERR_INVALID, "the density matrix needs the ground-state vector (not read from a file)");
+  const int L = model->L;
+  uint64_t mA = 0;
+  for (int s : sites) {
+    if (s < 0 || s >= L) fail(ED_ERR_INVALID, "density_matrix: orbital " + std::to_string(s) + " out of range");
+    mA |= uint64_t(1) << s;
+  }
+  mA |= mA << 32;
+  const State &st = *states[0];
+  const auto BU = half_basis(L, st.sec.Nup()), BD = half_basis(L, st.sec.Ndw());
+  const std::vector<double> psi = st.psi.to_host();
+  // the basis in label order (i_up fastest): the words up | down << 32 ascend,
+  // as the reference's mixed basis does for a trivial group
+  std::map<uint64_t, int> ia, ib;
+  std::vector<uint64_t> binA;
+  std::vector<std::vector<std::pair<int, size_t>>> groups;  // per B: (A index, label)
+  size_t I = 0;
+  for (uint32_t d = 0; d < BD->dim; ++d)
+    for (uint32_t u = 0; u < BU->dim; ++u, ++I) {
+      const uint64_t w = (uint64_t)BU->bin[u] | ((uint64_t)BD->bin[d] << 32);
+      const uint64_t A = w & mA, B = w ^ A;
+      auto a = ia.find(A);
+      if (a == ia.end()) {
+        a = ia.emplace(A, (int)binA.size()).first;
+        binA.push_back(A);
+      }
+      auto b = ib.find(B);
+      if (b == ib.end()) {
+        b = ib.emplace(B, (int)groups.size()).first;
+        groups.emplace_back();
+      }
+      groups[b->second].push_back({a->second, I});
+    }
+  const size_t dA = binA.size();
+  rho.assign(dA * dA, 0.0);
+  for (const auto &g : groups)
+    for (const auto &x : g)
+      for (const auto &y : g) rho[x.first + (size_t)y.first * dA] += psi[x.second] * psi[y.second];
+  // collapse (src_ed/binary_state.cpp:162-179)
+  basis.assign(dA, 0);
+  for (size_t k = 0; k < dA; ++k)
+    for (size_t i = 0; i < sites.size(); ++i) {
+      if (binA[k] >> sites[i] & 1u) basis[k] |= uint64_t(1) << i;
+      if (binA[k] >> (sites[i] + 32) & 1u) basis[k] |= uint64_t(1) << (i + 32);
+    }
+}
+
 double Instance::fidelity(Instance &other) {
   if (!gs_solved) ground_state_solve();
   if (!other.gs_solved) other.ground_state_solve();

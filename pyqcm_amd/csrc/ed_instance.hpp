@@ -107,6 +107,12 @@ struct Instance {
   // sum over pairs of low-energy states in the same sector of
   // |<psi_1|psi_2>|^2 w_1 w_2 (model_instance<T>::fidelity, :1261-1273)
   double fidelity(Instance &other);
+  // reduced density matrix of the orbitals `sites` in the first low-energy
+  // state (density_matrix_mixed, src_ed/model_instance.hpp:1547-1622): rho
+  // (dA x dA column-major) over the subsystem states in order of first
+  // appearance along the basis, and those states with orbital sites[i] moved
+  // to bit i (up) and bit 32 + i (down)
+  void density_matrix(const std::vector<int> &sites, std::vector<double> &rho, std::vector<uint64_t> &basis);
 };
 
 Instance &get_instance(int label);

@@ -30,7 +30,7 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
            "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
            "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version", "ED_complex_HS",
-           "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity"]
+           "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity", "density_matrix"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -335,6 +335,21 @@ def matrix_elements(model, op):
                                    c.ctypes.data_as(C.POINTER(C.c_int)), v.ctypes.data_as(C.POINTER(C.c_double)),
                                    C.byref(n)))
     return tb.value.decode(), [(int(r[k]), int(c[k]), complex(v[k])) for k in range(n.value)]
+
+
+def density_matrix(sites, label=0):
+    """qcm.density_matrix (src_python/qcm_ED_wrap.hpp:32-43): (rho, basis) for the
+    orbitals `sites` (0-based) in the ground state; rho with the reference's flat
+    copy of the column-major matrix, basis as uint64 words."""
+    s = (C.c_int * max(len(sites), 1))(*[int(x) for x in sites])
+    d = C.c_int()
+    check(lib().ed_density_matrix(int(label), s, len(sites), 0, None, None, C.byref(d)))
+    n = d.value
+    rho = np.zeros(max(n * n, 1), dtype=np.complex128)
+    basis = np.zeros(max(n, 1), dtype=np.uint64)
+    check(lib().ed_density_matrix(int(label), s, len(sites), n, rho.ctypes.data_as(C.POINTER(C.c_double)),
+                                  basis.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(d)))
+    return rho[:n * n].reshape(n, n), basis[:n]
 
 
 def fidelity(label1, label2):

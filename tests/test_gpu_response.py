@@ -397,3 +397,44 @@ def test_fidelity_matches_dense_overlaps():
     assert qcm.fidelity(a, d) == 0.0
     for x in (a, b, d):
         _finish(x)
+
+
+def test_density_matrix():
+    """reduced density matrices of the 4-site chain ground state: one site
+    (diagonal, the occupation probabilities of the dense ground state), all
+    sites (|psi><psi| in label order), and two complementary halves (the same
+    nonzero spectrum, as for any pure state)"""
+    L = 4
+    ops = _chain_ops(L)
+    vals = {"t": 1.0, "U": 4.0, "mu": 2.0}
+    lab = _instance("resp_chain4", L, 0, ops, vals, "R0:N4:S0")
+    terms = [(typ, float(vals.get(name, 0.0)), el) for name, (typ, el) in ops.items()]
+    E, V = np.linalg.eigh(dense_hamiltonian(L, terms, 2, 2))
+    psi = V[:, 0]
+    from tests.dense_ref import sector_states
+    words = sector_states(L, 2, 2)  # label order, word = up | down << L
+    rho, basis = qcm.density_matrix([0], lab)
+    assert sorted(int(b) for b in basis) == [0, 1, 1 << 32, 1 + (1 << 32)]
+    p = {0: 0.0, 1: 0.0, 1 << 32: 0.0, 1 + (1 << 32): 0.0}
+    for k, w in enumerate(words):
+        key = (w & 1) | (((w >> L) & 1) << 32)
+        p[key] += psi[k] ** 2
+    assert np.abs(rho - np.diag(np.diag(rho))).max() < 1e-12
+    for i, b in enumerate(basis):
+        assert abs(rho[i, i].real - p[int(b)]) < 1e-10
+    rho_all, basis_all = qcm.density_matrix(list(range(L)), lab)
+    assert len(basis_all) == len(words)
+    # all orbitals: the subsystem states are the basis states in label order
+    assert [int(b) for b in basis_all] == [(w & 0xF) | ((w >> L) << 32) for w in words]
+    assert np.abs(rho_all.real - np.outer(psi, psi)).max() < 1e-10
+    ra, _ = qcm.density_matrix([0, 1], lab)
+    rb, _ = qcm.density_matrix([2, 3], lab)
+    for r in (ra, rb):
+        assert np.abs(r - r.T.conj()).max() < 1e-12 and abs(np.trace(r).real - 1.0) < 1e-12
+    ea = np.sort(np.linalg.eigvalsh(ra.real))[::-1]
+    eb = np.sort(np.linalg.eigvalsh(rb.real))[::-1]
+    k = min(len(ea), len(eb))
+    assert np.abs(ea[:k] - eb[:k]).max() < 1e-10 and np.abs(ea[k:]).max(initial=0.0) < 1e-10
+    with pytest.raises(EDError):
+        qcm.density_matrix([7], lab)
+    _finish(lab)
