@@ -182,3 +182,25 @@ def test_lattice_sums_reject_bad_input():
         qcm.projected_Green_function([0.5j], LT.chain_superlattice(9), (1, (4,), 0.5), label=big)
     assert e.value.code == 8
     qcm.erase_model_instance(big)
+
+
+@pytest.mark.parametrize("cdw,omega_ref,e0_ref", [(1e-9, -2.5525862, -9.9531453), (0.01, -2.5526019, -9.9531709),
+                                                   (-0.01, -2.5526019, -9.9531709), (0.0, -2.5525862, -9.9531453)])
+def test_potthoff_functional_reproduces_the_reference_1D_4_example(cdw, omega_ref, e0_ref):
+    """The reference's own output (examples/1D_4/sef.tsv, first block): the
+    Hubbard chain (t = 1, U = 4, mu = 2) tiled by 4-site clusters carrying a
+    charge-density-wave Weiss field cdw_1 (site energies cdw_1 (-1)^i, both
+    spins), sector N = 4, S = 0; columns omega and E0_1, printed to 8 digits.
+    The other operators of those rows (V, V0m, V1m at 1e-9) change omega by
+    less than the last printed digit and are left out."""
+    L = 4
+    ops = dict(W.hubbard_chain(L, U=4.0, mu=2.0).operators)
+    stag = np.array([(-1.0) ** i for i in range(L)])
+    ops["cdw"] = ("one-body", [(i, i, stag[i]) for i in range(L)] + [(i + L, i + L, stag[i]) for i in range(L)])
+    lab = _instance("lat_chain4_cdw", L, 0, ops, {"t": 1.0, "U": 4.0, "mu": 2.0, "cdw": cdw}, "R0:N4:S0")
+    E0 = qcm.ground_state_solve(lab)[0]
+    assert abs(E0 - e0_ref) < 1e-7, (E0, e0_ref)
+    w, p = LT.legendre_frequency_grid(1.0, 20.0, 60, 60, 60)
+    omega = LT.potthoff_functional(lab, w, p, LT.chain_superlattice(L, 1.0, weiss=cdw * stag), (1, (128,), 0.0))
+    assert abs(omega - omega_ref) < 2e-7, (cdw, omega, omega_ref)
+    qcm.erase_model_instance(lab)
