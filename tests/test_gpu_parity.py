@@ -452,3 +452,41 @@ def test_krylov_vector_kernel_shapes(vector_block, blocks_per_cu):
     assert abs(E0 - Ed) <= 1e-10 * abs(Ed)
     assert abs(E0 - E_ref) <= 1e-12 * abs(Ed)
     assert np.abs(G - G_ref).max() < 1e-10 * np.abs(G_ref).max()
+
+
+def test_bath_cluster_1D_2_4b_known_answer():
+    """examples/1D_2_4b/cdmft.tsv:3 (the converged CDMFT bath of the reference's
+    own run): 2 cluster sites + 4 bath orbitals (model_1D_2_4b.py), U = 4,
+    mu = 2, t = 1, eb1 = 1.1438241, eb2 = -1.1438239, tb1 = 0.60939833,
+    tb2 = 0.60939828, sector N = 6, S = 0.  E0_1 and the averages / variances
+    of every operator (lattice operators t, U, mu scaled by 1/2, 1/2, -1/2 as
+    the lattice layer does on a 2-site cluster; bath operators as defined),
+    printed to 8 digits."""
+    ns, nb = 2, 4
+    L = ns + nb
+    one = {  # 0-based spin-up elements; the down ones are shifted by L
+        "t": [(0, 1, -1.0)], "mu": [(0, 0, -1.0), (1, 1, -1.0)],
+        "tb1": [(0, 2, -1.0), (1, 3, -1.0)], "tb2": [(0, 4, -1.0), (1, 5, -1.0)],
+        "eb1": [(2, 2, 1.0), (3, 3, 1.0)], "eb2": [(4, 4, 1.0), (5, 5, 1.0)],
+    }
+    name = "L2_4b"
+    qcm.new_model(name, ns, nb)
+    for op, el in one.items():
+        qcm.new_operator(name, op, "one-body", [(r + 1, c + 1, v) for (r, c, v) in el + [(r + L, c + L, v) for (r, c, v) in el]])
+    qcm.new_operator(name, "U", "interaction", [(i + 1, i + L + 1, 1.0) for i in range(ns)])
+    vals = {"U": 4.0, "mu": 2.0, "t": 1.0, "eb1": 1.1438241, "eb2": -1.1438239, "tb1": 0.60939833, "tb2": 0.60939828}
+    lab = _new_label()
+    qcm.new_model_instanceC(name, vals, "R0:N6:S0", lab)
+    E0, sec = qcm.ground_state_solve(lab)
+    assert abs(E0 - (-9.8979868)) < 2e-7, E0
+    ave = qcm.cluster_averages(lab)
+    norm = {"t": 0.5, "U": 0.5, "mu": -0.5}
+    ref = {"U": (0.10367723, 0.0416604), "eb1": (0.099838804, 0.094703557), "eb2": (3.9001612, 0.094703556),
+           "mu": (1.0, 0.038057271), "t": (-0.67470015, 0.41831011), "tb1": (-0.84600224, 1.7731342),
+           "tb2": (-0.84600223, 1.7731342)}
+    for op, (a, v) in ref.items():
+        f = norm.get(op, 1.0)
+        got_a, got_v = ave[op]
+        assert abs(got_a * f - a) < 1e-6, (op, got_a * f, a)
+        assert abs(got_v * f * f - v) < 1e-6, (op, got_v * f * f, v)
+    qcm.erase_model_instance(lab)
