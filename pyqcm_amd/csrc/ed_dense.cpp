@@ -21,19 +21,20 @@ namespace ed {
 // number of eigenvalues < sigma of the symmetric band matrix A (column-major
 // n x n, half-bandwidth b): inertia of LDL^T of A - sigma I.
 static int band_count_below(int n, int b, const double *A, double sigma, std::vector<double> &Lw,
-                            std::vector<double> &D, double eps_piv) {
+                            std::vector<double> &D, double eps_piv, int lda = -1) {
+  if (lda < 0) lda = n;
   // L stored as n x (b) : Lw[i * b + (i - k - 1)] = l_{i k} for k in [i-b, i-1]
   int neg = 0;
   for (int i = 0; i < n; ++i) {
     const int k0 = std::max(0, i - b);
     // l_{i k} for k in [k0, i-1]
     for (int k = k0; k < i; ++k) {
-      double s = A[i + (size_t)k * n];
+      double s = A[i + (size_t)k * lda];
       const int m0 = std::max(k0, k - b);
       for (int m = m0; m < k; ++m) s -= Lw[(size_t)i * b + (i - m - 1)] * Lw[(size_t)k * b + (k - m - 1)] * D[m];
       Lw[(size_t)i * b + (i - k - 1)] = s / D[k];
     }
-    double d = A[i + (size_t)i * n] - sigma;
+    double d = A[i + (size_t)i * lda] - sigma;
     for (int k = k0; k < i; ++k) {
       const double l = Lw[(size_t)i * b + (i - k - 1)];
       d -= l * l * D[k];
@@ -67,6 +68,52 @@ double band_kth_eigenvalue(int n, int b, const double *A, int k) {
     if (band_count_below(n, b, A, mid, Lw, D, eps_piv) >= k + 1) hi = mid;
     else lo = mid;
     if (hi - lo <= 2.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(lo), std::fabs(hi))) break;
+  }
+  return 0.5 * (lo + hi);
+}
+
+double band_lowest_eigenvalue(int n, int b, const double *A, int lda, double hint, double width) {
+  if (b < 1) b = 1;
+  if (b > n - 1) b = std::max(1, n - 1);
+  double glo = std::numeric_limits<double>::max(), ghi = -glo, scale = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r = 0.0;
+    for (int j = std::max(0, i - b); j <= std::min(n - 1, i + b); ++j)
+      if (j != i) r += std::fabs(A[i + (size_t)j * lda]);
+    const double a = A[i + (size_t)i * lda];
+    glo = std::min(glo, a - r);
+    ghi = std::max(ghi, a + r);
+    scale = std::max(scale, std::fabs(a) + r);
+  }
+  std::vector<double> Lw((size_t)n * b, 0.0), D(n, 0.0);
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double eps_piv = 1e-300 + 4.0 * eps * scale * 1e-3;
+  auto count = [&](double s) { return band_count_below(n, b, A, s, Lw, D, eps_piv, lda); };
+  double lo = glo, hi = ghi;
+  if (std::isfinite(hint) && hint > glo && hint < ghi) {
+    // upper end: the hint, nudged up, when it bounds the lowest eigenvalue
+    const double h = hint + 8.0 * eps * scale;
+    if (count(h) >= 1) {
+      hi = h;
+      double w = std::max(width, 1e3 * eps * scale);
+      while (true) {
+        const double l = hi - w;
+        if (l <= glo) break;
+        if (count(l) == 0) {
+          lo = l;
+          break;
+        }
+        hi = l;  // the eigenvalue lies below l: a tighter upper end
+        w *= 16.0;
+      }
+    }
+  }
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (count(mid) >= 1) hi = mid;
+    else lo = mid;
+    if (hi - lo <= 2.0 * eps * std::max(std::fabs(lo), std::fabs(hi))) break;
   }
   return 0.5 * (lo + hi);
 }

@@ -13,6 +13,15 @@ void ql_rows(int n, double *d, double *e, double *Z, int nz);
 // (column-major n x n, half-bandwidth b), by bisection on LDL^T inertia.
 double band_kth_eigenvalue(int n, int b, const double *A, int k);
 
+// lowest eigenvalue of the leading n x n block of a symmetric band matrix held
+// column-major with leading dimension lda (half-bandwidth b), by the same
+// bisection.  hint: the lowest eigenvalue of a leading block of it (an upper
+// bound, by Cauchy interlacing), or +inf; width: how far below hint the value
+// is expected.  The bracket [hint - width, hint] is checked by inertia counts
+// and widened until it holds the eigenvalue; the result is the same as from
+// the full Gershgorin bracket, to the bisection's final width.
+double band_lowest_eigenvalue(int n, int b, const double *A, int lda, double hint, double width);
+
 // all eigenvalues (ascending) of a symmetric band matrix and the first p
 // components of every eigenvector: R is p x n row-major, R[k*n + c] = U[k, c].
 void band_eig_first_rows(int n, int b, const double *A, int p, std::vector<double> &w, std::vector<double> &R);

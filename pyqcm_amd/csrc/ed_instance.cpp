@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <iomanip>
 #include <mutex>
 #include <random>
@@ -433,6 +434,23 @@ static void build_gf(Instance &I, State &st, int spin) {
   const int np = ns * (ns + 1) / 2;
   if (G.kind == 'F') { G.ea.assign(np, {}); G.eb.assign(np, {}); G.ha.assign(np, {}); G.hb.assign(np, {}); }
   std::vector<double> eh, Vh, ee, Ve;  // Lehmann parts (hole, electron)
+  // poles and residues of one part from its band-Lanczos run (host only):
+  // shifted by -E, negated for holes, streamlined
+  using Part = std::pair<std::vector<double>, std::vector<double>>;
+  const double E = st.energy;
+  auto lehmann = [E, ns](const BandLanczosRun &r, int pm) -> Part {
+    Part out;
+    r.finish(out.first, out.second);
+    vlog("band Lanczos: %zu poles", out.first.size());
+    for (auto &x : out.first) x -= E;
+    if (pm == -1)
+      for (auto &x : out.first) x = -x;
+    streamline(ns, out.first, out.second);
+    return out;
+  };
+  // the hole part's eigen-decomposition runs on a host thread while the
+  // device works on the electron part
+  std::future<Part> hole_part;
   for (int pm = -1; pm <= 1; pm += 2) {
     check_cancel();
     Sector tgt = shift_sector(st.sec, pm, spin);
@@ -452,16 +470,15 @@ static void build_gf(Instance &I, State &st, int spin) {
     if (G.kind == 'L') {
       std::vector<const double *> ph;
       for (auto &b : phi) ph.push_back(b.p);
-      std::vector<double> e, Q;
       vlog("GF %s (pm %+d, spin %d): dim %zu, %d start vectors ready", tgt.name().c_str(), pm, spin, dt, ns);
-      band_lanczos(*H, ph, e, Q);
-      vlog("band Lanczos: %zu poles", e.size());
-      for (auto &x : e) x -= st.energy;
-      if (pm == -1)
-        for (auto &x : e) x = -x;
-      streamline(ns, e, Q);
-      if (pm == -1) { eh = e; Vh = Q; }
-      else { ee = e; Ve = Q; }
+      BandLanczosRun run = band_lanczos_run(*H, ph);
+      if (pm == -1) {
+        hole_part = std::async(std::launch::async, [lehmann, r = std::move(run)]() { return lehmann(r, -1); });
+      } else {
+        Part el = lehmann(run, 1);
+        ee.swap(el.first);
+        Ve.swap(el.second);
+      }
     } else if (G.kind == 'F') {
       // build_cf (src_ed/model_instance.cpp:35-100)
       DevBuf<double> psi(dt);
@@ -531,6 +548,11 @@ static void build_gf(Instance &I, State &st, int spin) {
     } else {
       fail(ED_ERR_INVALID, std::string("GF_method must be L, F or M (got ") + G.kind + ")");
     }
+  }
+  if (hole_part.valid()) {
+    Part h = hole_part.get();
+    eh.swap(h.first);
+    Vh.swap(h.second);
   }
   if (G.kind == 'L') {
     G.e = eh;

@@ -243,8 +243,7 @@ void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, 
 // ---------------------------------------------------------------------------
 // band Lanczos (Freund), restated from src_ed/Hamiltonian/Lanczos.hpp:214-395
 // ---------------------------------------------------------------------------
-void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::vector<double> &evals,
-                  std::vector<double> &Q, int M_in) {
+BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *> &phi, int M_in) {
   Params &Pm = params();
   const size_t dim = H.dim;
   const int p = (int)phi.size();
@@ -253,9 +252,7 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
   const int max_iter_bl = (int)Pm.geti("max_iter_BL");
   const double min_gap = Pm.getd("band_lanczos_minimum_gap");
   const bool no_deg = Pm.getb("no_degenerate_BL");
-  evals.clear();
-  Q.clear();
-  if (dim == 0 || p == 0) return;
+  if (dim == 0 || p == 0) return BandLanczosRun{};
   int M0 = (M_in < 0) ? (int)(14 * p * std::log(1.0 * dim)) : M_in;
   int pc = p;
   if (M0 > max_iter_bl) M0 = max_iter_bl;
@@ -270,7 +267,7 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     d2d(v[i]->p, phi[i], dim);
   }
   int nd = 0, j;
-  double ev_old = 1e12;
+  double ev_old = 1e12, ev_change = 0.0;
   for (j = 0; j < M0; ++j) {
     check_cancel();
     // one multi-dot per step: ||v_j||^2, <v_j, phi_k> (P matrix), the step (6)
@@ -372,20 +369,29 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
     for (int k = 0; k < nd; ++k) T(j, Idef[k]) = T(Idef[k], j);
     // step (12): convergence of the lowest eigenvalue of the projected matrix
     if (j >= 4 * p && j % p == 0) {
-      std::vector<double> Hp((size_t)j * j);
-      for (int c = 0; c < j; ++c)
-        for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
       double e0, e1 = 0.0;
-      if (nd == 0) {
+      if (nd == 0 && !no_deg) {
+        // the leading j x j block of t in place; the previous check's value
+        // bounds this one from above (interlacing) and the last change sizes
+        // the bracket below it
+        e0 = band_lowest_eigenvalue(j, p, t.data(), M0, ev_old < 1e11 ? ev_old : HUGE_VAL, 2.0 * ev_change);
+      } else if (nd == 0) {
+        std::vector<double> Hp((size_t)j * j);
+        for (int c = 0; c < j; ++c)
+          for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
         e0 = band_kth_eigenvalue(j, p, Hp.data(), 0);
-        if (no_deg) e1 = band_kth_eigenvalue(j, p, Hp.data(), 1);
+        e1 = band_kth_eigenvalue(j, p, Hp.data(), 1);
       } else {
+        std::vector<double> Hp((size_t)j * j);
+        for (int c = 0; c < j; ++c)
+          for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
         std::vector<double> ev(j);
         symeig(j, Hp.data(), ev.data(), nullptr);
         e0 = ev[0];
         e1 = j > 1 ? ev[1] : ev[0];
       }
       const double test = std::fabs(e0 - ev_old);
+      if (ev_old < 1e11) ev_change = test;
       ev_old = e0;
       if (no_deg && e1 - e0 < min_gap)
         fail(ED_ERR_DEGENERATE, "band Lanczos: the gap between the first two eigenvalues is smaller than " +
@@ -395,32 +401,51 @@ void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::v
   }
   const int M = j;
   vlog("band Lanczos: %d steps, %d deflations, dim %zu", M, nd, dim);
-  if (M == 0) return;
-  std::vector<double> Hp((size_t)M * M);
+  BandLanczosRun out;
+  out.M = M;
+  out.p = p;
+  out.nd = nd;
+  if (M == 0) return out;
+  out.T.resize((size_t)M * M);
   for (int c = 0; c < M; ++c)
-    for (int r = 0; r < M; ++r) Hp[(size_t)r + (size_t)c * M] = T(r, c);
+    for (int r = 0; r < M; ++r) out.T[(size_t)r + (size_t)c * M] = T(r, c);
+  out.P.assign(Pmat.begin(), Pmat.begin() + (size_t)p * std::min(p, M));
+  return out;
+}
+
+// build_Q_matrix (Hamiltonian_base.hpp:230-254): eigenvalues of the projected
+// matrix and Q = P U restricted to the first p rows of U
+void BandLanczosRun::finish(std::vector<double> &evals, std::vector<double> &Q) const {
+  evals.clear();
+  Q.clear();
+  if (M == 0) return;
   const int kmax = std::min(p, M);
   Q.assign((size_t)p * M, 0.0);
   if (nd == 0) {
     std::vector<double> Rr;
-    band_eig_first_rows(M, p, Hp.data(), kmax, evals, Rr);
+    band_eig_first_rows(M, p, T.data(), kmax, evals, Rr);
     for (int c = 0; c < M; ++c)
       for (int i = 0; i < p; ++i) {
         double s = 0.0;
-        for (int k = 0; k < kmax; ++k) s += Pmat[(size_t)i + (size_t)k * p] * Rr[(size_t)k * M + c];
+        for (int k = 0; k < kmax; ++k) s += P[(size_t)i + (size_t)k * p] * Rr[(size_t)k * M + c];
         Q[(size_t)i + (size_t)c * p] = s;
       }
   } else {
     evals.resize(M);
     std::vector<double> U((size_t)M * M);
-    symeig(M, Hp.data(), evals.data(), U.data());
+    symeig(M, T.data(), evals.data(), U.data());
     for (int c = 0; c < M; ++c)
       for (int i = 0; i < p; ++i) {
         double s = 0.0;
-        for (int k = 0; k < kmax; ++k) s += Pmat[(size_t)i + (size_t)k * p] * U[(size_t)k + (size_t)c * M];
+        for (int k = 0; k < kmax; ++k) s += P[(size_t)i + (size_t)k * p] * U[(size_t)k + (size_t)c * M];
         Q[(size_t)i + (size_t)c * p] = s;
       }
   }
+}
+
+void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::vector<double> &evals,
+                  std::vector<double> &Q, int M_in) {
+  band_lanczos_run(H, phi, M_in).finish(evals, Q);
 }
 
 // ---------------------------------------------------------------------------

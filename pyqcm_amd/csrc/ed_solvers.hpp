@@ -25,6 +25,15 @@ void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, 
 // Q (p x M column-major) = P[:, :p] U[:p, :].
 void band_lanczos(Hamiltonian &H, const std::vector<const double *> &phi, std::vector<double> &evals,
                   std::vector<double> &Q, int M_in = -1);
+// the same in two parts: the Krylov steps on the device, and the host-only
+// eigen-decomposition of the projected matrix, which may run on another
+// thread while the device works on something else
+struct BandLanczosRun {
+  int M = 0, p = 0, nd = 0;
+  std::vector<double> T, P;  // projected matrix (M x M), P (p x min(p, M)), column-major
+  void finish(std::vector<double> &evals, std::vector<double> &Q) const;
+};
+BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *> &phi, int M_in = -1);
 
 // Davidson-Liu (src_ed/Hamiltonian/Davidson.hpp:38-230); evecs: K device buffers of dim.
 int davidson(Hamiltonian &H, int K, double accuracy, std::vector<double> &evals, std::vector<DevBuf<double>> &evecs);
