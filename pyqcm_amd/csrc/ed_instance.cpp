@@ -991,6 +991,8 @@ double Instance::fidelity(Instance &other) {
     for (auto &s2 : other.states) {
       if (!(s1->sec == s2->sec)) continue;
       if (!s1->psi.p || !s2->psi.p) fail(ED_ERR_INVALID, "fidelity needs the ground-state vectors (not read from a file)");
+      if (s1->psi.n != s2->psi.n || model->L != other.model->L)
+        fail(ED_ERR_INVALID, "fidelity: the two instances belong to models of different sizes");
       const double zz = ddot(s1->psi.n, s1->psi.p, s2->psi.p);
       z += zz * zz * s1->weight * s2->weight;
     }

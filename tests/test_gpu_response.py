@@ -395,7 +395,12 @@ def test_fidelity_matches_dense_overlaps():
     d = _instance("resp_chain3", 3, 0, _chain_ops(3), {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N3:S1/R0:N3:S-1")
     assert abs(qcm.fidelity(d, d) - 0.5) < 1e-12
     assert qcm.fidelity(a, d) == 0.0
-    for x in (a, b, d):
+    # same sector name, different model size: refused (the vectors differ in length)
+    f = _instance("resp_chain5", 5, 0, _chain_ops(5), {"t": 1.0, "U": 4.0, "mu": 2.0}, "R0:N4:S0")
+    with pytest.raises(EDError) as e:
+        qcm.fidelity(a, f)
+    assert e.value.code == 1
+    for x in (a, b, d, f):
         _finish(x)
 
 
