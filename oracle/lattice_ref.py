@@ -1,7 +1,7 @@
 """numpy restatement of the lattice sums built on a cluster Green function.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ and the cpu_baseline leg of
-bench.py, as the checker / CPU baseline.  The product package (pyqcm_amd)
+bench.py, for reference values / the CPU baseline.  The product package (pyqcm_amd)
 never imports this module.
 
 Follows:

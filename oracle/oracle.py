@@ -1,7 +1,7 @@
 """ctypes front-end of the CPU oracle (oracle_ed.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker / CPU baseline.  The product
+cpu_baseline leg of bench.py, for reference values / the CPU baseline.  The product
 package (pyqcm_amd) never imports this module.
 
 A model is given as the list of terms the reference's ``ED::new_operator``

@@ -1,7 +1,7 @@
 /*
  * oracle_ed.c -- CPU FP64 restatement of pyqcm's exact-diagonalization path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline).  The product
+ * TEST INFRASTRUCTURE ONLY (reference values for the tests, and the CPU baseline).  The product
  * (pyqcm_amd/) never links, loads or calls anything here.
  *
  * Written from the reference's behaviour; each function names the

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing under oracle/ is part of the product:
  * only tests/, __graft_entry__.smoke() and bench.py (its cpu_baseline leg)
- * may load this library, and only as the checker / CPU baseline.
+ * may load this library, and only for reference values / the CPU baseline.
  *
  * Every function cites the reference file:line whose behaviour it restates.
  * Conventions (reference src_ed/binary_state.hpp:60-63, src_ed/ED_basis.cpp:142-179):

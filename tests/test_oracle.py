@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle_ed.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle_ed.c) before its values are trusted in the comparisons.
 
 Each check is chosen so that a plausible mistake (dropped term, wrong sign or
 index, transposed operand) fails: textbook Jordan-Wigner dense matrices,
