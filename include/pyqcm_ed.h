@@ -236,6 +236,11 @@ int ed_matrix_elements(const char *model, const char *op, char *type, int type_c
    (For factorized Hamiltonian formats the reference returns an empty matrix; here the
    same result is returned for every format.) */
 int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *rho, uint64_t *basis, int *dim_a);
+/* ED::print_wavefunction (src_ed/qcm_ED.cpp:431-440): the low-energy states as text
+   (per state "state", then sector, energy and weight; for sectors of at most
+   max_dim_print states one line per basis state: |psi|^2, psi, |up bits,down bits>).
+   *len = length; the text and its terminating 0 are written when cap > *len. */
+int ed_print_wavefunction(int label, char *buf, int cap, int *len);
 /* ED::fidelity (src_ed/qcm_ED.cpp:360-368; model_instance.hpp:1261-1273): sum over pairs of
    low-energy states of the two instances in the same sector of |<psi_1|psi_2>|^2 w_1 w_2. */
 int ed_fidelity(int label1, int label2, double *f);

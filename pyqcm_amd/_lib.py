@@ -85,6 +85,7 @@ SIGNATURES = {
     "ed_hybridization_lehmann": (C.c_int, [C.c_int, C.c_int, _dp, _dp]),
     "ed_matrix_elements": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, _ip, _ip, _dp, _ip]),
     "ed_fidelity": (C.c_int, [C.c_int, C.c_int, _dp]),
+    "ed_print_wavefunction": (C.c_int, [C.c_int, C.c_char_p, C.c_int, _ip]),
     "ed_density_matrix": (C.c_int, [C.c_int, _ip, C.c_int, C.c_int, _dp, C.POINTER(C.c_uint64), _ip]),
     "ed_write_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),
     "ed_read_instance_hdf5": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p]),

@@ -466,6 +466,15 @@ int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *
   });
 }
 
+int ed_print_wavefunction(int label, char *buf, int cap, int *len) {
+  return guard([&] {
+    need(len != nullptr && (cap == 0 || buf), "ed_print_wavefunction: bad arguments");
+    const std::string s = get_instance(label).print_wavefunction();
+    *len = (int)s.size();
+    if ((int)s.size() < cap) std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
 int ed_fidelity(int label1, int label2, double *f) {
   return guard([&] {
     need(f != nullptr, "ed_fidelity: null output");

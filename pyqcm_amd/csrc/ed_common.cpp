@@ -36,6 +36,7 @@ Params::Params() {
   d["temperature"] = 0.0;
   i["n_states"] = 1;
   i["max_dim_full"] = 256;
+  i["max_dim_print"] = 64;
   i["max_iter_BL"] = 600;
   i["max_iter_CF"] = 400;
   i["max_iter_lanczos"] = 1000;

@@ -983,6 +983,42 @@ void Instance::density_matrix(const std::vector<int> &sites, std::vector<double>
     }
 }
 
+// PrintBinaryDouble / BinaryToString (src_ed/binary_state.cpp:81-113): the low
+// 32 bits of w, then the high 32 bits, m characters each, least significant first
+static void print_binary_double(std::ostream &o, uint64_t w, int m) {
+  o << '|';
+  for (int i = 0; i < m; ++i) o << ((w >> i) & 1u ? '1' : '0');
+  o << ',';
+  for (int i = 0; i < m; ++i) o << ((w >> (32 + i)) & 1u ? '1' : '0');
+  o << '>';
+}
+
+std::string Instance::print_wavefunction() {
+  if (!gs_solved) ground_state_solve();
+  const int L = model->L;
+  const long max_print = params().geti("max_dim_print");
+  std::ostringstream o;
+  for (auto &s : states) {
+    o << "state\n" << s->sec.name() << '\t' << s->energy << '\t' << s->weight << std::endl;
+    const auto BU = half_basis(L, s->sec.Nup()), BD = half_basis(L, s->sec.Ndw());
+    const size_t dim = (size_t)BU->dim * BD->dim;
+    if ((long)dim > max_print) continue;
+    if (!s->psi.p) fail(ED_ERR_INVALID, "instance " + std::to_string(label) + " holds no ground-state vector (read from a file)");
+    const std::vector<double> psi = s->psi.to_host();
+    for (size_t I = 0; I < dim; ++I) {
+      const uint64_t up = BU->bin[I % BU->dim], dn = BD->bin[I / BU->dim];
+      // the factorized basis ('F') prints its words with the down spins low
+      // (ED_factorized_basis::bin, src_ed/ED_basis.cpp:157-160); the other
+      // formats use the mixed basis, up spins low
+      const uint64_t w = format == 'F' ? (dn | (up << 32)) : (up | (dn << 32));
+      o << std::fabs(psi[I]) * std::fabs(psi[I]) << '\t' << psi[I] << '\t';
+      print_binary_double(o, w, L);
+      o << '\n';
+    }
+  }
+  return o.str();
+}
+
 double Instance::fidelity(Instance &other) {
   if (!gs_solved) ground_state_solve();
   if (!other.gs_solved) other.ground_state_solve();

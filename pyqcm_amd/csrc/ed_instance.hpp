@@ -113,6 +113,10 @@ struct Instance {
   // appearance along the basis, and those states with orbital sites[i] moved
   // to bit i (up) and bit 32 + i (down)
   void density_matrix(const std::vector<int> &sites, std::vector<double> &rho, std::vector<uint64_t> &basis);
+  // print_wavefunction (src_ed/model_instance.hpp:1473-1486, state<T>::write_wavefunction,
+  // src_ed/state.hpp:121-131): per state a header line, then, for sectors of at
+  // most max_dim_print states, |psi_i|^2, psi_i and the basis state
+  std::string print_wavefunction();
 };
 
 Instance &get_instance(int label);

@@ -30,7 +30,8 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "erase_model_instance", "Hamiltonian", "half_basis", "sector_parse", "states", "solve_batch", "Green_function_batch",
            "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
            "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version", "ED_complex_HS",
-           "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity", "density_matrix"]
+           "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity", "density_matrix",
+           "print_wavefunction"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -47,7 +48,7 @@ _PARAM_TYPES = {
     "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
     "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
     "Qmatrix_vtol": _DOUBLE, "temperature": _DOUBLE, "lanczos_basis_budget_GB": _DOUBLE,
-    "n_states": _INT, "max_dim_full": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
+    "n_states": _INT, "max_dim_full": _INT, "max_dim_print": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
     "sell_sort_window": _INT, "hv_block": _INT, "hv_row_order": _INT, "vector_block": _INT, "reduction_blocks_per_cu": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
@@ -350,6 +351,16 @@ def density_matrix(sites, label=0):
     check(lib().ed_density_matrix(int(label), s, len(sites), n, rho.ctypes.data_as(C.POINTER(C.c_double)),
                                   basis.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(d)))
     return rho[:n * n].reshape(n, n), basis[:n]
+
+
+def print_wavefunction(label=0):
+    """qcm.print_wavefunction (src_python/qcm_ED_wrap.hpp:335-337): the
+    ground-state wavefunction(s) as a string."""
+    n = C.c_int()
+    check(lib().ed_print_wavefunction(int(label), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib().ed_print_wavefunction(int(label), buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
 
 
 def fidelity(label1, label2):

@@ -443,3 +443,37 @@ def test_density_matrix():
     with pytest.raises(EDError):
         qcm.density_matrix([7], lab)
     _finish(lab)
+
+
+def test_print_wavefunction():
+    """the text form of the ground state: header (sector, energy, weight),
+    then, for sectors up to max_dim_print states, |psi|^2, psi and the basis
+    state |up bits,down bits> (least significant orbital first) in label order;
+    the probabilities are those of the dense ground state (6 printed digits)"""
+    L = 4
+    ops = _chain_ops(L)
+    vals = {"t": 1.0, "U": 4.0, "mu": 2.0}
+    lab = _instance("resp_chain4", L, 0, ops, vals, "R0:N4:S0")
+    txt = qcm.print_wavefunction(lab)
+    lines = txt.rstrip("\n").split("\n")
+    assert lines[0] == "state"
+    sec, e, w = lines[1].split("\t")
+    assert sec == "R0:N4:S0" and abs(float(e) - (-9.9531453)) < 1e-4 and float(w) == 1.0
+    from tests.dense_ref import sector_states
+    words = sector_states(L, 2, 2)
+    assert len(lines) == 2 + len(words)
+    terms = [(typ, float(vals.get(name, 0.0)), el) for name, (typ, el) in ops.items()]
+    _, V = np.linalg.eigh(dense_hamiltonian(L, terms, 2, 2))
+    for k, (line, word) in enumerate(zip(lines[2:], words)):
+        p2, amp, ket = line.split("\t")
+        up, dn = word & 0xF, word >> L
+        assert ket == "|%s,%s>" % ("".join("1" if up >> i & 1 else "0" for i in range(L)),
+                                   "".join("1" if dn >> i & 1 else "0" for i in range(L)))
+        assert abs(float(p2) - V[k, 0] ** 2) < 1e-5 * max(1.0, V[k, 0] ** 2) + 1e-7
+        assert abs(abs(float(amp)) - abs(V[k, 0])) < 1e-5
+    qcm.set_global_parameterC("max_dim_print", 10)
+    try:
+        assert qcm.print_wavefunction(lab).rstrip("\n").split("\n") == lines[:2]
+    finally:
+        qcm.set_global_parameterC("max_dim_print", 64)
+    _finish(lab)
