@@ -92,7 +92,7 @@ int ed_hybridization_function(int label, int nz, const double *z, int spin_down,
 /* ED::Potthoff_functional contribution E0 (src_ed/qcm_ED.cpp:330-335, bath part excluded) */
 int ed_ground_state_energy(int label, double *E0);
 /* ED::cluster_averages (src_ed/qcm_ED.cpp:192-204): for each instance operator
-   (same order as ed_instance_values), the ground-state average and variance. */
+   (same order as ed_instance_parameters), the ground-state average and variance. */
 int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n);
 /* ED::qmatrix (src_python/qcm_ED_wrap.hpp:339-349): the Lehmann representation of the
    first state's Green function (band-Lanczos method); e[M], v[L x M] column-major. */
@@ -208,6 +208,10 @@ int ed_sef_grid(int label, int nw, const double *w, const double *weights, const
 int ed_potthoff_functional(int label, double *omega);
 
 /* ---- one-body data, operators, overlaps ---- */
+/* parametersC (src_python/qcm_ED_wrap.hpp:263-273): the instance's parameters in name
+   order (names[k * name_len], values[k], k < min(*n, cap)) and its model's name. */
+int ed_instance_parameters(int label, int cap, int name_len, char *names, double *values, int *n, char *model,
+                           int model_cap);
 /* ED::complex_HS (src_ed/qcm_ED.hpp:94): always 0 here (real operators only) */
 int ed_complex_hs(int label, int *is_complex);
 /* ED::hopping_matrix / hopping_matrix_full (src_ed/qcm_ED.cpp:269-284;

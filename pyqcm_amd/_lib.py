@@ -80,6 +80,7 @@ SIGNATURES = {
     "ed_sef_grid": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _slp, _slp, _kgp, _dp, _dp]),
     "ed_potthoff_functional": (C.c_int, [C.c_int, _dp]),
     "ed_complex_hs": (C.c_int, [C.c_int, _ip]),
+    "ed_instance_parameters": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_char_p, _dp, _ip, C.c_char_p, C.c_int]),
     "ed_hopping_matrix": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "ed_interactions": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _dp, _ip]),
     "ed_hybridization_lehmann": (C.c_int, [C.c_int, C.c_int, _dp, _dp]),

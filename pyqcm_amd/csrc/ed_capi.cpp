@@ -355,6 +355,28 @@ int ed_potthoff_functional(int label, double *omega) {
   });
 }
 
+int ed_instance_parameters(int label, int cap, int name_len, char *names, double *values, int *n, char *model,
+                           int model_cap) {
+  return guard([&] {
+    need(n != nullptr && (cap == 0 || (names && values && name_len > 0)), "ed_instance_parameters: bad arguments");
+    Instance &I = get_instance(label);
+    *n = (int)I.value.size();
+    int k = 0;
+    for (const auto &kv : I.value) {
+      if (k < cap) {
+        need((int)kv.first.size() < name_len, "ed_instance_parameters: name buffer too small");
+        std::memcpy(names + (size_t)k * name_len, kv.first.c_str(), kv.first.size() + 1);
+        values[k] = kv.second;
+      }
+      ++k;
+    }
+    if (model) {
+      need(model_cap > (int)I.model->name.size(), "ed_instance_parameters: model buffer too small");
+      std::strcpy(model, I.model->name.c_str());
+    }
+  });
+}
+
 int ed_complex_hs(int label, int *is_complex) {
   return guard([&] {
     need(is_complex != nullptr, "ed_complex_hs: null output");

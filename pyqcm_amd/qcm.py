@@ -31,7 +31,7 @@ __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_m
            "Green_function_average", "Green_function_density", "tr_sigma_inf", "susceptibility", "susceptibility_poles",
            "interrupt", "write_instance_to_hdf5", "read_instance_from_hdf5", "hdf5_version", "ED_complex_HS",
            "hopping_matrix", "interactions", "hybridization", "matrix_elements", "fidelity", "density_matrix",
-           "print_wavefunction"]
+           "print_wavefunction", "parametersC", "new_operator_complex"]
 
 _models = {}      # name -> (n_sites, n_bath, [operator names])
 _instances = {}   # label -> (model name, {name: value})
@@ -275,6 +275,33 @@ def Potthoff_functionalC(label=0):
     return x.value
 
 
+def parametersC(label=0):
+    """qcm.parametersC (src_python/qcm_ED_wrap.hpp:263-273): (dict of the instance's
+    parameters, model name)."""
+    n = C.c_int()
+    model = C.create_string_buffer(256)
+    check(lib().ed_instance_parameters(int(label), 0, 0, None, None, C.byref(n), model, 256))
+    cap, w = max(n.value, 1), 128
+    names = C.create_string_buffer(cap * w)
+    vals = np.zeros(cap)
+    check(lib().ed_instance_parameters(int(label), n.value, w, names, vals.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(n), model, 256))
+    raw = names.raw
+    d = {raw[k * w:(k + 1) * w].split(b"\0", 1)[0].decode(): float(vals[k]) for k in range(n.value)}
+    return d, model.value.decode()
+
+
+def new_operator_complex(model, op, type, elements):
+    """qcm.new_operator_complex (src_python/qcm_ED_wrap.hpp:219-261): complex
+    Hilbert spaces are not supported here; elements whose values are all real
+    define the same operator as new_operator, anything else raises EDError
+    (code 8)."""
+    el = [(e[0], e[1], complex(e[2])) for e in elements]
+    if any(v.imag != 0.0 for (_, _, v) in el):
+        raise EDError(8, "operator %s has complex matrix elements: complex operators are not supported" % op)
+    new_operator(model, op, type, [(r, c, v.real) for (r, c, v) in el])
+
+
 def ED_complex_HS(label=0):
     """qcm.ED_complex_HS (src_python/qcm_ED_wrap.hpp:27-30): 0 (real Hilbert space)."""
     x = C.c_int()
@@ -442,7 +469,7 @@ def cluster_averages(label=0):
     n = C.c_int()
     check(lib().ed_cluster_averages(int(label), cap, ave.ctypes.data_as(C.POINTER(C.c_double)),
                                     var.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
-    names = sorted(_instances.get(int(label), ("", {}))[1].keys())
+    names = list(parametersC(label)[0].keys())
     return {names[k] if k < len(names) else str(k): (ave[k], var[k]) for k in range(n.value)}
 
 

@@ -202,3 +202,19 @@ def test_hopping_matrix_hybridization_interactions_matrix_elements():
     assert err.value.code == 1
     for lab in (61, 62, 63):
         qcm.erase_model_instance(lab)
+
+
+def test_parameters_and_complex_operator_entry():
+    """parametersC returns the instance's parameters and model name;
+    new_operator_complex takes real-valued elements and refuses complex ones
+    (code 8: complex Hilbert spaces are not supported)"""
+    qcm.new_model("capi_par", 2, 0)
+    qcm.new_operator("capi_par", "t", "one-body", [(1, 2, -1.0), (3, 4, -1.0)])
+    qcm.new_operator_complex("capi_par", "h", "one-body", [(1, 1, 1.0 + 0j), (3, 3, -1.0 + 0j)])
+    assert qcm.matrix_elements("capi_par", "h") == ("one_body", [(0, 0, 1 + 0j), (2, 2, -1 + 0j)])
+    with pytest.raises(_lib.EDError) as err:
+        qcm.new_operator_complex("capi_par", "ti", "one-body", [(1, 2, 1j), (3, 4, 1j)])
+    assert err.value.code == 8
+    qcm.new_model_instanceC("capi_par", {"t": 1.0, "h": 0.3}, "R0:N2:S0", 71)
+    assert qcm.parametersC(71) == ({"h": 0.3, "t": 1.0}, "capi_par")
+    qcm.erase_model_instance(71)
