@@ -531,7 +531,8 @@ static void build_gf(Instance &I, State &st, int spin) {
         for (int l = 0; l < p; ++l) mc.W[k + (size_t)l * pa] = W[k + (size_t)l * p];
       std::vector<const double *> q;
       for (int k = 0; k < pa; ++k) q.push_back(phi[k].p);
-      const int M0 = (int)(14 * pa * std::log(1.0 * dt));
+      // at least one floor (14 p log(dt) is 0 for a one-state sector)
+      const int M0 = std::max(1, (int)(14 * pa * std::log(1.0 * dt)));
       mc.nfl = block_lanczos(*H, q, M0, mc.A, mc.B);
       const int pp = pa * pa;
       for (int j = 0; j < mc.nfl; ++j) {

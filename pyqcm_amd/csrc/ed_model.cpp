@@ -1216,6 +1216,14 @@ static bool tridiag_lowest_rqi(int n, const double *d, const double *e, double g
 
 void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::vector<double> &z0, double guess) {
   if (n <= 0) fail(ED_ERR_INVALID, "empty tridiagonal matrix");
+  if (n == 1) {
+    // a one-state Krylov space (a sector of dimension 1, or an exact
+    // eigenvector as start): the shifted factorisation below would divide by a
+    // shift of the matrix's own (possibly zero) scale
+    ev0 = d[0];
+    z0.assign(1, 1.0);
+    return;
+  }
   double norm = 0.0;
   for (int i = 0; i < n; ++i)
     norm = std::max(norm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0));

@@ -254,6 +254,9 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
   const bool no_deg = Pm.getb("no_degenerate_BL");
   if (dim == 0 || p == 0) return BandLanczosRun{};
   int M0 = (M_in < 0) ? (int)(14 * p * std::log(1.0 * dim)) : M_in;
+  // 14 p log(dim) is 0 for a one-state sector: room for the p start vectors
+  // and the step that deflates them
+  if (M_in < 0) M0 = std::max(M0, p + 1);
   int pc = p;
   if (M0 > max_iter_bl) M0 = max_iter_bl;
   const int M0p = (M0 / p + 1) * p;
