@@ -185,6 +185,17 @@ __device__ __forceinline__ void tri_lowest_wave(int n, const double *a, const do
                                                 double *hi_io, double *z, double *D, double *l, double *lam,
                                                 double *zlast) {
   const int lane = threadIdx.x;
+  if (n == 1) {
+    // a one-step space: the eigenpair is a_0 and the unit vector (a zero
+    // a_0 would otherwise leave no scale for the shifted factorization)
+    if (lane == 0) {
+      z[0] = 1.0;
+      *lam = a[0];
+      *zlast = 1.0;
+      *hi_io = a[0] + 4.0 * DBL_EPSILON * fabs(a[0]) + 1e-300;
+    }
+    return;
+  }
   double lo = 1e300, hi = -1e300, nrm = 0.0;
   for (int i = lane; i < n; i += kW) {
     const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0);
@@ -338,7 +349,8 @@ __global__ __launch_bounds__(BS) void k_batch_gs(const BatchGSItem *items, int m
     ++k;
     if (threadIdx.x < kW) tri_lowest_wave(k, al, be, bb, &s_hi, z, D, l, &s_lam, &s_zl);
     __syncthreads();
-    if (fabs(s_zl * beta) <= accur) break;
+    // a zero beta: the Krylov space is exhausted (one-state sectors end here)
+    if (!(beta > 0.0) || fabs(s_zl * beta) <= accur) break;
     if (k > max_iter) {
       status = 1;
       break;

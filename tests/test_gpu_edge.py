@@ -137,3 +137,34 @@ def test_products_on_the_smallest_sectors():
                 assert np.abs(y - Hd @ x).max() < 1e-12, (L, sec, fmt, np.abs(y - Hd @ x).max())
                 del H
         qcm.erase_model_instance(lab)
+
+
+def test_batched_solver_on_the_smallest_sectors():
+    """the same one-state sectors solved by the batched solver (ground states
+    and continued fractions for many instances at once)"""
+    qcm.set_global_parameterC("GF_method", "F")
+    U = 3.0
+    labs = []
+    for k in range(3):
+        _label[0] += 1
+        qcm.new_model_instanceC(_chain(1), {"U": U + k, "mu": (U + k) / 2}, "R0:N1:S1/R0:N1:S-1", _label[0])
+        labs.append(_label[0])
+    _label[0] += 1
+    qcm.new_model_instanceC(_chain(3), {"t": 1.0, "U": 5.0, "mu": -2.0}, "R0:N0:S0", _label[0])
+    labs.append(_label[0])
+    qcm.solve_batch(labs)
+    for k, lab in enumerate(labs[:3]):
+        Uk = U + k
+        assert abs(qcm.ground_state_solve(lab)[0] + Uk / 2) < 1e-12
+        G = qcm.Green_function_grid(Z, label=lab)[:, 0, 0]
+        ref = 0.5 / (Z + Uk / 2) + 0.5 / (Z - Uk / 2)
+        assert np.abs(G - ref).max() < 1e-12, (k, np.abs(G - ref).max())
+    lab = labs[3]
+    assert abs(qcm.ground_state_solve(lab)[0]) < 1e-14
+    G = qcm.Green_function_grid(Z, label=lab)
+    h = _h(3, 1.0, -2.0)
+    for iz, z in enumerate(Z):
+        assert np.abs(G[iz] - np.linalg.inv(z * np.eye(3) - h)).max() < 1e-10
+    for lab in labs:
+        qcm.erase_model_instance(lab)
+    qcm.set_global_parameterC("GF_method", "L")
