@@ -128,6 +128,52 @@ def test_E0_config_chain8(oracle):
     assert np.linalg.norm(H @ gs - E0 * gs) < 1e-6
 
 
+def test_E0_c2v_cluster_S_format_ref(oracle):
+    """tests/test_diagonalization/S_format.ref: E0 = -75.0353 in sector
+    R0:N12:S0 of the 2x2 cluster with 8 bath orbitals (U = 14, mu = 9.68 on
+    the 4 sites, bath energies and hybridizations of the printed solution;
+    fixture tests/golden/s_format_c2v.json).  The reference reduces by C2v;
+    the full sector's lowest state is that R0 state (853 776 states, ~10 s)."""
+    import json
+    import os
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "s_format_c2v.json")))
+    P = fix["parameters"]
+    terms = [("one-body", P[op + "_1"], [(i - 1, j - 1, float(v)) for (i, j, v) in d["elements"]])
+             for op, d in fix["operators"].items() if d["type"] == "one-body" and P.get(op + "_1", 0.0) != 0.0]
+    terms.append(("interaction", P["U"], [(i, i + 12, 1.0) for i in range(4)]))
+    terms.append(("one-body", P["mu"], [(i, i, -1.0) for i in range(4)] + [(i + 12, i + 12, -1.0) for i in range(4)]))
+    M = oracle.Model(12, terms)
+    x = np.random.default_rng(3).normal(size=M.dim(6, 6))
+    E0, gs, al, be = M.lanczos_gs(6, 6, x, accur=1e-10)
+    assert round(E0, 4) == fix["E0"] == -75.0353
+
+
+def test_c2v_green_functions_S_format_ref():
+    """The oracle's converged Green functions of the C2v cluster's four
+    representations (tests/golden/make_sformat_green.py, 400-step continued
+    fractions from the oracle's ground state) against the four Lehmann blocks
+    of tests/test_diagonalization/S_format.ref: equal on the imaginary axis to
+    the printed digits, every block nearest its own representation, and the
+    oracle's E0 rounds to the printed one."""
+    import json
+    import os
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    fix = json.load(open(os.path.join(here, "s_format_c2v.json")))
+    gre = json.load(open(os.path.join(here, "s_format_c2v_green.json")))
+    z = np.array(gre["z"]) @ np.array([1.0, 1.0j])
+    im = z.real == 0.0
+    assert im.sum() == 7
+    assert round(gre["E0"], 4) == fix["E0"] and gre["residual"] < 1e-10
+    D = np.zeros((4, 4))
+    for r, blk in enumerate(fix["blocks"]):
+        b = np.array(blk)
+        g = (b[:, 1] ** 2 / (z[im, None] - b[:, 0])).sum(axis=1)
+        for s in range(4):
+            D[r, s] = np.abs(g - np.array(gre["G"][s])[im] @ np.array([1.0, 1.0j])).max()
+    assert np.diag(D).max() < 1e-10, D
+    assert (D - np.diag(np.diag(D))).max() > 0 and (D + 1e-6 * np.eye(4)).argmin(axis=1).tolist() == [0, 1, 2, 3], D
+
+
 def test_free_fermion_closed_form(oracle):
     """U = 0 open chain: E0 = 2 sum_{eps_k<0} eps_k, eps_k = -2t cos(k pi/(n+1));
     S_format_complex.ref:37 records -9.51754 for n = 8 (U = 1e-9, gauge-equivalent)."""
