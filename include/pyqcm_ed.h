@@ -137,6 +137,13 @@ int ed_hamiltonian_autotune(int64_t handle, int cap, double *ms, int *n, int *be
 /* ---- bases (src_ed/ED_basis.cpp:115-179) and sectors (src_ed/sector.hpp) ---- */
 int ed_half_basis(int L, int N, uint32_t *out, int64_t cap, int64_t *dim);
 int ed_sector_parse(const char *s, int *N, int *S, int *irrep);
+/* the Lanczos start vector of the reference (random(), src_util/vector_num.cpp:114-119):
+   n draws of std::normal_distribution<double>(0, 1) over std::default_random_engine(seed),
+   normalized.  One distribution object serves the process, as the reference's normal_dis
+   (src_ed/Hamiltonian/Lanczos.cpp:7): after an odd n its kept second draw starts the next
+   vector.  The ground-state solve uses it when host_random_start is true (host only, no
+   device call) */
+int ed_random_start(int64_t n, int64_t seed, double *out);
 
 /* ---- Krylov drivers on a Hamiltonian handle (host vectors in/out) ---- */
 /* LanczosEigenvalue + LanczosEigenvector (src_ed/Hamiltonian/Lanczos.hpp:41-143) */

@@ -107,6 +107,7 @@ SIGNATURES = {
     "ed_hamiltonian_set_variant": (C.c_int, [C.c_int64, C.c_int]),
     "ed_hamiltonian_autotune": (C.c_int, [C.c_int64, C.c_int, _dp, _ip, _ip]),
     "ed_half_basis": (C.c_int, [C.c_int, C.c_int, _u32p, C.c_int64, _i64p]),
+    "ed_random_start": (C.c_int, [C.c_int64, C.c_int64, _dp]),
     "ed_sector_parse": (C.c_int, [C.c_char_p, _ip, _ip, _ip]),
     "ed_lanczos_gs": (C.c_int, [C.c_int64, _dp, _dp, _dp, _ip]),
     "ed_lanczos_green": (C.c_int, [C.c_int64, _dp, C.c_int, _dp, _dp, _ip]),

@@ -728,6 +728,15 @@ int ed_half_basis(int L, int N, uint32_t *out, int64_t cap, int64_t *dim) {
   });
 }
 
+int ed_random_start(int64_t n, int64_t seed, double *out) {
+  return guard([&] {
+    need(n >= 0 && (n == 0 || out != nullptr), "ed_random_start: null output");
+    std::vector<double> x;
+    random_start((size_t)n, (long)seed, x);
+    if (n > 0) std::memcpy(out, x.data(), sizeof(double) * (size_t)n);
+  });
+}
+
 int ed_sector_parse(const char *s, int *N, int *S, int *irrep) {
   return guard([&] {
     Sector x = Sector::parse(s);

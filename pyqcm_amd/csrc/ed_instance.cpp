@@ -199,15 +199,20 @@ Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, doub
 // Hamiltonian<T>::states, src_ed/Hamiltonian/Hamiltonian_base.hpp:115-221;
 // compute_weights, src_ed/model_instance.hpp:356-417)
 // ---------------------------------------------------------------------------
-static void random_start(size_t n, long seed, std::vector<double> &x) {
+// one N(0, 1) distribution for the process, as the reference's normal_dis
+// (src_ed/Hamiltonian/Lanczos.cpp:7): the polar method makes its draws in pairs
+// and keeps the second, so after a vector of odd length the next vector starts
+// with that kept draw (from the previous seed's stream)
+static std::normal_distribution<double> g_normal(0.0, 1.0);
+
+void random_start(size_t n, long seed, std::vector<double> &x) {
   // random() of src_util/vector_num.cpp:114-119: default_random_engine(seed),
-  // N(0,1) draws, normalized
+  // N(0,1) draws, scaled by 1 / ||x|| (normalize, src_util/vector_num.hpp:139-145)
   std::default_random_engine gen((unsigned)seed);
-  std::normal_distribution<double> nd(0.0, 1.0);
   x.resize(n);
   double s = 0.0;
   for (size_t i = 0; i < n; ++i) {
-    x[i] = nd(gen);
+    x[i] = g_normal(gen);
     s += x[i] * x[i];
   }
   const double f = 1.0 / std::sqrt(s);

@@ -148,6 +148,10 @@ size_t sector_dim(int L, const Sector &s);
 // solved instances to / from HDF5 files (ed_hdf5.cpp; the HDF5 C library is
 // loaded at run time)
 std::string hdf5_version();
+// the reference's Lanczos start vector (random() of src_util/vector_num.cpp:114-119):
+// n N(0,1) draws of std::normal_distribution over std::default_random_engine(seed), normalized;
+// one distribution object for the process, as there (its kept second draw carries over)
+void random_start(size_t n, long seed, std::vector<double> &x);
 void write_instance_hdf5(const std::string &filename, int label, const std::string &group);
 void read_instance_hdf5(const std::string &filename, int label, const std::string &group);
 bool sector_valid(int L, const Sector &s);

@@ -218,3 +218,75 @@ def test_parameters_and_complex_operator_entry():
     qcm.new_model_instanceC("capi_par", {"t": 1.0, "h": 0.3}, "R0:N2:S0", 71)
     assert qcm.parametersC(71) == ({"h": 0.3, "t": 1.0}, "capi_par")
     qcm.erase_model_instance(71)
+
+
+class _LibstdcxxNormal:
+    """std::normal_distribution<double>(0, 1) over std::default_random_engine,
+    restated from libstdc++'s published algorithm: minstd_rand0
+    (x <- 16807 x mod 2^31 - 1; a seed s is taken mod m, 0 becomes 1),
+    generate_canonical<double, 53> from k = 2 engine outputs
+    ((g1 - 1) + (g2 - 1) R) / R^2 with R = 2^31 - 2, and Marsaglia's polar
+    method (u, v = 2 U - 1 until 0 < r2 <= 1; returns v f and keeps u f,
+    f = sqrt(-2 log r2 / r2)).  The kept draw lives in the distribution
+    object, which outlives an engine's seeding."""
+
+    M = 2147483647
+
+    def __init__(self):
+        self.saved = None
+
+    def seed(self, s):
+        self.x = (s % 2 ** 32) % self.M or 1
+
+    def _canonical(self):
+        R = float(self.M - 1)
+        s = 0.0
+        tmp = 1.0
+        for _ in range(2):
+            self.x = (16807 * self.x) % self.M
+            s += float(self.x - 1) * tmp
+            tmp *= R
+        r = s / tmp
+        return r if r < 1.0 else float(np.nextafter(1.0, 0.0))
+
+    def draw(self):
+        import math
+        if self.saved is not None:
+            v, self.saved = self.saved, None
+            return v
+        while True:
+            u = 2.0 * self._canonical() - 1.0
+            v = 2.0 * self._canonical() - 1.0
+            r2 = u * u + v * v
+            if not (r2 > 1.0 or r2 == 0.0):
+                break
+        f = math.sqrt(-2.0 * math.log(r2) / r2)
+        self.saved = u * f
+        return v * f
+
+
+def test_host_start_vector_is_the_reference_stream():
+    """ed_random_start = random() of src_util/vector_num.cpp:114-119 with the
+    process-wide normal_dis (src_ed/Hamiltonian/Lanczos.cpp:7): the draws of
+    libstdc++'s engine and distribution, one distribution object across
+    calls (an odd length leaves a kept draw that starts the next vector),
+    scaled to unit norm (the reference's dnrm2 and the sum here agree to
+    rounding).  The calls below take an even number of draws in all, so the
+    library's distribution ends with no kept draw, as it started."""
+    ref = _LibstdcxxNormal()
+    # the library's distribution starts with no kept draw in this process
+    # (nothing else here draws from the host stream); 28 draws in all
+    calls = [(5, 1), (4, 2), (7, 12345), (1, 0), (6, 2147483647), (3, 42), (2, 7)]
+    for n, s in calls:
+        out = np.zeros(n)
+        _lib.check(_lib.lib().ed_random_start(n, s, out.ctypes.data_as(_lib._dp)))
+        ref.seed(s)
+        raw = np.array([ref.draw() for _ in range(n)])
+        exp = raw / np.sqrt(np.sum(raw * raw))
+        assert np.abs(out - exp).max() <= 4e-16 * np.abs(exp).max(), (n, s, out, exp)
+    # seeds 0 and m both start the engine at 1: from a clean state the same vector
+    a = np.zeros(6)
+    b = np.zeros(6)
+    _lib.check(_lib.lib().ed_random_start(6, 0, a.ctypes.data_as(_lib._dp)))
+    _lib.check(_lib.lib().ed_random_start(6, 2147483647, b.ctypes.data_as(_lib._dp)))
+    assert np.array_equal(a, b)
