@@ -168,3 +168,20 @@ def test_batched_solver_on_the_smallest_sectors():
     for lab in labs:
         qcm.erase_model_instance(lab)
     qcm.set_global_parameterC("GF_method", "L")
+
+
+def test_host_start_vector_solve():
+    """the reference's host start vector (host_random_start) and the device
+    one give the same ground state and Green function"""
+    vals = {"t": 1.0, "U": 4.0, "mu": 2.0}
+    out = []
+    for host in (False, True):
+        qcm.set_global_parameterC("host_random_start", host)
+        lab = _instance(4, vals, "R0:N4:S0", "L")
+        E0, _ = qcm.ground_state_solve(lab)
+        out.append((E0, qcm.Green_function_grid(Z, label=lab)))
+        qcm.erase_model_instance(lab)
+    qcm.set_global_parameterC("host_random_start", False)
+    assert abs(out[0][0] - out[1][0]) < 1e-10 * abs(out[0][0])
+    assert round(out[1][0], 7) == -9.9531453  # examples/1D_4/GS.tsv:2
+    assert np.abs(out[0][1] - out[1][1]).max() < 1e-8
