@@ -145,7 +145,11 @@ int ed_sector_parse(const char *s, int *N, int *S, int *irrep);
    device call) */
 int ed_random_start(int64_t n, int64_t seed, double *out);
 
-/* ---- Krylov drivers on a Hamiltonian handle (host vectors in/out) ---- */
+/* ---- Krylov drivers on a Hamiltonian handle (host vectors in/out) ----
+   Output capacity: `cap` counts coefficients (a: cap, b: cap + 1), poles (evals: cap,
+   Q: p x cap) or floors (A, B: cap p x p blocks).  When the result is longer, nothing
+   is written, the size needed is stored in *n / *M / *nfl and ED_ERR_INVALID is
+   returned, so the caller can retry with that capacity. */
 /* LanczosEigenvalue + LanczosEigenvector (src_ed/Hamiltonian/Lanczos.hpp:41-143) */
 int ed_lanczos_gs(int64_t h, const double *start, double *E0, double *gs, int *niter);
 /* LanczosGreen (src_ed/Hamiltonian/Lanczos.hpp:156-183): a[n], b[n+1] */

@@ -11,9 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpyqcm_ed.so")
 
 ED_OK = 0
+ED_ERR_INVALID = 1
+ED_ERR_CANCELLED = 11
 _STATUS = {1: "invalid argument", 2: "invalid sector", 3: "not converged", 4: "Q-matrix norm violated",
            5: "out of memory", 6: "HIP error", 7: "not found", 8: "unsupported", 9: "internal error",
-           10: "degenerate band Lanczos gap"}
+           10: "degenerate band Lanczos gap", 11: "cancelled"}
 
 
 class EDError(RuntimeError):

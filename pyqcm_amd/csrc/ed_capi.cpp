@@ -767,10 +767,11 @@ int ed_lanczos_green(int64_t h, const double *psi, int cap, double *a, double *b
     std::vector<double> av, bv;
     lanczos_green(H, d0.p, av, bv);
     *n = (int)av.size();
-    if ((int)av.size() <= cap) {
-      std::memcpy(a, av.data(), sizeof(double) * av.size());
-      std::memcpy(b, bv.data(), sizeof(double) * bv.size());
-    }
+    if ((int)av.size() > cap)
+      fail(ED_ERR_INVALID, "ed_lanczos_green: " + std::to_string(av.size()) + " coefficients exceed cap " +
+                               std::to_string(cap) + " (*n holds the size needed)");
+    std::memcpy(a, av.data(), sizeof(double) * av.size());
+    std::memcpy(b, bv.data(), sizeof(double) * bv.size());
   });
 }
 
@@ -787,10 +788,11 @@ int ed_band_lanczos(int64_t h, int p, const double *phi, int cap, double *evals,
     std::vector<double> e, q;
     band_lanczos(H, ptrs, e, q);
     *M = (int)e.size();
-    if (*M <= cap) {
-      std::memcpy(evals, e.data(), sizeof(double) * e.size());
-      std::memcpy(Q, q.data(), sizeof(double) * q.size());
-    }
+    if (*M > cap)
+      fail(ED_ERR_INVALID, "ed_band_lanczos: " + std::to_string(e.size()) + " poles exceed cap " +
+                               std::to_string(cap) + " (*M holds the size needed)");
+    std::memcpy(evals, e.data(), sizeof(double) * e.size());
+    std::memcpy(Q, q.data(), sizeof(double) * q.size());
   });
 }
 
@@ -821,10 +823,11 @@ int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double
     std::vector<double> Av, Bv;
     const int n = block_lanczos(H, ptrs, M0, Av, Bv);
     *nfl = n;
-    if (n <= cap) {
-      std::memcpy(A, Av.data(), sizeof(double) * Av.size());
-      std::memcpy(B, Bv.data(), sizeof(double) * Bv.size());
-    }
+    if (n > cap)
+      fail(ED_ERR_INVALID, "ed_block_lanczos: " + std::to_string(n) + " floors exceed cap " + std::to_string(cap) +
+                               " (*nfl holds the size needed)");
+    std::memcpy(A, Av.data(), sizeof(double) * Av.size());
+    std::memcpy(B, Bv.data(), sizeof(double) * Bv.size());
   });
 }
 

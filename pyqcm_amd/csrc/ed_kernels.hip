@@ -1118,7 +1118,8 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   const size_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
   const size_t nb = (size_t)Q.nxcd * slabs_per_xcd * Q.nchunk;
   // per-block partials go to a scratch array folded back to <= kPartialsCap
-  static thread_local DevBuf<double> big;
+  // (one array for the process: library calls are serialized)
+  static DevBuf<double> big;
   if (part && nb > (size_t)kPartialsCap && big.n < nb) big.alloc(nb);
   Q.part = part ? (nb > (size_t)kPartialsCap ? big.p : part) : nullptr;
 #define ED_SLAB(SWV, NTV) hipLaunchKernelGGL((k_hv_down_slab<SWV, 256 / SWV, NTV>), dim3((unsigned)nb), dim3(256), 0, s, Q)

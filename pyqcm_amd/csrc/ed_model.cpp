@@ -223,15 +223,26 @@ struct PartialArena {
     return (int)((size_t)(p - d) / slot_len);
   }
 };
+// One arena for the process: every C-ABI call holds the library lock, so the
+// slots are never used from two threads at once, and the pinned block is
+// allocated once however many threads call in.
+PartialArena *g_arena = nullptr;
 PartialArena &arena() {
-  static thread_local PartialArena a;
+  static PartialArena a;
+  g_arena = &a;
   return a;
 }
-inline bool is_sentinel(const double *p) {
-  uint64_t u;
-  std::memcpy(&u, (const void *)p, 8);
-  return u == kSentinel;
+// slot of p in the arena, -1 if p is elsewhere (or no arena exists yet: the
+// lookup never allocates one)
+int arena_slot_of(const double *p) { return g_arena ? g_arena->slot_of(p) : -1; }
+
+// the device writes these entries while the host polls them: each read is an
+// atomic acquire load, so the compiler re-reads memory on every poll and the
+// row is read after the sentinel test that saw it complete
+inline uint64_t load_entry(const double *p) {
+  return __atomic_load_n(reinterpret_cast<const uint64_t *>(p), __ATOMIC_ACQUIRE);
 }
+inline bool is_sentinel(const double *p) { return load_entry(p) == kSentinel; }
 double pairwise(std::vector<double> &w) {
   size_t n = w.size();
   while (n > 1) {
@@ -273,8 +284,7 @@ double *partials_acquire() {
 }
 
 double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std::vector<double> *out) {
-  PartialArena &A = arena();
-  const int slot = A.slot_of(dpart);
+  const int slot = arena_slot_of(dpart);
   if (out) out->assign(std::max(m, 0), 0.0);
   if (slot < 0) {
     // device partials: fold on the host after a copy
@@ -291,6 +301,7 @@ double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std:
     }
     return first;
   }
+  PartialArena &A = *g_arena;
   const double *hp = A.h + (dpart - A.d);
   if (m > 0 && nb > 0) {
     if ((size_t)(dpart - A.d) - (size_t)slot * A.slot_len + (size_t)(m - 1) * stride + nb > A.slot_len)
@@ -312,12 +323,15 @@ double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std:
         }
       }
     }
-    std::atomic_thread_fence(std::memory_order_acquire);
   }
   double first = 0.0;
   for (int r = 0; r < m; ++r) {
     double *row = const_cast<double *>(hp) + (size_t)r * stride;
-    std::vector<double> w(row, row + std::max(nb, 0));
+    std::vector<double> w(std::max(nb, 0));
+    for (int i = 0; i < nb; ++i) {
+      const uint64_t u = load_entry(row + i);
+      std::memcpy(&w[i], &u, 8);
+    }
     const double s = pairwise(w);
     if (out) (*out)[r] = s;
     if (r == 0) first = s;

@@ -21,7 +21,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import EDError, check, lib
+from ._lib import ED_ERR_INVALID, EDError, check, lib
 
 __all__ = ["EDError", "new_model", "new_operator", "new_model_instanceC", "new_model_instance",
            "ground_state_solve", "Green_function_solveC", "Green_function", "Green_function_grid",
@@ -603,21 +603,31 @@ class Hamiltonian:
 
     def lanczos_green(self, psi, cap=2000):
         psi = np.ascontiguousarray(psi, dtype=np.float64)
-        a = np.zeros(cap)
-        b = np.zeros(cap + 1)
-        n = C.c_int()
-        check(lib().ed_lanczos_green(self.h, _dptr(psi), cap, _dptr(a), _dptr(b), C.byref(n)))
-        return a[:n.value], b[:n.value + 1]
+        while True:
+            a = np.zeros(cap)
+            b = np.zeros(cap + 1)
+            n = C.c_int()
+            rc = lib().ed_lanczos_green(self.h, _dptr(psi), cap, _dptr(a), _dptr(b), C.byref(n))
+            if rc == ED_ERR_INVALID and n.value > cap:
+                cap = n.value  # the entry point reports the size it needs
+                continue
+            check(rc)
+            return a[:n.value], b[:n.value + 1]
 
     def band_lanczos(self, phi, cap=4096):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         p = phi.shape[0]
-        e = np.zeros(cap)
-        Q = np.zeros(cap * p)
-        M = C.c_int()
-        check(lib().ed_band_lanczos(self.h, p, _dptr(phi), cap, _dptr(e), _dptr(Q), C.byref(M)))
-        m = M.value
-        return e[:m], Q[:p * m].reshape(m, p).T.copy()
+        while True:
+            e = np.zeros(cap)
+            Q = np.zeros(cap * p)
+            M = C.c_int()
+            rc = lib().ed_band_lanczos(self.h, p, _dptr(phi), cap, _dptr(e), _dptr(Q), C.byref(M))
+            if rc == ED_ERR_INVALID and M.value > cap:
+                cap = M.value
+                continue
+            check(rc)
+            m = M.value
+            return e[:m], Q[:p * m].reshape(m, p).T.copy()
 
     def davidson(self, K):
         ev = np.zeros(K)
@@ -629,10 +639,16 @@ class Hamiltonian:
     def block_lanczos(self, q0, M0, cap=2000):
         q0 = np.ascontiguousarray(q0, dtype=np.float64)
         p = q0.shape[0]
-        A = np.zeros(cap * p * p)
-        B = np.zeros(cap * p * p)
-        n = C.c_int()
-        check(lib().ed_block_lanczos(self.h, p, _dptr(q0), int(M0), cap, _dptr(A), _dptr(B), C.byref(n)))
+        while True:
+            A = np.zeros(cap * p * p)
+            B = np.zeros(cap * p * p)
+            n = C.c_int()
+            rc = lib().ed_block_lanczos(self.h, p, _dptr(q0), int(M0), cap, _dptr(A), _dptr(B), C.byref(n))
+            if rc == ED_ERR_INVALID and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            break
         k = n.value
         return (A[:k * p * p].reshape(k, p, p).transpose(0, 2, 1).copy(),
                 B[:k * p * p].reshape(k, p, p).transpose(0, 2, 1).copy())

@@ -410,6 +410,48 @@ def test_block_lanczos_mcf_matches_oracle(oracle):
     assert np.allclose(np.abs(B[:k]), np.abs(Br[:k]), atol=1e-9)
 
 
+def test_krylov_outputs_longer_than_cap_fail_with_the_size_needed():
+    """ed_lanczos_green / ed_band_lanczos / ed_block_lanczos write nothing and
+    return ED_ERR_INVALID with the size needed when the result exceeds cap; the
+    Python wrappers then retry with that capacity"""
+    import ctypes as C
+    wl = W.hubbard_chain(6, U=4.0, mu=2.0)
+    lab = _instance(wl, "R0:N7:S1")
+    H = qcm.Hamiltonian(lab, "R0:N7:S1", "G")
+    rng = np.random.default_rng(10)
+    psi = rng.normal(size=H.dim)
+    psi /= np.linalg.norm(psi)
+    a_full, b_full = H.lanczos_green(psi, cap=4)  # retried with the size needed
+    assert len(a_full) > 4 and len(b_full) == len(a_full) + 1
+    a = np.full(4, -7.0)
+    b = np.full(5, -7.0)
+    n = C.c_int()
+    rc = _lib.lib().ed_lanczos_green(H.h, psi.ctypes.data_as(_lib._dp), 4, a.ctypes.data_as(_lib._dp),
+                                     b.ctypes.data_as(_lib._dp), C.byref(n))
+    assert rc == _lib.ED_ERR_INVALID and n.value == len(a_full)
+    assert np.all(a == -7.0) and np.all(b == -7.0)
+    phi = rng.normal(size=(2, H.dim))
+    e_full, Q_full = H.band_lanczos(phi, cap=3)
+    assert len(e_full) > 3 and Q_full.shape == (2, len(e_full))
+    e = np.full(3, -7.0)
+    Q = np.full(6, -7.0)
+    M = C.c_int()
+    rc = _lib.lib().ed_band_lanczos(H.h, 2, phi.ctypes.data_as(_lib._dp), 3, e.ctypes.data_as(_lib._dp),
+                                    Q.ctypes.data_as(_lib._dp), C.byref(M))
+    assert rc == _lib.ED_ERR_INVALID and M.value == len(e_full)
+    assert np.all(e == -7.0) and np.all(Q == -7.0)
+    q = np.linalg.qr(rng.normal(size=(H.dim, 2)))[0].T.copy()
+    A_full, B_full = H.block_lanczos(q, 20, cap=2)
+    assert len(A_full) > 2
+    A = np.full(8, -7.0)
+    B = np.full(8, -7.0)
+    nfl = C.c_int()
+    rc = _lib.lib().ed_block_lanczos(H.h, 2, q.ctypes.data_as(_lib._dp), 20, 2, A.ctypes.data_as(_lib._dp),
+                                     B.ctypes.data_as(_lib._dp), C.byref(nfl))
+    assert rc == _lib.ED_ERR_INVALID and nfl.value == len(A_full)
+    assert np.all(A == -7.0) and np.all(B == -7.0)
+
+
 def test_mcf_green_function_matches_band(oracle):
     wl = W.hubbard_chain(6, U=4.0, mu=2.0)
     z = W.matsubara(32, 30.0)

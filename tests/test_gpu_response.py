@@ -297,6 +297,7 @@ def test_interrupt_stops_the_next_step_once():
     with pytest.raises(EDError) as e:
         qcm.ground_state_solve(lab)
     assert e.value.code == 11
+    assert str(e.value).startswith("[cancelled]"), str(e.value)
     E0, _ = qcm.ground_state_solve(lab)
     qcm.interrupt()
     with pytest.raises(EDError) as e:
