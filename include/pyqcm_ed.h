@@ -120,6 +120,17 @@ int ed_hamiltonian_dim(int64_t handle, int64_t *dim);
 int ed_hamiltonian_mult_add(int64_t handle, const double *x, double *y);
 /* y = H x + beta y on device arrays, on the library stream (y not read when beta == 0) */
 int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, double beta);
+/* Allocation-free variant of ed_hamiltonian_apply_device (no counterpart in the
+   reference): y = H x + beta y on device arrays, and *d_dot = <x, y_new> written
+   to device memory (the Lanczos alpha when beta == 0).  For callers that cannot
+   allocate (HIP graph capture, tight serving loops): it calls no hipMalloc /
+   hipFree; the per-block partials go to a fixed device scratch of 3.2 MB per
+   device (8 192 partials + 393 216 two-pass slab partials), and a product needing
+   more fails with ED_ERR_INVALID.  It never autotunes (the tuned schedule once
+   ed_hamiltonian_autotune or ed_hamiltonian_apply_device has tuned the handle,
+   else the default schedule).  Calls that overlap on one device race on the
+   scratch. */
+int ed_hamiltonian_apply_dot_noalloc(int64_t handle, const double *dx, double *dy, double beta, double *d_dot);
 /* Hamiltonian::diag */
 int ed_hamiltonian_diag(int64_t handle, double *d);
 /* strict-lower CSR pattern of the 'S' format (Hamiltonian_CSR.hpp:60-81): Iptr[dim+1], J[nnz] */

@@ -103,6 +103,7 @@ SIGNATURES = {
     "ed_hamiltonian_dim": (C.c_int, [C.c_int64, _i64p]),
     "ed_hamiltonian_mult_add": (C.c_int, [C.c_int64, _dp, _dp]),
     "ed_hamiltonian_apply_device": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_double]),
+    "ed_hamiltonian_apply_dot_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "ed_hamiltonian_diag": (C.c_int, [C.c_int64, _dp]),
     "ed_hamiltonian_csr_pattern": (C.c_int, [C.c_int64, _i64p, _i32p, _i32p, _dp]),
     "ed_hamiltonian_time": (C.c_int, [C.c_int64, C.c_int, _dp, C.c_int]),

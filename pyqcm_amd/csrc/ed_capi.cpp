@@ -657,6 +657,18 @@ int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, do
   });
 }
 
+int ed_hamiltonian_apply_dot_noalloc(int64_t handle, const double *dx, double *dy, double beta, double *d_dot) {
+  return guard([&] {
+    need(d_dot != nullptr, "ed_hamiltonian_apply_dot_noalloc: null dot output");
+    Hamiltonian &H = get_h(handle);
+    need(H.dim == 0 || (dx && dy), "ed_hamiltonian_apply_dot_noalloc: null vector");
+    const NoAllocScratch S = noalloc_scratch();
+    const int nb = H.apply_fixed(dx, dy, 1.0, beta, S.part, S.aux, S.aux_cap);
+    fold_partials_device(S.part, nb, d_dot, stream());
+    sync();
+  });
+}
+
 int ed_hamiltonian_diag(int64_t handle, double *d) {
   return guard([&] {
     Hamiltonian &H = get_h(handle);

@@ -1002,6 +1002,56 @@ void allow_lds(K kern, size_t bytes) {
 
 int partials_cap() { return kPartialsCap; }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free entry points (ed_*_noalloc): the
+// per-block partials of one product (kPartialsCap) followed by the two-pass
+// schedule's per-slab-block partials (kNoAllocAux; at most 8 XCDs x 202 slabs
+// x 202 row chunks = 326 432 on the 12 870 x 12 870 sector, with 8-column
+// slabs).  3.2 MB per device, held until the process ends.  Every launch overwrites
+// the entries it later reads, so nothing is cleared between calls; calls that
+// overlap on one device race on it.  In host code the array's name is only a
+// stand-in: the device address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocAux = 393216;
+constexpr int kMaxNoAllocDevices = 64;
+static __device__ __align__(256) double scratch[kPartialsCap + kNoAllocAux];
+
+NoAllocScratch noalloc_scratch() {
+  static double *base[kMaxNoAllocDevices] = {};
+  int dev = 0;
+  ED_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxNoAllocDevices) fail(ED_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  if (!base[dev]) {
+    void *p = nullptr;
+    ED_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(ED_ERR_HIP, "no device address for the fixed scratch");
+    base[dev] = static_cast<double *>(p);
+  }
+  NoAllocScratch S;
+  S.part = base[dev];
+  S.aux = base[dev] + kPartialsCap;
+  S.aux_cap = kNoAllocAux;
+  return S;
+}
+
+namespace {
+// one block: out[0] = sum of in[0 .. n), lanes strided then a fixed-order tree
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fold_one(const double *in, int n, double *out) {
+  __shared__ double red[BS / kWave];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += BS) s += in[i];
+  s = block_sum<BS>(s, red);
+  if (threadIdx.x == 0) out[0] = s;
+}
+}  // namespace
+
+void fold_partials_device(const double *part, int nb, double *out, hipStream_t s) {
+  hipLaunchKernelGGL((k_fold_one<256>), dim3(1), dim3(256), 0, s, part, nb, out);
+  ED_HIP(hipGetLastError());
+}
+
+
 static FusedParams make_fused(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part,
                               uint32_t R) {
   FusedParams P;
@@ -1026,7 +1076,7 @@ static int fused_grid(const FactArgs &a, uint32_t R) {
 int hv_factorized_max_partials(const FactArgs &a, int variant) { return kPartialsCap; }
 
 int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
-                      const SlabConfig &sc, hipStream_t s) {
+                      const SlabConfig &sc, hipStream_t s, double *aux, size_t aux_cap) {
   const uint32_t du = a.up.dim, dd = a.dn.dim;
   if ((size_t)du * dd == 0) return 0;
   uint32_t R = rows_per_block(du);
@@ -1118,10 +1168,22 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   const size_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
   const size_t nb = (size_t)Q.nxcd * slabs_per_xcd * Q.nchunk;
   // per-block partials go to a scratch array folded back to <= kPartialsCap
-  // (one array for the process: library calls are serialized)
+  // (one array for the process: library calls are serialized), or to the
+  // caller's aux array
   static DevBuf<double> big;
-  if (part && nb > (size_t)kPartialsCap && big.n < nb) big.alloc(nb);
-  Q.part = part ? (nb > (size_t)kPartialsCap ? big.p : part) : nullptr;
+  double *wide = nullptr;
+  if (part && nb > (size_t)kPartialsCap) {
+    if (aux) {
+      if (nb > aux_cap)
+        fail(ED_ERR_INVALID, "two-pass product: " + std::to_string(nb) + " block partials exceed the fixed scratch of " +
+                                 std::to_string(aux_cap));
+      wide = aux;
+    } else {
+      if (big.n < nb) big.alloc(nb);
+      wide = big.p;
+    }
+  }
+  Q.part = part ? (wide ? wide : part) : nullptr;
 #define ED_SLAB(SWV, NTV) hipLaunchKernelGGL((k_hv_down_slab<SWV, 256 / SWV, NTV>), dim3((unsigned)nb), dim3(256), 0, s, Q)
   if (SW == 8) { if (sc.nt) ED_SLAB(8, true); else ED_SLAB(8, false); }
   else if (SW == 16) { if (sc.nt) ED_SLAB(16, true); else ED_SLAB(16, false); }
@@ -1130,7 +1192,7 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
 #undef ED_SLAB
   ED_HIP(hipGetLastError());
   if (part && nb > (size_t)kPartialsCap) {
-    hipLaunchKernelGGL((k_fold_partials<256>), dim3(kPartialsCap), dim3(256), 0, s, (const double *)big.p, nb, part);
+    hipLaunchKernelGGL((k_fold_partials<256>), dim3(kPartialsCap), dim3(256), 0, s, (const double *)wide, nb, part);
     ED_HIP(hipGetLastError());
     return kPartialsCap;
   }

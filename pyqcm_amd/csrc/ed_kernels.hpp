@@ -67,8 +67,12 @@ struct FactArgs {
 };
 
 enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
+// aux / aux_cap: device array for the two-pass schedule's per-slab-block
+// partials when they exceed partials_cap() (nullptr: a library-owned array,
+// allocated on first need; otherwise more than aux_cap of them is an error)
 int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
-                      int variant, const SlabConfig &sc, hipStream_t s);
+                      int variant, const SlabConfig &sc, hipStream_t s, double *aux = nullptr, size_t aux_cap = 0);
+
 
 // y = ax H x + beta y (y not read when beta == 0); when dot_partials != nullptr,
 // per-block partial sums of <y_new, x> are written there (nblocks returned).
@@ -106,6 +110,19 @@ int hv_otf_max_partials(const OtfArgs &a);
 
 // ---- vector kernels (FP64, wave64 reductions, per-block partials) ----
 int partials_cap();
+
+// Fixed device scratch of the allocation-free entry points: a file-scope
+// device array (one copy per device, kept until the process ends) whose
+// address is looked up once per device.  part holds partials_cap() per-block
+// partials, aux the two-pass schedule's larger partial set (aux_cap doubles).
+struct NoAllocScratch {
+  double *part = nullptr;
+  double *aux = nullptr;
+  size_t aux_cap = 0;
+};
+NoAllocScratch noalloc_scratch();
+// out[0] = sum of part[0 .. nb) in a fixed order (one block; out on the device)
+void fold_partials_device(const double *part, int nb, double *out, hipStream_t s);
 // out[k * nb + block] = partial of <a, b_k>, k < m (m <= 16). returns nb
 int multi_dot(size_t n, const double *a, int m, const double *const *b, double *partials, hipStream_t s);
 // x *= xs (written back when xs != 1), then y_k += c_k x for k < m; if

@@ -789,6 +789,16 @@ struct HamFactorized : Hamiltonian {
     if (part) pending_nb = nb;
     return nb;
   }
+  int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *aux,
+                  size_t aux_cap) override {
+    int v = variant;
+    SlabConfig sc;
+    if (v == HV_AUTO && tuned) {
+      v = best_variant;
+      sc = best_sc;
+    }
+    return hv_factorized_cfg(args, x, y, ax, beta, part, v, sc, stream(), aux, aux_cap);
+  }
   void diag(double *d) override { fact_full_diag(args, d, stream()); }
 };
 
@@ -909,6 +919,9 @@ struct HamCSR : Hamiltonian {
     if (part) pending_nb = nb;
     return nb;
   }
+  int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *, size_t) override {
+    return hv_csr(args, x, y, ax, beta, part, stream());
+  }
   void diag(double *d) override {
     if (dim) ED_HIP(hipMemcpyAsync(d, dgv.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   }
@@ -972,6 +985,9 @@ struct HamOTF : Hamiltonian {
     const int nb = hv_otf(args, x, y, ax, beta, part, stream());
     if (part) pending_nb = nb;
     return nb;
+  }
+  int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *, size_t) override {
+    return hv_otf(args, x, y, ax, beta, part, stream());
   }
   void diag(double *d) override {
     if (dim) ED_HIP(hipMemcpyAsync(d, dg.p, dim * sizeof(double), hipMemcpyDeviceToDevice, stream()));

@@ -97,6 +97,11 @@ struct Hamiltonian {
   // y = ax H x + beta y on device; returns the number of per-block partials of
   // <y_new, x> written to a partials slot (take_partials) when want_dot
   virtual int apply(const double *x, double *y, double ax, double beta, bool want_dot) = 0;
+  // the same product with the partials written to part (device memory, when
+  // not null) and nothing allocated: no autotuning (the tuned schedule once
+  // tuned, else the default one), larger partial sets in aux (aux_cap doubles)
+  virtual int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *aux,
+                          size_t aux_cap) = 0;
   // diagonal of H on device
   virtual void diag(double *d) = 0;
   double dot_result(int nb);  // sum of the partials of the last apply (deterministic order)

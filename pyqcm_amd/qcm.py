@@ -562,6 +562,12 @@ class Hamiltonian:
     def apply_device(self, dx_ptr, dy_ptr, beta=0.0):
         check(lib().ed_hamiltonian_apply_device(self.h, C.c_void_p(dx_ptr), C.c_void_p(dy_ptr), float(beta)))
 
+    def apply_dot_noalloc(self, dx_ptr, dy_ptr, ddot_ptr, beta=0.0):
+        """y = H x + beta y and dot = <x, y> on device pointers; allocates nothing
+        (the partials use the library's fixed device scratch)."""
+        check(lib().ed_hamiltonian_apply_dot_noalloc(self.h, C.c_void_p(dx_ptr), C.c_void_p(dy_ptr), float(beta),
+                                                     C.c_void_p(ddot_ptr)))
+
     def diag(self):
         d = np.zeros(self.dim)
         check(lib().ed_hamiltonian_diag(self.h, _dptr(d)))
