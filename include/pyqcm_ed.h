@@ -62,7 +62,13 @@ int ed_get_global_double(const char *name, double *value);
 int ed_get_global_int(const char *name, long *value);
 
 /* ---- model / operator / instance: namespace ED (src_ed/qcm_ED.hpp:34-339) ---- */
-/* ED::new_model (src_ed/qcm_ED.cpp:43-59); n_gen must be 0 (trivial point group). */
+/* ED::new_model (src_ed/qcm_ED.cpp:43-59, symmetry_group.cpp:16-330): n_gen generators of
+   n_sites + n_bath ints each, 0-based orbital images (with bath_irrep, the site images
+   followed by the bath orbitals' phases, exp(2 pi i phase / g)), as the Python binding
+   passes them (src_python/qcm_ED_wrap.hpp:145-160).  Groups with real characters only
+   (generators of order <= 2; others ED_ERR_UNSUPPORTED).  States of sector R{r}:N:S are
+   found in the full (N, S) sector projected on irrep r; Green functions are those of the
+   full sectors, returned in the site basis or, with blocks, in the symmetric-orbital basis. */
 int ed_new_model(const char *name, int n_sites, int n_bath, const int *generators, int n_gen, int bath_irrep);
 /* ED::new_operator, real elements (src_ed/qcm_ED.cpp:74-90); type "one-body" or "interaction". */
 int ed_new_operator(const char *model, const char *name, const char *type, int n,
@@ -305,6 +311,39 @@ int ed_susceptibility(int label, const char *op, int nw, const double *w, double
    for k < cap: poles above 1e-8, residues above 1e-6, poles closer than 1e-4 merged. */
 int ed_susceptibility_poles(int label, const char *op, int cap, double *poles, int *n);
 
+/* Continued fractions of low-energy state `state` of a solved instance
+   (GF_method 'F'; continued_fraction_set, src_ed/continued_fraction_set.cpp:12-98):
+   fraction f = 2 q + eh, q the orbital pair o1 >= o2 row by row (q = o1 (o1 + 1) / 2
+   + o2), eh 0 for the electron and 1 for the hole part, as evaluated (a_n shifted
+   by the state energy, b_n squared, b_0 the weighted norm).  a[f * cap + n] and
+   b[f * cap + n] for n < min(len[f], cap), zero beyond; len[f] the full length;
+   *nfrac = n_sites (n_sites + 1).  Serves the learned surrogate's training targets. */
+int ed_continued_fractions(int label, int spin_down, int state, int cap, double *a, double *b, int *len,
+                           int *nfrac);
+/* The same fractions for a list of orbital pairs only (o1[k] == o2[k]: c_o1 |Omega>;
+   else c_o1 |Omega> + c_o2 |Omega>, as build_cf, src_ed/model_instance.cpp:35-100),
+   from the ground state, with at most cap levels each (the first levels of the
+   unbounded fraction; LanczosGreen, Lanczos.hpp:156-183): fraction 2 k + eh for
+   pair k, a and b as above ([2 npairs][cap]).  Any GF_method; the instance's own
+   Green function is not built. */
+int ed_continued_fractions_pairs(int label, int spin_down, int npairs, const int *o1, const int *o2, int cap,
+                                 double *a, double *b, int *len);
+/* The inverse (no reference counterpart; the HDF5 reader does the same from a
+   file, model_instance.hpp:1351-1420): the instance becomes solved, with one
+   state of energy E0 in `sector` whose spin block holds these fractions (same
+   layout, nfrac = n_sites (n_sites + 1)); call once per spin block.  Used to
+   evaluate surrogate predictions through the Green-function, CPT and Potthoff
+   entry points; quantities that need a wavefunction then return ED_ERR_INVALID. */
+int ed_set_continued_fractions(int label, double E0, const char *sector, int spin_down, int nfrac, int cap,
+                               const double *a, const double *b, const int *len);
+
+/* ED::get_combined_mcf (src_ed/qcm_ED.hpp:319-339, qcm_ED.cpp:457-537): the combined
+   matrix continued fraction (G_e + G_h^T in one fraction, built when combine_mcf is
+   set, GF_method 'M') of the first low-energy state: W (p x d) and nfl floors A_j,
+   B_j (p x p), all column-major, A / B at floor offsets j p p; *p and *nfl report the
+   sizes (nothing written when nfl > cap). */
+int ed_combined_mcf(int label, int spin_down, int cap, double *W, double *A, double *B, int *p, int *nfl);
+
 /* ---- device memory helpers for callers that keep vectors resident ---- */
 int ed_device_alloc(int64_t bytes, void **ptr);
 int ed_device_free(void *ptr, int64_t bytes);
@@ -314,6 +353,41 @@ int ed_device_sync(void);
    included, up to ~235 GB on the 4x4 sector); this returns them to the driver,
    e.g. before another library in the process allocates */
 int ed_release_cache(void);
+
+/* ---- learned surrogate (no counterpart in the reference: pyqcm has no learned
+        model).  A GELU MLP, n_in -> depth hidden layers of `width` -> n_out,
+        trained and evaluated on the device: bf16 MFMA GEMMs with fp32
+        accumulation, fp32 master weights and Adam.  It maps cluster parameters
+        to E0 and truncated continued-fraction coefficients; the VCA / CDMFT
+        drivers use its predictions as candidates that the exact solver
+        verifies.  Inputs x are row-major n x n_in, outputs n x n_out, in
+        physical units (the scaling below is applied inside).  Weight order:
+        for each layer, W (out x in, row-major) then b (out). ---- */
+int ed_surrogate_create(int n_in, int n_out, int width, int depth, int64_t seed, int64_t *handle);
+int ed_surrogate_destroy(int64_t handle);
+int ed_surrogate_n_params(int64_t handle, int64_t *n);
+/* x' = (x - x_mean) / x_std, y' = (y - y_mean) / y_std; the loss is
+   sum_o loss_weight[o] (y'_o - t'_o)^2 averaged over samples.  Null pointers
+   leave a part unchanged. */
+int ed_surrogate_set_scaling(int64_t handle, const double *x_mean, const double *x_std, const double *y_mean,
+                             const double *y_std, const double *loss_weight);
+int ed_surrogate_set_optimizer(int64_t handle, double lr, double beta1, double beta2, double eps);
+int ed_surrogate_get_weights(int64_t handle, double *w);
+/* sets the fp32 master weights (and their bf16 copies) and restarts Adam */
+int ed_surrogate_set_weights(int64_t handle, const double *w);
+int ed_surrogate_predict(int64_t handle, int n, const double *x, double *y);
+/* loss of one batch (normalized units) and its gradient in the weight order
+   (no update) */
+int ed_surrogate_loss_grad(int64_t handle, int n, const double *x, const double *y, double *loss, double *grad);
+/* `epochs` passes over the n samples in shuffled batches of `batch` (full
+   batches only; shuffles from `seed`), one Adam step per batch; epoch_loss
+   receives the mean training loss of each epoch; ms_per_step the device time
+   of one step (gather, forward, backward, Adam) */
+int ed_surrogate_train(int64_t handle, int n, const double *x, const double *y, int epochs, int batch,
+                       int64_t seed, double *epoch_loss, double *ms_per_step);
+/* device time of the forward pass and of forward + backward (weights
+   unchanged) on `batch` resident rows, averaged over nrep */
+int ed_surrogate_time(int64_t handle, int batch, int nrep, double *ms_forward, double *ms_forward_backward);
 
 #ifdef __cplusplus
 }

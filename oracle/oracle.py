@@ -68,6 +68,8 @@ def _declare(L):
     L.orc_csr_build.argtypes = [P, C.c_int, C.c_int, _szp, _i32p, _i32p, _dp, _dp]
     L.orc_hv_csr.argtypes = [C.c_size_t, _i32p, _i32p, _dp, _dp, _dp, _dp]
     L.orc_hv_csr_full_omp.argtypes = [C.c_size_t, _i32p, _i32p, _dp, _dp, _dp, _dp, C.c_int]
+    L.orc_hv_factorized_omp_timed.argtypes = [P, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
+    L.orc_hv_csr_full_omp_timed.argtypes = [C.c_size_t, _i32p, _i32p, _dp, _dp, _dp, _dp, C.c_int, _dp]
     L.orc_hv_otf.argtypes = [P, C.c_int, C.c_int, _dp, _dp]
     L.orc_create_or_destroy.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp]
     L.orc_tridiag_eig.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
@@ -84,6 +86,8 @@ def _declare(L):
     L.orc_davidson.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp, _ip]
     L.orc_block_lanczos.argtypes = [P, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_double,
                                     C.c_double, _ip, _dp, _dp]
+    L.orc_block_lanczos_v.argtypes = [P, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_int, _ip, _dp, _dp]
     L.orc_mcf_eval.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp]
     L.orc_openmp_threads.restype = C.c_int
 
@@ -164,6 +168,14 @@ class Model:
         y = np.zeros_like(x)
         _chk(lib().orc_hv_factorized_omp(self.ref, Nup, Ndw, _d(x), _d(y), nrep))
         return y
+
+    def hv_factorized_omp_times(self, Nup, Ndw, x, nrep):
+        """ms of each of nrep OpenMP products (tables built once, untimed)"""
+        x = _f64(x)
+        y = np.zeros_like(x)
+        ms = np.zeros(nrep)
+        _chk(lib().orc_hv_factorized_omp_timed(self.ref, Nup, Ndw, _d(x), _d(y), nrep, _d(ms)))
+        return ms
 
     def diag(self, Nup, Ndw):
         d = np.zeros(self.dim(Nup, Ndw))
@@ -247,14 +259,14 @@ class Model:
         _chk(lib().orc_davidson(self.ref, Nup, Ndw, K, accuracy, max_iter, _d(ev), _d(vec), C.byref(nit)))
         return ev, vec.reshape(K, dim), nit.value
 
-    def block_lanczos(self, Nup, Ndw, q0, M0, max_iter_bl=600, accur_deflation=1e-7, accur_bl=1e-12):
+    def block_lanczos(self, Nup, Ndw, q0, M0, max_iter_bl=600, accur_deflation=1e-7, accur_bl=1e-12, qr=True):
         q0 = _f64(q0)
         p = q0.shape[0]
         A = np.zeros(M0 * p * p)
         B = np.zeros(M0 * p * p)
         nf = C.c_int()
-        _chk(lib().orc_block_lanczos(self.ref, Nup, Ndw, p, _d(q0), M0, max_iter_bl, accur_deflation, accur_bl,
-                                     C.byref(nf), _d(A), _d(B)))
+        _chk(lib().orc_block_lanczos_v(self.ref, Nup, Ndw, p, _d(q0), M0, max_iter_bl, accur_deflation, accur_bl,
+                                       1 if qr else 0, C.byref(nf), _d(A), _d(B)))
         n = nf.value
         return (A[:n * p * p].reshape(n, p, p).transpose(0, 2, 1).copy(),
                 B[:n * p * p].reshape(n, p, p).transpose(0, 2, 1).copy())
@@ -371,6 +383,16 @@ def csr_hv_full_omp(Iptr, J, v, dg, x, nrep=1):
     _chk(lib().orc_hv_csr_full_omp(len(x), Iptr.ctypes.data_as(_i32p), J.ctypes.data_as(_i32p), _d(v), _d(dg),
                                    _d(x), _d(y), nrep))
     return y
+
+
+def csr_hv_full_omp_times(Iptr, J, v, dg, x, nrep):
+    """ms of each of nrep row-parallel full-CSR products (rows expanded once, untimed)"""
+    x = _f64(x)
+    y = np.zeros_like(x)
+    ms = np.zeros(nrep)
+    _chk(lib().orc_hv_csr_full_omp_timed(len(x), Iptr.ctypes.data_as(_i32p), J.ctypes.data_as(_i32p), _d(v),
+                                         _d(dg), _d(x), _d(y), nrep, _d(ms)))
+    return ms
 
 
 def openmp_threads():

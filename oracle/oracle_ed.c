@@ -444,6 +444,13 @@ static void free_sym(symcsr *S) { free(S->ptr); free(S->col); free(S->val); free
    same operator, rows of X[i_dn][i_up] processed in parallel, diagonal
    precomputed as in HS_interaction_operator.  nrep repetitions, y += H x each. */
 int orc_hv_factorized_omp(const orc_model *m, int Nup, int Ndw, const double *x, double *y, int nrep) {
+  return orc_hv_factorized_omp_timed(m, Nup, Ndw, x, y, nrep, NULL);
+}
+
+/* the same, with the wall time of each product in ms[rep] (tables built first,
+   outside the timed products) */
+int orc_hv_factorized_omp_timed(const orc_model *m, int Nup, int Ndw, const double *x, double *y, int nrep,
+                                double *ms) {
   pmodel *P = prepare(m);
   if (!factorizable(P)) { release(P); return -3; }
   fbasis B;
@@ -464,6 +471,7 @@ int orc_hv_factorized_omp(const orc_model *m, int Nup, int Ndw, const double *x,
     }
   }
   for (int rep = 0; rep < nrep; ++rep) {
+    const double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 4)
     for (size_t d = 0; d < dd; ++d) {
       double *yr = y + d * du;
@@ -481,6 +489,7 @@ int orc_hv_factorized_omp(const orc_model *m, int Nup, int Ndw, const double *x,
         yr[u] += acc;
       }
     }
+    if (ms) ms[rep] = 1e3 * (omp_get_wtime() - t0);
   }
   free(dg);
   free_sym(&KU); free_sym(&KD);
@@ -587,6 +596,13 @@ int orc_hv_csr(size_t dim, const int32_t *Iptr, const int32_t *J, const double *
    the behaviour of the default Eigen RowMajor format, Hamiltonian_Eigen.hpp:75-82) */
 int orc_hv_csr_full_omp(size_t dim, const int32_t *Iptr, const int32_t *J, const double *v,
                         const double *diag, const double *x, double *y, int nrep) {
+  return orc_hv_csr_full_omp_timed(dim, Iptr, J, v, diag, x, y, nrep, NULL);
+}
+
+/* the same, with the wall time of each product in ms[rep] (the full rows are
+   expanded first, outside the timed products) */
+int orc_hv_csr_full_omp_timed(size_t dim, const int32_t *Iptr, const int32_t *J, const double *v,
+                              const double *diag, const double *x, double *y, int nrep, double *ms) {
   /* expand lower CSR into full CSR */
   size_t nnz = (size_t)Iptr[dim];
   int32_t *cnt = (int32_t *)calloc(dim + 1, sizeof(int32_t));
@@ -605,12 +621,14 @@ int orc_hv_csr_full_omp(size_t dim, const int32_t *Iptr, const int32_t *J, const
       fc[pos[J[j]]] = (int32_t)i; fv[pos[J[j]]++] = v[j];
     }
   for (int rep = 0; rep < nrep; ++rep) {
+    const double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < dim; ++i) {
       double acc = diag[i] * x[i];
       for (int32_t j = cnt[i]; j < cnt[i + 1]; ++j) acc += fv[j] * x[fc[j]];
       y[i] += acc;
     }
+    if (ms) ms[rep] = 1e3 * (omp_get_wtime() - t0);
   }
   free(cnt); free(fc); free(fv); free(pos);
   return 0;
@@ -1350,6 +1368,16 @@ int orc_davidson(const orc_model *m, int Nup, int Ndw, int K, double accuracy, i
 /* ------------------------------------------------------------------------ */
 int orc_block_lanczos(const orc_model *m, int Nup, int Ndw, int p, const double *q0, int M0, int max_iter_bl,
                       double accur_deflation, double accur_bl, int *n_floors, double *A, double *B) {
+  return orc_block_lanczos_v(m, Nup, Ndw, p, q0, M0, max_iter_bl, accur_deflation, accur_bl, 1, n_floors, A, B);
+}
+
+/* qr = 1: blockLanczos (Lanczos.hpp:426-572), B_j upper triangular from
+   modified Gram-Schmidt; qr = 0: blockLanczosSVD (Lanczos.hpp:603-779), the
+   polar form: G = Z^T Z = V D V^T, B_j = V D^(1/2) V^T, Q_{j+1} = Z V D^(-1/2) V^T,
+   deflation when some D_m < accur_deflation^2, checked before the window moves
+   and before the convergence test (which then follows the move). */
+int orc_block_lanczos_v(const orc_model *m, int Nup, int Ndw, int p, const double *q0, int M0, int max_iter_bl,
+                        double accur_deflation, double accur_bl, int qr, int *n_floors, double *A, double *B) {
   happly H;
   int rc = happly_make(&H, m, Nup, Ndw);
   if (rc) return rc;
@@ -1386,21 +1414,51 @@ int orc_block_lanczos(const orc_model *m, int Nup, int Ndw, int p, const double 
     double *Bj = B + (size_t)j * pp;
     memset(Bj, 0, sizeof(double) * (size_t)pp);
     int singular = 0;
-    for (int l = 0; l < p; ++l) {
-      for (int k = 0; k < l; ++k) {
-        double z = vdot(dim, VEC(Z, k), VEC(Z, l));
-        Bj[k + l * p] = z;
-        vaxpy(dim, -z, VEC(Z, k), VEC(Z, l));
+    if (qr) {
+      for (int l = 0; l < p; ++l) {
+        for (int k = 0; k < l; ++k) {
+          double z = vdot(dim, VEC(Z, k), VEC(Z, l));
+          Bj[k + l * p] = z;
+          vaxpy(dim, -z, VEC(Z, k), VEC(Z, l));
+        }
+        double nrm = sqrt(vdot(dim, VEC(Z, l), VEC(Z, l)));
+        if (nrm < accur_deflation) {
+          Bj[l + l * p] = 0.0;
+          for (int ll = l; ll < p; ++ll) memset(VEC(Z, ll), 0, sizeof(double) * dim);
+          singular = 1;
+          break;
+        }
+        Bj[l + l * p] = nrm;
+        vscal(dim, 1.0 / nrm, VEC(Z, l));
       }
-      double nrm = sqrt(vdot(dim, VEC(Z, l), VEC(Z, l)));
-      if (nrm < accur_deflation) {
-        Bj[l + l * p] = 0.0;
-        for (int ll = l; ll < p; ++ll) memset(VEC(Z, ll), 0, sizeof(double) * dim);
-        singular = 1;
-        break;
+    } else {
+      double *G = (double *)malloc(sizeof(double) * (size_t)pp);
+      double *D = (double *)malloc(sizeof(double) * (size_t)p);
+      double *V = (double *)malloc(sizeof(double) * (size_t)pp);
+      for (int l = 0; l < p; ++l)
+        for (int k = 0; k <= l; ++k) G[k + l * p] = G[l + k * p] = vdot(dim, VEC(Z, k), VEC(Z, l));
+      orc_symeig(p, G, D, V);
+      for (int q = 0; q < p; ++q)
+        if (D[q] < accur_deflation * accur_deflation) singular = 1;
+      for (int r = 0; r < p; ++r)
+        for (int c = 0; c < p; ++c) {
+          double t = 0.0;
+          for (int q = 0; q < p; ++q) t += V[r + q * p] * sqrt(D[q] > 0.0 ? D[q] : 0.0) * V[c + q * p];
+          Bj[r + c * p] = t;
+        }
+      if (!singular) {
+        /* Q_{j+1} = Z Binv, written over Q_{j-1}, then the window moves */
+        memset(Qp, 0, sizeof(double) * (size_t)p * dim);
+        for (int l = 0; l < p; ++l)
+          for (int k = 0; k < p; ++k) {
+            double t = 0.0;
+            for (int q = 0; q < p; ++q) t += V[k + q * p] * V[l + q * p] / sqrt(D[q]);
+            vaxpy(dim, t, VEC(Z, k), VEC(Qp, l));
+          }
+        double *tmp = Qp; Qp = Qc; Qc = tmp;
       }
-      Bj[l + l * p] = nrm;
-      vscal(dim, 1.0 / nrm, VEC(Z, l));
+      free(G); free(D); free(V);
+      if (singular) break;
     }
     int conv = 0;
     if ((j + 1) >= 2 * p && (j + 1) % p == 0) {
@@ -1425,8 +1483,10 @@ int orc_block_lanczos(const orc_model *m, int Nup, int Ndw, int p, const double 
     }
     if (conv) break;
     if (singular) break;
-    memcpy(Qp, Qc, sizeof(double) * (size_t)p * dim);
-    memcpy(Qc, Z, sizeof(double) * (size_t)p * dim);
+    if (qr) {
+      memcpy(Qp, Qc, sizeof(double) * (size_t)p * dim);
+      memcpy(Qc, Z, sizeof(double) * (size_t)p * dim);
+    }
   }
 #undef VEC
   *n_floors = (j < M0) ? j + 1 : M0;

@@ -54,6 +54,8 @@ int orc_half_table(const orc_model *m, int t, int spin, int N,
 /* ---- Hamiltonian-vector products: y += H x ---- */
 int orc_hv_factorized(const orc_model *m, int Nup, int Ndw, const double *x, double *y);
 int orc_hv_factorized_omp(const orc_model *m, int Nup, int Ndw, const double *x, double *y, int nrep);
+int orc_hv_factorized_omp_timed(const orc_model *m, int Nup, int Ndw, const double *x, double *y, int nrep,
+                                double *ms);
 int orc_diag(const orc_model *m, int Nup, int Ndw, double *d);
 int orc_csr_build(const orc_model *m, int Nup, int Ndw, size_t *nnz,
                   int32_t *Iptr, int32_t *J, double *v, double *diag);
@@ -61,6 +63,8 @@ int orc_hv_csr(size_t dim, const int32_t *Iptr, const int32_t *J, const double *
                const double *diag, const double *x, double *y);
 int orc_hv_csr_full_omp(size_t dim, const int32_t *Iptr, const int32_t *J, const double *v,
                         const double *diag, const double *x, double *y, int nrep);
+int orc_hv_csr_full_omp_timed(size_t dim, const int32_t *Iptr, const int32_t *J, const double *v,
+                              const double *diag, const double *x, double *y, int nrep, double *ms);
 int orc_hv_otf(const orc_model *m, int Nup, int Ndw, const double *x, double *y);
 
 /* ---- c / c^dagger between sectors (model::create_or_destroy, factorized) ---- */
@@ -91,6 +95,8 @@ int orc_davidson(const orc_model *m, int Nup, int Ndw, int K, double accuracy, i
 int orc_block_lanczos(const orc_model *m, int Nup, int Ndw, int p, const double *q0,
                       int M0, int max_iter_bl, double accur_deflation, double accur_bl,
                       int *n_floors, double *A, double *B);
+int orc_block_lanczos_v(const orc_model *m, int Nup, int Ndw, int p, const double *q0, int M0, int max_iter_bl,
+                        double accur_deflation, double accur_bl, int qr, int *n_floors, double *A, double *B);
 int orc_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q,
                  int nz, const double *z, double *G);
 

@@ -122,6 +122,22 @@ SIGNATURES = {
     "ed_cf_eval": (C.c_int, [C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]),
     "ed_vdvh": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]),
     "ed_mcf_eval": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),
+    "ed_continued_fractions": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _ip]),
+    "ed_continued_fractions_pairs": (C.c_int, [C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_int, _dp, _dp, _ip]),
+    "ed_set_continued_fractions": (C.c_int, [C.c_int, C.c_double, C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                             _ip]),
+    "ed_combined_mcf": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip]),
+    "ed_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, _i64p]),
+    "ed_surrogate_destroy": (C.c_int, [C.c_int64]),
+    "ed_surrogate_n_params": (C.c_int, [C.c_int64, _i64p]),
+    "ed_surrogate_set_scaling": (C.c_int, [C.c_int64, _dp, _dp, _dp, _dp, _dp]),
+    "ed_surrogate_set_optimizer": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "ed_surrogate_get_weights": (C.c_int, [C.c_int64, _dp]),
+    "ed_surrogate_set_weights": (C.c_int, [C.c_int64, _dp]),
+    "ed_surrogate_predict": (C.c_int, [C.c_int64, C.c_int, _dp, _dp]),
+    "ed_surrogate_loss_grad": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, _dp, _dp]),
+    "ed_surrogate_train": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int64, _dp, _dp]),
+    "ed_surrogate_time": (C.c_int, [C.c_int64, C.c_int, C.c_int, _dp, _dp]),
     "ed_device_alloc": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
     "ed_device_free": (C.c_int, [C.c_void_p, C.c_int64]),
     "ed_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),

@@ -11,6 +11,7 @@
 #include "ed_common.hpp"
 #include "ed_batch.hpp"
 #include "ed_instance.hpp"
+#include "ed_surrogate.hpp"
 #include "ed_lattice.hpp"
 #include "ed_model.hpp"
 #include "ed_solvers.hpp"
@@ -49,6 +50,15 @@ int guard(F &&f) {
 std::mutex g_h_mu;
 std::map<int64_t, std::unique_ptr<Hamiltonian>> g_hams;
 std::atomic<int64_t> g_next_h{1};
+
+std::map<int64_t, std::unique_ptr<Surrogate>> g_surr;
+
+Surrogate &get_surr(int64_t h) {
+  std::lock_guard<std::mutex> l(g_h_mu);
+  auto it = g_surr.find(h);
+  if (it == g_surr.end()) fail(ED_ERR_NOT_FOUND, "unknown surrogate handle " + std::to_string(h));
+  return *it->second;
+}
 
 Hamiltonian &get_h(int64_t h) {
   std::lock_guard<std::mutex> l(g_h_mu);
@@ -130,10 +140,11 @@ int ed_get_global_int(const char *name, long *value) {
 // ---- model / operator / instance ----
 int ed_new_model(const char *name, int n_sites, int n_bath, const int *generators, int n_gen, int bath_irrep) {
   return guard([&] {
-    (void)generators;
-    (void)bath_irrep;
-    if (n_gen > 0) fail(ED_ERR_UNSUPPORTED, "cluster point-group generators are not supported (trivial group only)");
-    new_model(name, n_sites, n_bath);
+    need(n_gen >= 0 && (n_gen == 0 || generators), "ed_new_model: bad generators");
+    const int L = n_sites + n_bath;
+    std::vector<std::vector<int>> gens(n_gen);
+    for (int j = 0; j < n_gen; ++j) gens[j].assign(generators + (size_t)j * L, generators + (size_t)(j + 1) * L);
+    new_model(name, n_sites, n_bath, gens, bath_irrep != 0);
   });
 }
 
@@ -189,8 +200,35 @@ int ed_green_function_dimension(int label, int *d) {
 
 int ed_green_function(int label, int nz, const double *z, int spin_down, int blocks, double *out) {
   return guard([&] {
-    (void)blocks;  // trivial group: block form == site form
-    get_instance(label).green_function(nz, z, spin_down != 0, out);
+    Instance &I = get_instance(label);
+    I.green_function(nz, z, spin_down != 0, out);
+    const PointGroup &G = I.model->group;
+    if (!blocks || G.g == 1) return;  // trivial group: block form == site form
+    // the symmetric-orbital basis (Green_function with blocks, qcm_ED.cpp:220-240):
+    // G_sym = S G S^T, the blocks of each representation on the diagonal; the
+    // couplings between representations vanish by symmetry and are set to 0
+    const int d = I.gf_dim();
+    std::vector<int> rep(d);
+    for (int r = 0; r < G.g; ++r)
+      for (int a = 0; a < G.irrep_dim[r]; ++a) rep[G.irrep_off[r] + a] = r;
+    std::vector<cplx> Gs((size_t)d * d), T((size_t)d * d);
+    for (int iz = 0; iz < nz; ++iz) {
+      cplx *Gz = reinterpret_cast<cplx *>(out) + (size_t)iz * d * d;  // column-major d x d
+      for (int a = 0; a < d; ++a)
+        for (int j = 0; j < d; ++j) {
+          cplx t = 0.0;
+          for (int i = 0; i < d; ++i) t += G.S[(size_t)a * d + i] * Gz[i + (size_t)j * d];
+          T[a + (size_t)j * d] = t;
+        }
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) {
+          cplx t = 0.0;
+          if (rep[a] == rep[b])
+            for (int j = 0; j < d; ++j) t += T[a + (size_t)j * d] * G.S[(size_t)b * d + j];
+          Gs[a + (size_t)b * d] = t;
+        }
+      std::copy(Gs.begin(), Gs.end(), Gz);
+    }
   });
 }
 
@@ -618,7 +656,8 @@ int ed_hamiltonian_create(int label, const char *sector, char format, int64_t *h
   return guard([&] {
     Instance &I = get_instance(label);
     Sector s = Sector::parse(sector);
-    if (!sector_valid(I.model->L, s)) fail(ED_ERR_SECTOR, std::string("sector ") + sector + " is not valid");
+    if (!sector_valid(I.model->L, s, I.model->group.g))
+      fail(ED_ERR_SECTOR, std::string("sector ") + sector + " is not valid");
     auto H = make_hamiltonian(I.terms(), s, format ? format : I.format);
     const int64_t id = g_next_h++;
     std::lock_guard<std::mutex> l(g_h_mu);
@@ -833,7 +872,7 @@ int ed_block_lanczos(int64_t h, int p, const double *q0, int M0, int cap, double
     }
     for (auto &b : bufs) ptrs.push_back(b.p);
     std::vector<double> Av, Bv;
-    const int n = block_lanczos(H, ptrs, M0, Av, Bv);
+    const int n = block_lanczos(H, ptrs, M0, Av, Bv, params().getb("block_Lanczos_QR"));
     *nfl = n;
     if (n > cap)
       fail(ED_ERR_INVALID, "ed_block_lanczos: " + std::to_string(n) + " floors exceed cap " + std::to_string(cap) +
@@ -928,6 +967,160 @@ int ed_device_sync(void) {
 }
 int ed_release_cache(void) {
   return guard([&] { dev_release_cache(); });
+}
+
+int ed_continued_fractions(int label, int spin_down, int state, int cap, double *a, double *b, int *len,
+                           int *nfrac) {
+  return guard([&] {
+    need(cap >= 0 && nfrac != nullptr, "ed_continued_fractions: bad arguments");
+    Instance &I = get_instance(label);
+    std::vector<std::vector<double>> av, bv;
+    cf_export(I, spin_down ? 1 : 0, state, av, bv);
+    *nfrac = (int)av.size();
+    if (!a || !b || !len) return;
+    for (size_t f = 0; f < av.size(); ++f) {
+      len[f] = (int)av[f].size();
+      for (int k = 0; k < cap; ++k) {
+        a[f * cap + k] = k < (int)av[f].size() ? av[f][k] : 0.0;
+        b[f * cap + k] = k < (int)bv[f].size() ? bv[f][k] : 0.0;
+      }
+    }
+  });
+}
+
+int ed_continued_fractions_pairs(int label, int spin_down, int npairs, const int *o1, const int *o2, int cap,
+                                 double *a, double *b, int *len) {
+  return guard([&] {
+    need(npairs >= 0 && cap >= 1 && (npairs == 0 || (o1 && o2 && a && b && len)),
+         "ed_continued_fractions_pairs: bad arguments");
+    Instance &I = get_instance(label);
+    std::vector<std::pair<int, int>> pr;
+    for (int k = 0; k < npairs; ++k) pr.push_back({o1[k], o2[k]});
+    std::vector<std::vector<double>> av, bv;
+    cf_pairs(I, spin_down ? 1 : 0, pr, cap, av, bv);
+    for (size_t f = 0; f < av.size(); ++f) {
+      len[f] = (int)av[f].size();
+      for (int k = 0; k < cap; ++k) {
+        a[f * cap + k] = k < (int)av[f].size() ? av[f][k] : 0.0;
+        b[f * cap + k] = k < (int)bv[f].size() ? bv[f][k] : 0.0;
+      }
+    }
+  });
+}
+
+int ed_set_continued_fractions(int label, double E0, const char *sector, int spin_down, int nfrac, int cap,
+                               const double *a, const double *b, const int *len) {
+  return guard([&] {
+    need(sector && a && b && len && cap >= 0 && nfrac >= 0, "ed_set_continued_fractions: bad arguments");
+    Instance &I = get_instance(label);
+    std::vector<std::vector<double>> av(nfrac), bv(nfrac);
+    for (int f = 0; f < nfrac; ++f) {
+      need(len[f] >= 0 && len[f] <= cap, "ed_set_continued_fractions: a length exceeds cap");
+      av[f].assign(a + (size_t)f * cap, a + (size_t)f * cap + len[f]);
+      bv[f].assign(b + (size_t)f * cap, b + (size_t)f * cap + len[f]);
+    }
+    cf_import(I, E0, Sector::parse(sector), spin_down ? 1 : 0, av, bv);
+  });
+}
+
+int ed_combined_mcf(int label, int spin_down, int cap, double *W, double *A, double *B, int *p, int *nfl) {
+  return guard([&] {
+    need(p && nfl && cap >= 0, "ed_combined_mcf: bad arguments");
+    Instance &I = get_instance(label);
+    if (I.gf_method != 'M')
+      fail(ED_ERR_INVALID, "get_combined_mcf: the instance at label " + std::to_string(label) +
+                               " does not use the MCF format (GF_method must be 'M').");
+    if (!params().getb("combine_mcf"))
+      fail(ED_ERR_INVALID, "get_combined_mcf: the combined MCF is not built unless the global parameter "
+                           "'combine_mcf' is set to True.");
+    if (!I.gf_solved) I.green_function_solve();
+    if (I.states.empty()) fail(ED_ERR_INVALID, "get_combined_mcf: no valid ground state found.");
+    const GFRep::MCF &m = I.states[0]->gf[I.gf_spin(spin_down != 0)].comb;
+    if (m.nfl == 0) fail(ED_ERR_INVALID, "get_combined_mcf: combined MCF is empty.");
+    *p = m.p;
+    *nfl = m.nfl;
+    if (m.nfl > cap || !W || !A || !B) return;
+    std::copy(m.W.begin(), m.W.end(), W);
+    std::copy(m.A.begin(), m.A.end(), A);
+    std::copy(m.B.begin(), m.B.end(), B);
+  });
+}
+
+// ---- learned surrogate ----
+int ed_surrogate_create(int n_in, int n_out, int width, int depth, int64_t seed, int64_t *handle) {
+  return guard([&] {
+    need(handle != nullptr, "ed_surrogate_create: null handle");
+    auto S = std::make_unique<Surrogate>(n_in, n_out, width, depth, (uint64_t)seed);
+    const int64_t h = g_next_h.fetch_add(1);
+    std::lock_guard<std::mutex> l(g_h_mu);
+    g_surr[h] = std::move(S);
+    *handle = h;
+  });
+}
+int ed_surrogate_destroy(int64_t handle) {
+  return guard([&] {
+    std::lock_guard<std::mutex> l(g_h_mu);
+    if (!g_surr.erase(handle)) fail(ED_ERR_NOT_FOUND, "unknown surrogate handle " + std::to_string(handle));
+  });
+}
+int ed_surrogate_n_params(int64_t handle, int64_t *n) {
+  return guard([&] {
+    need(n != nullptr, "ed_surrogate_n_params: null output");
+    *n = (int64_t)get_surr(handle).n_params();
+  });
+}
+int ed_surrogate_set_scaling(int64_t handle, const double *x_mean, const double *x_std, const double *y_mean,
+                             const double *y_std, const double *loss_weight) {
+  return guard([&] { get_surr(handle).set_scaling(x_mean, x_std, y_mean, y_std, loss_weight); });
+}
+int ed_surrogate_set_optimizer(int64_t handle, double lr, double beta1, double beta2, double eps) {
+  return guard([&] {
+    need(lr > 0.0 && beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0 && eps > 0.0,
+         "ed_surrogate_set_optimizer: bad hyper-parameters");
+    Surrogate &S = get_surr(handle);
+    S.lr = lr;
+    S.beta1 = beta1;
+    S.beta2 = beta2;
+    S.eps = eps;
+  });
+}
+int ed_surrogate_get_weights(int64_t handle, double *w) {
+  return guard([&] {
+    need(w != nullptr, "ed_surrogate_get_weights: null output");
+    get_surr(handle).get_weights(w);
+  });
+}
+int ed_surrogate_set_weights(int64_t handle, const double *w) {
+  return guard([&] {
+    need(w != nullptr, "ed_surrogate_set_weights: null input");
+    get_surr(handle).set_weights(w);
+  });
+}
+int ed_surrogate_predict(int64_t handle, int n, const double *x, double *y) {
+  return guard([&] {
+    need(n >= 0 && (n == 0 || (x && y)), "ed_surrogate_predict: bad arguments");
+    if (n) get_surr(handle).predict(n, x, y);
+  });
+}
+int ed_surrogate_loss_grad(int64_t handle, int n, const double *x, const double *y, double *loss, double *grad) {
+  return guard([&] {
+    need(n >= 1 && x && y && loss, "ed_surrogate_loss_grad: bad arguments");
+    *loss = get_surr(handle).loss_grad_host(n, x, y, grad);
+  });
+}
+int ed_surrogate_train(int64_t handle, int n, const double *x, const double *y, int epochs, int batch,
+                       int64_t seed, double *epoch_loss, double *ms_per_step) {
+  return guard([&] {
+    need(n >= 1 && x && y && epochs >= 0 && batch >= 1, "ed_surrogate_train: bad arguments");
+    const std::vector<double> h = get_surr(handle).train(n, x, y, epochs, batch, (uint64_t)seed, ms_per_step);
+    if (epoch_loss) std::copy(h.begin(), h.end(), epoch_loss);
+  });
+}
+int ed_surrogate_time(int64_t handle, int batch, int nrep, double *ms_forward, double *ms_forward_backward) {
+  return guard([&] {
+    need(batch >= 1 && nrep >= 1 && ms_forward && ms_forward_backward, "ed_surrogate_time: bad arguments");
+    get_surr(handle).time(batch, nrep, ms_forward, ms_forward_backward);
+  });
 }
 
 }  // extern "C"

@@ -4,12 +4,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <future>
 #include <iomanip>
 #include <mutex>
 #include <random>
 #include <sstream>
 
+#include "ed_dense.hpp"
 #include "ed_solvers.hpp"
 
 namespace ed {
@@ -34,8 +36,8 @@ void erase_instance(int label) {
   g_instances.erase(label);
 }
 
-// src_ed/symmetry_group.cpp:515-533 (g = 1)
-bool sector_valid(int L, const Sector &s) {
+// src_ed/symmetry_group.cpp:515-533; g: order of the model's point group
+bool sector_valid(int L, const Sector &s, int g) {
   if (s.N < Sector::odd) {
     if (s.N < 0 || s.N > 2 * L) return false;
     if (s.S < Sector::odd) {
@@ -43,12 +45,15 @@ bool sector_valid(int L, const Sector &s) {
     }
   }
   if (s.S < Sector::odd && (s.S > L || s.S < -L)) return false;
-  return s.irrep == 0;
+  return s.irrep >= 0 && s.irrep < g;
 }
 
-// src_ed/symmetry_group.cpp:540-558 (g = 1)
+// src_ed/symmetry_group.cpp:540-558.  The Green function is built here in the
+// full target sector (N +- 1, S +- 1) whatever the representation of the
+// state: the target carries no irrep label (0).
 Sector shift_sector(const Sector &s0, int pm, int spin) {
   Sector s = s0;
+  s.irrep = 0;
   if (s.N == Sector::odd) s.N = Sector::even;
   else if (s.N == Sector::even) s.N = Sector::odd;
   else s.N += pm;
@@ -91,12 +96,43 @@ void new_instance(const std::string &model, const std::map<std::string, double> 
     if (op.spin_asym && kv.second != 0.0) asym = true;
   }
   I->mixing = asym ? 4 : 0;
+  if (M.group.g > 1) {
+    // every operator in use must commute with the group (the reference builds
+    // its Hamiltonian in the symmetric basis on that assumption)
+    const PointGroup &G = M.group;
+    auto image = [&](int a, int k, int &sg) {
+      const int spin = a / M.L, o = a % M.L;
+      int o2 = o, ph = 0;
+      if (o < G.np) o2 = G.e[k][o];
+      else ph = G.e[k][o];
+      sg = std::cos(2.0 * M_PI * (double)(ph % G.g) / G.g) < 0.0 ? -1 : 1;
+      return o2 + spin * M.L;
+    };
+    for (const auto &kv : I->value) {
+      if (kv.second == 0.0) continue;
+      const Operator &op = M.ops.at(kv.first);
+      std::map<std::pair<int, int>, double> el;
+      for (const auto &x : op.el) el[{x.r, x.c}] += x.v;
+      for (int k = 1; k < G.g; ++k)
+        for (const auto &x : op.el) {
+          int s1 = 1, s2 = 1;
+          const int r2 = image(x.r, k, s1), c2 = image(x.c, k, s2);
+          const double vx = el.at({x.r, x.c});
+          const double want = vx * (op.kind == OP_ONE_BODY ? s1 * s2 : 1);
+          auto it = el.find({r2, c2});
+          if (it == el.end() && op.kind != OP_ONE_BODY) it = el.find({c2, r2});
+          const double got = it == el.end() ? 0.0 : it->second;
+          if (std::fabs(got - want) > 1e-10 * std::max(1.0, std::fabs(vx)))
+            fail(ED_ERR_INVALID, "operator " + op.name + " is not invariant under the cluster's point group");
+        }
+    }
+  }
   std::stringstream ss(sectors);
   std::string item;
   while (std::getline(ss, item, '/')) {
     if (item.empty()) continue;
     Sector s = Sector::parse(item);
-    if (!sector_valid(M.L, s)) fail(ED_ERR_SECTOR, "sector " + item + " is not valid");
+    if (!sector_valid(M.L, s, M.group.g)) fail(ED_ERR_SECTOR, "sector " + item + " is not valid");
     if (s.N >= Sector::odd || s.S >= Sector::odd)
       fail(ED_ERR_UNSUPPORTED, "sector string " + s.name() + " defines a non conserved quantity, not supported");
     if (s.S && I->mixing == 0) I->mixing = 4;
@@ -239,6 +275,14 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     auto H = make_hamiltonian(T, s, format);
     long ns = std::max<long>(1, std::min<long>(n_states, (long)dim));
     char meth = (method == 'L' && ns > 1) ? 'D' : method;
+    // a point group: the sector's states are those of the full (N, S) sector in
+    // representation s.irrep
+    std::unique_ptr<GroupProjector> proj;
+    if (model->group.g > 1) {
+      proj = std::make_unique<GroupProjector>(model->group, s.irrep, s.Nup(), s.Ndw());
+      if (meth == 'D')
+        fail(ED_ERR_UNSUPPORTED, "several states per sector (Davidson) with a point group are not supported");
+    }
     std::vector<double> ev;
     std::vector<DevBuf<double>> vecs;
     vlog("ground state: sector %s dim %zu, Hamiltonian ready", s.name().c_str(), dim);
@@ -261,9 +305,20 @@ std::pair<double, std::string> Instance::ground_state_solve() {
         random_normal(dim, (uint64_t)seed, d0.p, stream());
         scale(dim, d0.p, 1.0 / std::sqrt(ddot(dim, d0.p, d0.p)), stream());
       }
+      if (proj) {
+        DevBuf<double> t(dim);
+        proj->apply(d0.p, t.p);
+        const double nrm = std::sqrt(ddot(dim, t.p, t.p));
+        if (!(nrm > 1e-8)) {
+          vlog("sector %s holds no state of its representation", s.name().c_str());
+          continue;
+        }
+        scale(dim, t.p, 1.0 / nrm, stream());
+        d0 = std::move(t);
+      }
       vlog("random start vector ready");
       vecs.emplace_back(dim);
-      LanczosGS R = lanczos_gs(*H, d0.p, vecs[0].p);
+      LanczosGS R = lanczos_gs(*H, d0.p, vecs[0].p, proj.get());
       ev.push_back(R.E0);
       vlog("Lanczos: %d iterations, E0 = %.15g", R.niter, R.E0);
     } else {
@@ -428,6 +483,191 @@ static double qnorm_error(int L, const std::vector<double> &V) {
   return tot;
 }
 
+// ---------------------------------------------------------------------------
+// combined matrix continued fraction (combine_via_lanczos,
+// src_ed/matrix_continued_fraction.hpp:443-505): block Lanczos, in the QR or
+// the polar form, on the block-tridiagonal operator T_e (+) T_h of the electron
+// and hole fractions (dimension (M_e + M_h) p, on the host), from the block
+// whose level-0 components are the two weights; G = W^T F_0 W then gives
+// G_e + G_h^T in one fraction.  Both fractions need the same block size, as
+// there (a deflated start block leaves them separate here).
+// ---------------------------------------------------------------------------
+namespace {
+// y = T x for the block-tridiagonal matrix of one fraction: diagonal A_j,
+// T(j+1, j) = B_j, T(j, j+1) = B_j^T (p x p column-major floors)
+void mcf_apply(const GFRep::MCF &m, const double *x, double *y) {
+  const int p = m.p, pp = p * p;
+  for (int j = 0; j < m.nfl; ++j) {
+    double *yj = y + (size_t)j * p;
+    const double *Aj = m.A.data() + (size_t)j * pp;
+    for (int r = 0; r < p; ++r) {
+      double t = 0.0;
+      for (int c = 0; c < p; ++c) t += Aj[r + c * p] * x[(size_t)j * p + c];
+      if (j > 0) {
+        const double *Bm = m.B.data() + (size_t)(j - 1) * pp;
+        for (int c = 0; c < p; ++c) t += Bm[r + c * p] * x[(size_t)(j - 1) * p + c];
+      }
+      if (j + 1 < m.nfl) {
+        const double *Bj = m.B.data() + (size_t)j * pp;
+        for (int c = 0; c < p; ++c) t += Bj[c + r * p] * x[(size_t)(j + 1) * p + c];
+      }
+      yj[r] = t;
+    }
+  }
+}
+
+double hdot(size_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+// the block Lanczos recurrence of block_lanczos() on a host operator
+int host_block_lanczos(size_t N, const std::function<void(const double *, double *)> &op,
+                       std::vector<std::vector<double>> q, int M0, bool qr, std::vector<double> &A,
+                       std::vector<double> &B) {
+  Params &Pm = params();
+  const double accur_deflation = Pm.getd("accur_deflation");
+  const double accur_bl = std::max(0.0, Pm.getd("accur_band_lanczos"));
+  M0 = std::min(M0, (int)Pm.geti("max_iter_BL"));
+  const int p = (int)q.size(), pp = p * p;
+  A.clear();
+  B.clear();
+  for (int l = 0; l < p; ++l) {
+    for (int k = 0; k < l; ++k) {
+      const double z = hdot(N, q[k].data(), q[l].data());
+      for (size_t i = 0; i < N; ++i) q[l][i] -= z * q[k][i];
+    }
+    const double nrm = std::sqrt(hdot(N, q[l].data(), q[l].data()));
+    if (nrm < accur_deflation) fail(ED_ERR_INVALID, "combined MCF: starting block is rank-deficient");
+    for (auto &v : q[l]) v /= nrm;
+  }
+  std::vector<std::vector<double>> Qp(p, std::vector<double>(N, 0.0)), Z(p, std::vector<double>(N));
+  double ev_old = 1e12;
+  int j;
+  for (j = 0; j < M0; ++j) {
+    for (int k = 0; k < p; ++k) op(q[k].data(), Z[k].data());
+    std::vector<double> Aj(pp);
+    for (int l = 0; l < p; ++l)
+      for (int k = 0; k <= l; ++k) Aj[k + l * p] = Aj[l + k * p] = hdot(N, q[k].data(), Z[l].data());
+    A.insert(A.end(), Aj.begin(), Aj.end());
+    for (int l = 0; l < p; ++l)
+      for (int k = 0; k < p; ++k) {
+        const double c = Aj[k + l * p];
+        const double d = j > 0 ? B[(size_t)(j - 1) * pp + l + k * p] : 0.0;
+        for (size_t i = 0; i < N; ++i) Z[l][i] -= c * q[k][i] + d * Qp[k][i];
+      }
+    std::vector<double> Bj(pp, 0.0);
+    bool singular = false;
+    if (qr) {
+      for (int l = 0; l < p && !singular; ++l) {
+        for (int k = 0; k < l; ++k) {
+          const double z = hdot(N, Z[k].data(), Z[l].data());
+          Bj[k + l * p] = z;
+          for (size_t i = 0; i < N; ++i) Z[l][i] -= z * Z[k][i];
+        }
+        const double nrm = std::sqrt(hdot(N, Z[l].data(), Z[l].data()));
+        if (nrm < accur_deflation) {
+          for (int ll = l; ll < p; ++ll) std::fill(Z[ll].begin(), Z[ll].end(), 0.0);
+          singular = true;
+          break;
+        }
+        Bj[l + l * p] = nrm;
+        for (auto &v : Z[l]) v /= nrm;
+      }
+      B.insert(B.end(), Bj.begin(), Bj.end());
+    } else {
+      std::vector<double> G(pp), D(p), V(pp);
+      for (int l = 0; l < p; ++l)
+        for (int k = 0; k <= l; ++k) G[k + l * p] = G[l + k * p] = hdot(N, Z[k].data(), Z[l].data());
+      symeig(p, G.data(), D.data(), V.data());
+      for (int m = 0; m < p; ++m)
+        if (D[m] < accur_deflation * accur_deflation) singular = true;
+      for (int r = 0; r < p; ++r)
+        for (int c = 0; c < p; ++c) {
+          double t = 0.0;
+          for (int m = 0; m < p; ++m) t += V[r + m * p] * std::sqrt(std::max(D[m], 0.0)) * V[c + m * p];
+          Bj[r + c * p] = t;
+        }
+      B.insert(B.end(), Bj.begin(), Bj.end());
+      if (singular) { ++j; break; }
+      std::vector<std::vector<double>> Qn(p, std::vector<double>(N, 0.0));
+      for (int l = 0; l < p; ++l)
+        for (int k = 0; k < p; ++k) {
+          double t = 0.0;
+          for (int m = 0; m < p; ++m) t += V[k + m * p] * V[l + m * p] / std::sqrt(D[m]);
+          for (size_t i = 0; i < N; ++i) Qn[l][i] += t * Z[k][i];
+        }
+      Qp.swap(q);
+      q.swap(Qn);
+    }
+    bool conv = false;
+    if ((j + 1) >= 2 * p && (j + 1) % p == 0) {
+      const int sz = (j + 1) * p;
+      std::vector<double> Tm((size_t)sz * sz, 0.0);
+      for (int jj = 0; jj <= j; ++jj)
+        for (int r = 0; r < p; ++r)
+          for (int c = 0; c < p; ++c) Tm[(jj * p + r) + (size_t)(jj * p + c) * sz] = A[(size_t)jj * pp + r + c * p];
+      for (int jj = 0; jj < j; ++jj)
+        for (int r = 0; r < p; ++r)
+          for (int c = 0; c < p; ++c) {
+            const double bv = B[(size_t)jj * pp + r + c * p];
+            Tm[((jj + 1) * p + r) + (size_t)(jj * p + c) * sz] = bv;
+            Tm[(jj * p + c) + (size_t)((jj + 1) * p + r) * sz] = bv;
+          }
+      const double e0 = band_kth_eigenvalue(sz, 2 * p, Tm.data(), 0);
+      conv = std::fabs(e0 - ev_old) < accur_bl;
+      ev_old = e0;
+    }
+    if (conv || singular) { ++j; break; }
+    if (qr) {
+      Qp.swap(q);
+      q.swap(Z);
+      Z.assign(p, std::vector<double>(N, 0.0));
+    }
+  }
+  return (int)(A.size() / pp);
+}
+}  // namespace
+
+GFRep::MCF combine_mcf(const GFRep::MCF &e, const GFRep::MCF &h, int n, bool qr) {
+  GFRep::MCF out;
+  if (e.nfl == 0 || h.nfl == 0 || e.p != h.p) return out;
+  const int p = e.p;
+  const size_t N = (size_t)(e.nfl + h.nfl) * p, Ne = (size_t)e.nfl * p;
+  // start block: column i of W_e at the electron part's level 0, of W_h at the hole part's
+  std::vector<std::vector<double>> phi(n, std::vector<double>(N, 0.0));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < p; ++k) {
+      phi[i][k] = e.W[k + (size_t)i * p];
+      phi[i][Ne + k] = h.W[k + (size_t)i * p];
+    }
+  // weight: the upper-triangular factor of the start block (modified Gram-Schmidt)
+  std::vector<double> W((size_t)n * n, 0.0);
+  std::vector<std::vector<double>> q(phi);
+  int nd = n;
+  for (int l = 0; l < n; ++l) {
+    for (int k = 0; k < l; ++k) {
+      const double z = hdot(N, q[k].data(), q[l].data());
+      W[k + (size_t)l * n] = z;
+      for (size_t i = 0; i < N; ++i) q[l][i] -= z * q[k][i];
+    }
+    const double nrm = std::sqrt(hdot(N, q[l].data(), q[l].data()));
+    if (nrm < params().getd("accur_deflation")) { nd = l; break; }
+    W[l + (size_t)l * n] = nrm;
+    for (auto &v : q[l]) v /= nrm;
+  }
+  if (nd < n) return out;  // dependent weights: the two fractions stay separate
+  auto op = [&](const double *x, double *y) {
+    mcf_apply(e, x, y);
+    mcf_apply(h, x + Ne, y + Ne);
+  };
+  out.p = n;
+  out.nfl = host_block_lanczos(N, op, phi, e.nfl + h.nfl, qr, out.A, out.B);
+  out.W = W;
+  return out;
+}
+
 static void build_gf(Instance &I, State &st, int spin) {
   const int L = I.model->L, ns = I.model->n_sites;
   const TermList T = I.terms();
@@ -538,7 +778,7 @@ static void build_gf(Instance &I, State &st, int spin) {
       for (int k = 0; k < pa; ++k) q.push_back(phi[k].p);
       // at least one floor (14 p log(dt) is 0 for a one-state sector)
       const int M0 = std::max(1, (int)(14 * pa * std::log(1.0 * dt)));
-      mc.nfl = block_lanczos(*H, q, M0, mc.A, mc.B);
+      mc.nfl = block_lanczos(*H, q, M0, mc.A, mc.B, params().getb("block_Lanczos_QR"));
       const int pp = pa * pa;
       for (int j = 0; j < mc.nfl; ++j) {
         double *Aj = mc.A.data() + (size_t)j * pp;
@@ -551,6 +791,8 @@ static void build_gf(Instance &I, State &st, int spin) {
       }
       if (pm == 1) G.me = mc;
       else G.mh = mc;
+      if (pm == 1 && params().getb("combine_mcf"))
+        G.comb = combine_mcf(G.me, G.mh, ns, params().getb("block_Lanczos_QR"));
     } else {
       fail(ED_ERR_INVALID, std::string("GF_method must be L, F or M (got ") + G.kind + ")");
     }
@@ -582,6 +824,140 @@ void Instance::green_function_solve() {
     if (mixing & 4) build_gf(*this, *s, 1);
   }
   gf_solved = true;
+}
+
+// ---------------------------------------------------------------------------
+// continued fractions out of / into an instance (the learned surrogate's
+// targets and predictions)
+// ---------------------------------------------------------------------------
+void cf_export(Instance &I, int spin, int state, std::vector<std::vector<double>> &a,
+               std::vector<std::vector<double>> &b) {
+  if (!I.gf_solved) I.green_function_solve();
+  if (state < 0 || state >= (int)I.states.size()) fail(ED_ERR_INVALID, "cf_export: no such low-energy state");
+  const GFRep &G = I.states[state]->gf[I.gf_spin(spin == 1)];
+  if (G.kind != 'F') fail(ED_ERR_UNSUPPORTED, "continued fractions exist for GF_method 'F' only");
+  const int np = G.n * (G.n + 1) / 2;
+  a.assign(2 * np, {});
+  b.assign(2 * np, {});
+  if (G.store) {
+    const CfStore &st = *G.store;
+    sync();  // the store is written on the library stream
+    std::vector<int> off(st.nfrac + 1);
+    ED_HIP(hipMemcpy(off.data(), st.off.p, off.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int f = 0; f < 2 * np; ++f) {
+      const int g = G.frac0 + f;
+      const int n = off[g + 1] - off[g];
+      a[f].resize(n);
+      b[f].resize(n);
+      if (n) {
+        ED_HIP(hipMemcpy(a[f].data(), st.A.p + off[g], n * sizeof(double), hipMemcpyDeviceToHost));
+        ED_HIP(hipMemcpy(b[f].data(), st.B.p + off[g], n * sizeof(double), hipMemcpyDeviceToHost));
+      }
+    }
+    return;
+  }
+  for (int q = 0; q < np; ++q)
+    for (int eh = 0; eh < 2; ++eh) {
+      const auto &av = eh ? G.ha[q] : G.ea[q];
+      const auto &bv = eh ? G.hb[q] : G.eb[q];
+      a[2 * q + eh] = av;
+      b[2 * q + eh].assign(bv.begin(), bv.begin() + std::min(bv.size(), av.size()));
+    }
+}
+
+void cf_pairs(Instance &I, int spin, const std::vector<std::pair<int, int>> &pairs, int cap,
+              std::vector<std::vector<double>> &a, std::vector<std::vector<double>> &b) {
+  if (!I.gs_solved) I.ground_state_solve();
+  if (I.states.empty() || !I.states[0]->psi.p) fail(ED_ERR_INVALID, "cf_pairs needs the ground-state vector");
+  const int L = I.model->L, ns = I.model->n_sites;
+  for (const auto &pr : pairs)
+    if (pr.first < 0 || pr.first >= ns || pr.second < 0 || pr.second >= ns)
+      fail(ED_ERR_INVALID, "cf_pairs: orbital outside the cluster");
+  const int sp = I.gf_spin(spin == 1);
+  State &st = *I.states[0];
+  const TermList T = I.terms();
+  a.assign(2 * pairs.size(), {});
+  b.assign(2 * pairs.size(), {});
+  for (int pm = -1; pm <= 1; pm += 2) {
+    check_cancel();
+    const Sector tgt = shift_sector(st.sec, pm, sp);
+    if (!sector_valid(L, tgt)) continue;
+    const size_t dt = sector_dim(L, tgt);
+    if (dt == 0) continue;
+    auto H = make_hamiltonian(T, tgt, I.format);
+    std::map<int, DevBuf<double>> phi;
+    for (const auto &pr : pairs)
+      for (int o : {pr.first, pr.second})
+        if (!phi.count(o)) {
+          DevBuf<double> v(dt);
+          fill(dt, v.p, 0.0, stream());
+          cdag_apply(L, pm, o + sp * L, st.sec, st.psi.p, 1.0, v.p);
+          phi.emplace(o, std::move(v));
+        }
+    DevBuf<double> psi(dt);
+    for (size_t k = 0; k < pairs.size(); ++k) {
+      const int o1 = pairs[k].first, o2 = pairs[k].second;
+      std::vector<const double *> src{phi.at(o1).p};
+      std::vector<double> cf{1.0};
+      if (o2 != o1) { src.push_back(phi.at(o2).p); cf.push_back(1.0); }
+      double *pp = partials_acquire();
+      const int nb = combine(dt, psi.p, 0.0, (int)src.size(), src.data(), cf.data(), pp, nullptr, nullptr, stream());
+      const double nrm = sum_partials(pp, nb);
+      if (!(nrm > 1e-14)) continue;
+      scale(dt, psi.p, 1.0 / std::sqrt(nrm), stream());
+      std::vector<double> av, bv;
+      lanczos_green(*H, psi.p, av, bv, cap);
+      for (auto &x : bv) x *= x;
+      bv[0] = nrm * st.weight;
+      if (pm == 1) for (auto &x : av) x -= st.energy;
+      else for (auto &x : av) x = st.energy - x;
+      bv.resize(av.size());
+      const int f = 2 * (int)k + (pm == 1 ? 0 : 1);
+      a[f] = std::move(av);
+      b[f] = std::move(bv);
+    }
+  }
+}
+
+void cf_import(Instance &I, double E0, const Sector &sec, int spin, const std::vector<std::vector<double>> &a,
+               const std::vector<std::vector<double>> &b) {
+  const int n = I.gf_dim(), np = n * (n + 1) / 2;
+  if ((int)a.size() != 2 * np || (int)b.size() != 2 * np)
+    fail(ED_ERR_INVALID, "cf_import: expected " + std::to_string(2 * np) + " fractions");
+  if (!sector_valid(I.model->L, sec)) fail(ED_ERR_SECTOR, "cf_import: invalid sector " + sec.name());
+  const int sp = I.gf_spin(spin == 1);
+  const bool fresh = I.states.size() != 1 || I.states[0]->sec.name() != sec.name() || I.states[0]->energy != E0 ||
+                     I.gf_method != 'F' || !I.gf_solved;
+  if (fresh) {
+    auto st = std::make_unique<State>();
+    st->sec = sec;
+    st->energy = E0;
+    st->weight = 1.0;
+    I.states.clear();
+    I.states.push_back(std::move(st));
+  }
+  GFRep G;
+  G.kind = 'F';
+  G.n = n;
+  G.ea.assign(np, {});
+  G.eb.assign(np, {});
+  G.ha.assign(np, {});
+  G.hb.assign(np, {});
+  for (int q = 0; q < np; ++q)
+    for (int eh = 0; eh < 2; ++eh) {
+      const auto &av = a[2 * q + eh], &bv = b[2 * q + eh];
+      if (av.size() != bv.size()) fail(ED_ERR_INVALID, "cf_import: a and b of one fraction differ in length");
+      (eh ? G.ha[q] : G.ea[q]) = av;
+      (eh ? G.hb[q] : G.eb[q]) = bv;
+    }
+  I.states[0]->gf[sp] = std::move(G);
+  I.gf_method = 'F';
+  I.GS_energy = E0;
+  I.E0 = E0;
+  I.averages.clear();
+  I.averages_done = false;
+  I.gs_solved = true;
+  I.gf_solved = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -767,9 +1143,10 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
       std::vector<double> h = dG.to_host();
       for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
     } else if (G.kind == 'M') {
-      for (int eh = 0; eh < 2; ++eh) {
-        const GFRep::MCF &mc = eh ? G.mh : G.me;
-        if (mc.nfl == 0) continue;
+      for (int eh = 0; eh < 3; ++eh) {
+        // the combined fraction when it exists, otherwise electron and hole
+        const GFRep::MCF &mc = eh == 2 ? G.comb : (eh ? G.mh : G.me);
+        if (mc.nfl == 0 || (G.comb.nfl > 0) != (eh == 2)) continue;
         DevBuf<double> dA, dB, dW, dG((size_t)nz * n * n * 2);
         dA.upload(mc.A);
         dB.upload(mc.B);

@@ -38,6 +38,9 @@ struct GFRep {
   // matrix continued fractions: floors A, B (p x p), weight W (p x n)
   struct MCF { int p = 0, nfl = 0; std::vector<double> A, B, W; };
   MCF me, mh;
+  // with combine_mcf: one fraction for G_e + G_h^T (mcf_set::build_combined,
+  // src_ed/mcf_set.hpp:79-90), evaluated in place of the two
+  MCF comb;
 };
 
 struct State {
@@ -119,6 +122,28 @@ struct Instance {
   std::string print_wavefunction();
 };
 
+// continued fractions of state `state`'s spin block (GF_method 'F'), fraction
+// f = 2 q + eh (q: pair o1 >= o2 row by row, eh 0 electron / 1 hole), as
+// evaluated: a_n and b_n (b_0 the weighted norm); lengths in len
+void cf_export(Instance &I, int spin, int state, std::vector<std::vector<double>> &a,
+               std::vector<std::vector<double>> &b);
+// one state of energy E0 in sector `sec` whose spin block `spin` is the given
+// set of continued fractions (same layout); the instance is then solved (no
+// wavefunction: quantities that need one return ED_ERR_INVALID)
+void cf_import(Instance &I, double E0, const Sector &sec, int spin, const std::vector<std::vector<double>> &a,
+               const std::vector<std::vector<double>> &b);
+
+// continued fractions of the ground state's spin block for the given orbital
+// pairs only (o1 == o2: c_o1 |Omega>; else c_o1 |Omega> + c_o2 |Omega>), electron
+// and hole, at most cap levels each (the first levels of the full fraction):
+// fraction 2 k + eh for pair k, transformed as the instance's own
+void cf_pairs(Instance &I, int spin, const std::vector<std::pair<int, int>> &pairs, int cap,
+              std::vector<std::vector<double>> &a, std::vector<std::vector<double>> &b);
+
+// the combined fraction of two matrix continued fractions (block size p, both
+// non-empty) for n orbitals; empty when it cannot be formed
+GFRep::MCF combine_mcf(const GFRep::MCF &e, const GFRep::MCF &h, int n, bool qr);
+
 Instance &get_instance(int label);
 void new_instance(const std::string &model, const std::map<std::string, double> &values, const std::string &sectors,
                   int label);
@@ -154,7 +179,7 @@ std::string hdf5_version();
 void random_start(size_t n, long seed, std::vector<double> &x);
 void write_instance_hdf5(const std::string &filename, int label, const std::string &group);
 void read_instance_hdf5(const std::string &filename, int label, const std::string &group);
-bool sector_valid(int L, const Sector &s);
+bool sector_valid(int L, const Sector &s, int g = 1);
 Sector shift_sector(const Sector &s, int pm, int spin);
 
 }  // namespace ed

@@ -1051,7 +1051,6 @@ void fold_partials_device(const double *part, int nb, double *out, hipStream_t s
   ED_HIP(hipGetLastError());
 }
 
-
 static FusedParams make_fused(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part,
                               uint32_t R) {
   FusedParams P;
@@ -1358,6 +1357,29 @@ void precond(size_t n, double *r, const double *d, double mu, double tiny, hipSt
 
 void set_unit(double *x, size_t i, hipStream_t s) {
   hipLaunchKernelGGL(k_set1, dim3(1), dim3(1), 0, s, x, i);
+  ED_HIP(hipGetLastError());
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_group_project(ProjArgs A, const double *x, double *y) {
+  const size_t n = (size_t)A.du * A.dd;
+  for (size_t J = (size_t)blockIdx.x * 256 + threadIdx.x; J < n; J += (size_t)gridDim.x * 256) {
+    const uint32_t ju = (uint32_t)(J % A.du), jd = (uint32_t)(J / A.du);
+    double acc = 0.0;
+    for (int k = 0; k < A.g; ++k) {
+      const size_t ou = (size_t)k * A.du + ju, od = (size_t)k * A.dd + jd;
+      const size_t I = (size_t)A.inv_up[ou] + (size_t)A.du * (size_t)A.inv_dn[od];
+      acc += A.c[k] * (double)(A.sg_up[ou] * A.sg_dn[od]) * x[I];
+    }
+    y[J] = acc;
+  }
+}
+}  // namespace
+
+void group_project(const ProjArgs &A, const double *x, double *y, hipStream_t s) {
+  const size_t n = (size_t)A.du * A.dd;
+  if (!n) return;
+  hipLaunchKernelGGL(k_group_project, dim3(grid_for(n, 256, 8)), dim3(256), 0, s, A, x, y);
   ED_HIP(hipGetLastError());
 }
 

@@ -73,7 +73,6 @@ enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
 int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
                       int variant, const SlabConfig &sc, hipStream_t s, double *aux = nullptr, size_t aux_cap = 0);
 
-
 // y = ax H x + beta y (y not read when beta == 0); when dot_partials != nullptr,
 // per-block partial sums of <y_new, x> are written there (nblocks returned).
 int hv_factorized(const FactArgs &a, const double *x, double *y, double ax, double beta, double *dot_partials,
@@ -158,6 +157,19 @@ struct CdagArgs {
   double z;
 };
 void cdag_gather(const CdagArgs &a, const double *x, double *y, hipStream_t s);
+
+// projection on an irreducible representation of the cluster's point group, in
+// a factorized sector: y[J] = sum_k c_k s_k(J) x[inv_k(J)] with inv_k(J) the
+// state that element k maps to J and s_k its sign (per spin: J = j_up + du j_dn)
+constexpr int kMaxGroup = 64;
+struct ProjArgs {
+  int g = 1;
+  uint32_t du = 0, dd = 0;
+  const int32_t *inv_up = nullptr, *inv_dn = nullptr;  // g x du, g x dd
+  const int8_t *sg_up = nullptr, *sg_dn = nullptr;
+  double c[kMaxGroup];
+};
+void group_project(const ProjArgs &a, const double *x, double *y, hipStream_t s);
 
 // ---- Green-function kernels ----
 // continued fractions: out[iz * ncf + f] = G_f(z_iz)

@@ -116,7 +116,8 @@ Model &get_model(const std::string &name) {
 }
 
 // ED::new_model (src_ed/qcm_ED.cpp:43-59)
-void new_model(const std::string &name, int n_sites, int n_bath) {
+void new_model(const std::string &name, int n_sites, int n_bath, const std::vector<std::vector<int>> &generators,
+               bool bath_irrep) {
   std::lock_guard<std::mutex> l(g_model_mu);
   if (g_models.count(name)) return;  // reference: warning, keeps the existing model
   if (n_bath < 0 || n_bath > 32) fail(ED_ERR_INVALID, "number of bath orbitals exceeds limits or negative!");
@@ -127,7 +128,201 @@ void new_model(const std::string &name, int n_sites, int n_bath) {
   M->n_sites = n_sites;
   M->n_bath = n_bath;
   M->L = n_sites + n_bath;
+  M->group = PointGroup::build(n_sites, M->L, generators, bath_irrep);
   g_models[name] = std::move(M);
+}
+
+// ---------------------------------------------------------------------------
+// point group (src_ed/symmetry_group.cpp:16-330, index.hpp)
+// ---------------------------------------------------------------------------
+PointGroup PointGroup::build(int n_sites, int L, const std::vector<std::vector<int>> &gens, bool bath_irrep) {
+  PointGroup G;
+  G.L = L;
+  G.ns = n_sites;
+  G.bath_irrep = bath_irrep && !gens.empty();
+  G.np = G.bath_irrep ? n_sites : L;
+  std::vector<std::vector<int>> gen = gens;
+  if (gen.empty()) {
+    std::vector<int> id(L, 0);
+    for (int i = 0; i < G.np; ++i) id[i] = i;
+    gen.push_back(id);
+  }
+  if (gen.size() > 8) fail(ED_ERR_INVALID, "Number of group generators exceeds the limit of 8");
+  auto identity = [&]() {
+    std::vector<int> v(L, 0);
+    for (int i = 0; i < G.np; ++i) v[i] = i;
+    return v;
+  };
+  auto is_identity_perm = [&](const std::vector<int> &v) {
+    for (int i = 0; i < G.np; ++i)
+      if (v[i] != i) return false;
+    return true;
+  };
+  // product x.y: the permutation part composes, phases add (modulo g, once known)
+  auto product = [&](const std::vector<int> &x, const std::vector<int> &y, int g) {
+    std::vector<int> v(L, 0);
+    for (int i = 0; i < G.np; ++i) v[i] = x[y[i]];
+    for (int i = G.np; i < L; ++i) v[i] = g ? (x[i] + y[i]) % g : x[i] + y[i];
+    return v;
+  };
+  for (size_t j = 0; j < gen.size(); ++j) {
+    const auto &v = gen[j];
+    if ((int)v.size() != L)
+      fail(ED_ERR_INVALID, "generator " + std::to_string(j + 1) + " should have " + std::to_string(L) + " elements");
+    std::vector<bool> seen(G.np, false);
+    for (int i = 0; i < G.np; ++i) {
+      if (v[i] < 0 || v[i] >= G.np || seen[v[i]])
+        fail(ED_ERR_INVALID, "generator " + std::to_string(j + 1) + " is not valid!");
+      seen[v[i]] = true;
+    }
+    for (int i = G.np; i < L; ++i)
+      if (v[i] < 0) fail(ED_ERR_INVALID, "generator " + std::to_string(j + 1) + " is not valid!");
+  }
+  for (size_t i = 0; i < gen.size(); ++i)
+    for (size_t j = 0; j < i; ++j) {
+      const auto a = product(gen[i], gen[j], 0), b = product(gen[j], gen[i], 0);
+      for (int k = 0; k < G.np; ++k)
+        if (a[k] != b[k])
+          fail(ED_ERR_INVALID, "generators " + std::to_string(i + 1) + " and " + std::to_string(j + 1) +
+                                   " do not commute!");
+    }
+  G.g = 1;
+  for (const auto &v : gen) {
+    std::vector<int> p = identity();
+    int n = 1;
+    while (true) {
+      p = product(p, v, 0);
+      if (is_identity_perm(p)) break;
+      if (++n > 64) fail(ED_ERR_INVALID, "generator of order above 64");
+    }
+    G.order.push_back(n);
+    G.g *= n;
+  }
+  for (int n : G.order)
+    if (n > 2)
+      fail(ED_ERR_UNSUPPORTED, "point groups with complex characters (a generator of order " + std::to_string(n) +
+                                   ") need a complex Hilbert space, which this solver does not provide");
+  for (int i = 0; i < L; ++i)
+    for (const auto &v : gen)
+      if (i >= G.np && v[i] >= G.g) fail(ED_ERR_INVALID, "generator phase outside 0..g-1");
+  const int ng = (int)gen.size();
+  auto digits = [&](int k) {
+    std::vector<int> d(ng);
+    for (int j = 0; j < ng; ++j) {
+      d[j] = k % G.order[j];
+      k /= G.order[j];
+    }
+    return d;
+  };
+  for (int k = 0; k < G.g; ++k) {
+    std::vector<int> p = identity();
+    const auto d = digits(k);
+    for (int j = 0; j < ng; ++j)
+      for (int m = 0; m < d[j]; ++m) p = product(p, gen[j], G.g);
+    if (k > 0 && is_identity_perm(p)) {
+      bool trivial_phase = true;
+      for (int i = G.np; i < L; ++i)
+        if (p[i] != 0) trivial_phase = false;
+      if (trivial_phase) fail(ED_ERR_INVALID, "The identity is generated as a non trivial element");
+    }
+    G.e.push_back(p);
+  }
+  G.chi.assign(G.g, std::vector<double>(G.g, 0.0));
+  for (int r = 0; r < G.g; ++r) {
+    const auto dr = digits(r);
+    for (int k = 0; k < G.g; ++k) {
+      const auto dk = digits(k);
+      long ph = 0;
+      for (int j = 0; j < ng; ++j) ph += (long)dr[j] * dk[j] * (G.g / G.order[j]);
+      G.chi[r][k] = std::cos(2.0 * M_PI * (double)(ph % G.g) / G.g);
+    }
+  }
+  // symmetric orbitals: per irrep, per orbit representative, sum_k chi_r(k) |e_k(site)>
+  G.S.clear();
+  G.irrep_dim.assign(G.g, 0);
+  G.irrep_off.assign(G.g, 0);
+  int nrow = 0;
+  for (int r = 0; r < G.g; ++r) {
+    G.irrep_off[r] = nrow;
+    std::vector<bool> done(n_sites, false);
+    for (int site = 0; site < n_sites; ++site) {
+      if (done[site]) continue;
+      std::vector<double> orb(n_sites, 0.0);
+      for (int k = 0; k < G.g; ++k) {
+        const int t = G.e[k][site] < n_sites ? G.e[k][site] : site;
+        done[t] = true;
+        orb[t] += G.chi[r][k];
+      }
+      double nn = 0.0;
+      for (double x : orb) nn += x * x;
+      if (nn < 1e-12) continue;
+      for (double &x : orb) x /= std::sqrt(nn);
+      G.S.insert(G.S.end(), orb.begin(), orb.end());
+      ++G.irrep_dim[r];
+      ++nrow;
+    }
+  }
+  if (nrow != n_sites)
+    fail(ED_ERR_INVALID, "Problem with construction of symmetric orbitals: " + std::to_string(nrow) +
+                             " found instead of " + std::to_string(n_sites));
+  return G;
+}
+
+GroupProjector::GroupProjector(const PointGroup &G, int irrep, int Nup, int Ndw) {
+  if (irrep < 0 || irrep >= G.g) fail(ED_ERR_SECTOR, "irreducible representation outside the point group");
+  if (G.g > kMaxGroup) fail(ED_ERR_UNSUPPORTED, "point group larger than 64 elements");
+  auto BU = half_basis(G.L, Nup), BD = half_basis(G.L, Ndw);
+  dim = (size_t)BU->dim * BD->dim;
+  args.g = G.g;
+  args.du = BU->dim;
+  args.dd = BD->dim;
+  for (int k = 0; k < G.g; ++k) args.c[k] = G.chi[irrep][k] / G.g;
+  auto maps = [&](const HalfBasis &B, DevBuf<int32_t> &inv, DevBuf<int8_t> &sg) {
+    std::vector<int32_t> iv((size_t)G.g * B.dim, -1);
+    std::vector<int8_t> sv((size_t)G.g * B.dim, 0);
+    for (int k = 0; k < G.g; ++k)
+      for (uint32_t i = 0; i < B.dim; ++i) {
+        int sign = 1;
+        const uint32_t j = B.index(G.apply(k, B.bin[i], sign));
+        if (j >= B.dim) fail(ED_ERR_INTERNAL, "point-group image outside the half basis");
+        iv[(size_t)k * B.dim + j] = (int32_t)i;
+        sv[(size_t)k * B.dim + j] = (int8_t)sign;
+      }
+    inv.upload(iv.empty() ? std::vector<int32_t>{0} : iv);
+    sg.upload(sv.empty() ? std::vector<int8_t>{0} : sv);
+  };
+  maps(*BU, inv_up, sg_up);
+  maps(*BD, inv_dn, sg_dn);
+  args.inv_up = inv_up.p;
+  args.inv_dn = inv_dn.p;
+  args.sg_up = sg_up.p;
+  args.sg_dn = sg_dn.p;
+}
+
+void GroupProjector::apply(const double *x, double *y) const { group_project(args, x, y, stream()); }
+
+uint32_t PointGroup::apply(int k, uint32_t b, int &sign) const {
+  const std::vector<int> &v = e[k];
+  uint32_t out = 0;
+  int inv = 0, ph = 0;
+  int placed[32];
+  int n = 0;
+  for (int i = 0; i < L; ++i) {
+    if (!((b >> i) & 1u)) continue;
+    if (i < np) {
+      out |= 1u << v[i];
+      for (int m = 0; m < n; ++m)
+        if (placed[m] > v[i]) ++inv;
+      placed[n++] = v[i];
+    } else {
+      out |= 1u << i;
+      ph += v[i];
+    }
+  }
+  // exp(2 pi i ph / g) is real (+-1) for the groups solved here
+  const double c = std::cos(2.0 * M_PI * (double)(ph % g) / g);
+  sign = ((inv & 1) ? -1 : 1) * (c < 0.0 ? -1 : 1);
+  return out;
 }
 
 // ED::new_operator (src_ed/qcm_ED.cpp:74-90) with the element handling of the

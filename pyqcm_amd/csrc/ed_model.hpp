@@ -62,16 +62,56 @@ struct Operator {
   bool spin_asym = false;    // up and down parts differ (mixing up_down)
 };
 
+// ---------------------------------------------------------------------------
+// Abelian point group of a cluster (symmetry_group, src_ed/symmetry_group.cpp:16-330):
+// an element maps orbital i -> perm[i] for the first np orbitals (the sites with
+// bath_irrep, all orbitals otherwise) and multiplies an occupied orbital i >= np
+// by exp(2 pi i phase[i] / g).  Elements are products of the generators'
+// powers, numbered with the first generator's power the fastest digit;
+// irrep r has characters chi_r(k) = exp(2 pi i sum_j r_j k_j / n_j) (n_j the
+// order of generator j, r_j / k_j the digits of r / k).  Only groups with real
+// characters (generators of order 1 or 2) are solved here.
+// ---------------------------------------------------------------------------
+struct PointGroup {
+  int g = 1, L = 0, ns = 0, np = 0;
+  bool bath_irrep = false;
+  std::vector<int> order;                 // order of each generator
+  std::vector<std::vector<int>> e;        // elements (length L: permutation, then phases)
+  std::vector<std::vector<double>> chi;   // chi[r][k] (real)
+  // symmetric orbitals: rows of S (n_sites x n_sites, real), irrep by irrep;
+  // irrep_dim[r] orbitals in irrep r, starting at row irrep_off[r]
+  std::vector<double> S;
+  std::vector<int> irrep_dim, irrep_off;
+  static PointGroup build(int n_sites, int L, const std::vector<std::vector<int>> &gens, bool bath_irrep);
+  // image of a one-spin occupation word under element k and its sign
+  // (fermionic reordering x orbital phases)
+  uint32_t apply(int k, uint32_t b, int &sign) const;
+};
+
+// P_r = (1 / g) sum_k chi_r(k) U(e_k) on the factorized sector (Nup, Ndw): the
+// many-body action of an element is a signed permutation of each spin's half
+// states (site permutation, fermionic reordering sign, orbital phases)
+struct GroupProjector {
+  size_t dim = 0;
+  ProjArgs args;
+  DevBuf<int32_t> inv_up, inv_dn;
+  DevBuf<int8_t> sg_up, sg_dn;
+  GroupProjector(const PointGroup &G, int irrep, int Nup, int Ndw);
+  void apply(const double *x, double *y) const;
+};
+
 struct Model {
   std::string name;
   int n_sites = 0, n_bath = 0, L = 0;
   std::vector<std::string> order;            // definition order
   std::map<std::string, Operator> ops;
   bool closed = false;
+  PointGroup group;                          // trivial (g = 1) without generators
 };
 
 Model &get_model(const std::string &name);
-void new_model(const std::string &name, int n_sites, int n_bath);
+void new_model(const std::string &name, int n_sites, int n_bath,
+               const std::vector<std::vector<int>> &generators = {}, bool bath_irrep = false);
 void new_operator(const std::string &model, const std::string &name, const std::string &type,
                   const std::vector<Element> &el);
 

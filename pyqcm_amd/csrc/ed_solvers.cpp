@@ -89,7 +89,7 @@ double ddot(size_t n, const double *a, const double *b) {
 // eigenvector is one combination pass; otherwise (or when the HBM budget is
 // exceeded) the recurrence is replayed as in the reference.
 // ---------------------------------------------------------------------------
-LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
+LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs, const GroupProjector *proj) {
   Params &P = params();
   const int max_iter = (int)P.geti("max_iter_lanczos");
   const double accur = P.getd("accur_lanczos");
@@ -97,7 +97,8 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
   const double gb = P.getd("lanczos_basis_budget_GB");
   const double budget = gb >= 0.0 ? gb * 1e9 : std::max(0.0, 0.9 * (double)dev_available_bytes() - 4e9);
   const bool reorth = P.getb("lanczos_reorth");
-  bool store = P.getb("store_lanczos_basis") || reorth;
+  // a projected recurrence keeps its basis (the replay would have to project too)
+  bool store = P.getb("store_lanczos_basis") || reorth || proj;
   LanczosGS R;
   std::vector<DevBuf<double>> basis;  // basis[k] = s_k v_k
   std::vector<double> sc;             // s_k
@@ -117,6 +118,9 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
     if (store) {
       // w = H v_k - beta_{k-1} v_{k-1} - alpha v_k into a fresh buffer
       if ((double)(basis.size() + 1) * n * 8.0 > budget) {
+        if (proj)
+          fail(ED_ERR_OOM, "a point-group sector keeps the Krylov basis in HBM: " + std::to_string(basis.size() + 1) +
+                               " vectors exceed lanczos_basis_budget_GB");
         if (reorth)
           fail(ED_ERR_OOM, "lanczos_reorth keeps the Krylov basis in HBM: " + std::to_string(basis.size() + 1) +
                                " vectors exceed lanczos_basis_budget_GB");
@@ -155,6 +159,14 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
           if (!again) break;
         }
       }
+      if (proj) {
+        // back onto the representation: rounding leaks weight into the others,
+        // which the recurrence would otherwise amplify
+        DevBuf<double> Tp(n);
+        proj->apply(W.p, Tp.p);
+        W = std::move(Tp);
+        w2 = ddot(n, W.p, W.p);
+      }
       beta = std::sqrt(w2);
       basis.push_back(std::move(W));
       sc.push_back(beta);
@@ -190,6 +202,11 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
       cf.push_back(z[j] / sc[j]);
     }
     comb(n, d_gs, 0.0, src, cf, false);
+    if (proj) {
+      DevBuf<double> Tp(n);
+      proj->apply(d_gs, Tp.p);
+      d2d(d_gs, Tp.p, n);
+    }
   } else {
     // LanczosEigenvector replay (Lanczos.hpp:109-143)
     d2d(A.p, d_start, n);
@@ -214,15 +231,26 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs) {
 // ---------------------------------------------------------------------------
 // LanczosGreen: same recurrence, stops on the products of beta.
 // ---------------------------------------------------------------------------
-void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, std::vector<double> &b) {
+void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, std::vector<double> &b,
+                   long max_steps) {
   Params &P = params();
   const double acc = P.getd("accur_continued_fraction");
-  const size_t max_cf = (size_t)P.geti("max_iter_CF");
+  const size_t max_cf = max_steps < 0 ? (size_t)P.geti("max_iter_CF") : (size_t)max_steps;
   const size_t n = H.dim;
+  // with lanczos_reorth, every new vector is reorthogonalised (classical
+  // Gram-Schmidt, twice) against the normalized Krylov vectors kept in HBM:
+  // the fraction is then that of the exact Krylov process, down to the
+  // exhaustion of the space (b_n -> 0 when accur_continued_fraction allows it)
+  const bool reorth = P.getb("lanczos_reorth");
   a.clear();
   b.assign(1, 1.0);
   DevBuf<double> A(n), B(n);
   d2d(A.p, d_psi, n);
+  std::vector<DevBuf<double>> basis;
+  if (reorth) {
+    basis.emplace_back(n);
+    d2d(basis[0].p, d_psi, n);
+  }
   double bA = 1.0, bB = 1.0, prod = 1.0;
   while (prod > acc && std::fabs(b.back()) > acc && a.size() < max_cf) {
     check_cancel();
@@ -230,13 +258,28 @@ void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, 
     const double cb = k ? -b.back() / bB : 0.0;
     const int nb = H.apply(A.p, B.p, 1.0 / bA, cb, true);
     const double alpha = H.dot_result(nb) / bA;
-    const double beta = std::sqrt(comb(n, B.p, 1.0, {A.p}, {-alpha / bA}, true));
+    double w2 = comb(n, B.p, 1.0, {A.p}, {-alpha / bA}, true);
+    if (reorth) {
+      std::vector<const double *> all;
+      for (const auto &v : basis) all.push_back(v.p);
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<double> c = dots(n, B.p, all);
+        for (auto &x : c) x = -x;
+        w2 = comb(n, B.p, 1.0, all, c, true);
+      }
+    }
+    const double beta = std::sqrt(w2);
     a.push_back(alpha);
     b.push_back(beta);
     prod *= std::fabs(beta);
     std::swap(A.p, B.p);
     bB = bA;
     bA = beta;
+    if (reorth && beta > 0.0) {
+      basis.emplace_back(n);
+      d2d(basis.back().p, A.p, n);
+      scale(n, basis.back().p, 1.0 / beta, stream());
+    }
   }
 }
 
@@ -599,7 +642,7 @@ int davidson(Hamiltonian &H, int K, double accuracy, std::vector<double> &evals,
 // block Lanczos, QR variant (src_ed/Hamiltonian/Lanczos.hpp:426-572)
 // ---------------------------------------------------------------------------
 int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q0, int M0, std::vector<double> &A,
-                  std::vector<double> &B) {
+                  std::vector<double> &B, bool qr) {
   Params &Pm = params();
   const size_t dim = H.dim;
   const int p = (int)q0.size();
@@ -656,23 +699,63 @@ int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q0, int M0,
     }
     std::vector<double> Bj(pp, 0.0);
     bool singular = false;
-    for (int l = 0; l < p; ++l) {
-      for (int k = 0; k < l; ++k) {
-        const double z = ddot(dim, Z[k].p, Z[l].p);
-        Bj[k + l * p] = z;
-        comb(dim, Z[l].p, 1.0, {Z[k].p}, {-z}, false);
+    if (qr) {
+      // Z = Q_{j+1} B_j by modified Gram-Schmidt (B_j upper triangular)
+      for (int l = 0; l < p; ++l) {
+        for (int k = 0; k < l; ++k) {
+          const double z = ddot(dim, Z[k].p, Z[l].p);
+          Bj[k + l * p] = z;
+          comb(dim, Z[l].p, 1.0, {Z[k].p}, {-z}, false);
+        }
+        const double nrm = std::sqrt(ddot(dim, Z[l].p, Z[l].p));
+        if (nrm < accur_deflation) {
+          Bj[l + l * p] = 0.0;
+          for (int ll = l; ll < p; ++ll) fill(dim, Z[ll].p, 0.0, stream());
+          singular = true;
+          break;
+        }
+        Bj[l + l * p] = nrm;
+        scale(dim, Z[l].p, 1.0 / nrm, stream());
       }
-      const double nrm = std::sqrt(ddot(dim, Z[l].p, Z[l].p));
-      if (nrm < accur_deflation) {
-        Bj[l + l * p] = 0.0;
-        for (int ll = l; ll < p; ++ll) fill(dim, Z[ll].p, 0.0, stream());
-        singular = true;
-        break;
+      B.insert(B.end(), Bj.begin(), Bj.end());
+    } else {
+      // polar form (blockLanczosSVD, Lanczos.hpp:603-779): Gram matrix
+      // G = Z^T Z = V D V^T, B_j = V D^(1/2) V^T (symmetric positive
+      // semi-definite), Q_{j+1} = Z V D^(-1/2) V^T; deflation when a singular
+      // value D^(1/2) falls below accur_deflation
+      std::vector<double> G(pp), D(p), V(pp);
+      for (int l = 0; l < p; ++l) {
+        std::vector<const double *> zs;
+        for (int k = 0; k <= l; ++k) zs.push_back(Z[k].p);
+        std::vector<double> gg = dots(dim, Z[l].p, zs);
+        for (int k = 0; k <= l; ++k) G[k + l * p] = G[l + k * p] = gg[k];
       }
-      Bj[l + l * p] = nrm;
-      scale(dim, Z[l].p, 1.0 / nrm, stream());
+      symeig(p, G.data(), D.data(), V.data());
+      for (int m = 0; m < p; ++m)
+        if (D[m] < accur_deflation * accur_deflation) singular = true;
+      for (int r = 0; r < p; ++r)
+        for (int c = 0; c < p; ++c) {
+          double t = 0.0;
+          for (int m = 0; m < p; ++m) t += V[r + m * p] * std::sqrt(std::max(D[m], 0.0)) * V[c + m * p];
+          Bj[r + c * p] = t;
+        }
+      B.insert(B.end(), Bj.begin(), Bj.end());
+      if (singular) { ++j; break; }
+      // Q_{j+1}[l] = sum_k Binv(k, l) Z[k] into the buffers of Q_{j-1}
+      // (no longer needed), which then become Q_cur
+      for (int l = 0; l < p; ++l) {
+        std::vector<const double *> src;
+        std::vector<double> cf;
+        for (int k = 0; k < p; ++k) {
+          double t = 0.0;
+          for (int m = 0; m < p; ++m) t += V[k + m * p] * V[l + m * p] / std::sqrt(D[m]);
+          src.push_back(Z[k].p);
+          cf.push_back(t);
+        }
+        comb(dim, Qp[l].p, 0.0, src, cf, false);
+      }
+      for (int k = 0; k < p; ++k) std::swap(Qp[k].p, Qc[k].p);
     }
-    B.insert(B.end(), Bj.begin(), Bj.end());
     bool conv = false;
     if ((j + 1) >= 2 * p && (j + 1) % p == 0) {
       const int sz = (j + 1) * p;
@@ -693,10 +776,11 @@ int block_lanczos(Hamiltonian &H, const std::vector<const double *> &q0, int M0,
       if (delta < accur_bl) conv = true;
     }
     if (conv || singular) { ++j; break; }
-    for (int k = 0; k < p; ++k) {
-      std::swap(Qp[k].p, Qc[k].p);
-      std::swap(Qc[k].p, Z[k].p);
-    }
+    if (qr)
+      for (int k = 0; k < p; ++k) {
+        std::swap(Qp[k].p, Qc[k].p);
+        std::swap(Qc[k].p, Z[k].p);
+      }
   }
   sync();
   return (int)(A.size() / pp);

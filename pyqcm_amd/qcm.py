@@ -79,9 +79,19 @@ set_global_parameter = set_global_parameterC
 
 
 def new_model(name, n_sites, n_bath, generators=None, bath_irrep=False):
-    """qcm.new_model (src_python/qcm_ED_wrap.hpp:145-163); trivial point group only."""
-    ng = 0 if generators is None else len(generators)
-    check(lib().ed_new_model(name.encode(), int(n_sites), int(n_bath), None, ng, int(bool(bath_irrep))))
+    """qcm.new_model (src_python/qcm_ED_wrap.hpp:145-163): generators are lists of
+    n_sites + n_bath 1-based orbital images (with bath_irrep, the bath entries
+    are phases and kept as they are), converted to 0-based as the binding does."""
+    n_orb = int(n_sites) + int(n_bath)
+    gens = [] if generators is None else [list(g) for g in generators]
+    flat = []
+    for j, g in enumerate(gens):
+        if len(g) != n_orb:
+            raise EDError(1, "generator %d should have %d elements" % (j + 1, n_orb))
+        flat += [int(v) - 1 for v in g[:n_sites]]
+        flat += [int(v) if bath_irrep else int(v) - 1 for v in g[n_sites:]]
+    arr = (C.c_int * max(len(flat), 1))(*flat)
+    check(lib().ed_new_model(name.encode(), int(n_sites), int(n_bath), arr, len(gens), int(bool(bath_irrep))))
     _models.setdefault(name, (int(n_sites), int(n_bath), []))
 
 
@@ -482,6 +492,25 @@ def qmatrix(label=0, spin_down=0):
     check(lib().ed_qmatrix(int(label), int(spin_down), M_.value, e.ctypes.data_as(C.POINTER(C.c_double)),
                            v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(L_), C.byref(M_)))
     return e[:M_.value], v[:M_.value * L_.value].reshape(M_.value, L_.value).T.copy()
+
+
+def combined_mcf(label=0, spin_down=0):
+    """(W, [A_j], [B_j]) of the combined matrix continued fraction (pyqcm's
+    combined_mcf, src_python/qcm_wrap.hpp:502-520; needs GF_method 'M' and
+    combine_mcf = True); W is p x d."""
+    p, nfl = C.c_int(), C.c_int()
+    check(lib().ed_combined_mcf(int(label), int(spin_down), 0, None, None, None, C.byref(p), C.byref(nfl)))
+    d = Green_function_dimensionC(label)
+    W = np.zeros(p.value * d)
+    A = np.zeros(nfl.value * p.value * p.value)
+    B = np.zeros(nfl.value * p.value * p.value)
+    check(lib().ed_combined_mcf(int(label), int(spin_down), nfl.value, _dptr(W), _dptr(A), _dptr(B), C.byref(p),
+                                C.byref(nfl)))
+    P = p.value
+    W = W.reshape(d, P).T
+    A = [A[j * P * P:(j + 1) * P * P].reshape(P, P).T for j in range(nfl.value)]
+    B = [B[j * P * P:(j + 1) * P * P].reshape(P, P).T for j in range(nfl.value)]
+    return W, A, B
 
 
 def states(label=0):

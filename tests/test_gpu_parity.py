@@ -397,17 +397,28 @@ def test_davidson_matches_dense():
     assert np.allclose(ev, ref, atol=1e-8)
 
 
-def test_block_lanczos_mcf_matches_oracle(oracle):
+@pytest.mark.parametrize("qr", [True, False])
+def test_block_lanczos_mcf_matches_oracle(oracle, qr):
+    """blockLanczos (QR) and blockLanczosSVD (polar, the reference's default,
+    block_Lanczos_QR = false) floor by floor against the oracle"""
     wl = W.hubbard_chain(6, U=4.0, mu=2.0)
     lab = _instance(wl, "R0:N7:S1")
     H = qcm.Hamiltonian(lab, "R0:N7:S1", "G")
     rng = np.random.default_rng(9)
     q = np.linalg.qr(rng.normal(size=(H.dim, 3)))[0].T.copy()
-    A, B = H.block_lanczos(q, 30)
-    Ar, Br = oracle.Model(wl.L, wl.terms()).block_lanczos(4, 3, q, 30)
+    qcm.set_global_parameterC("block_Lanczos_QR", qr)
+    try:
+        A, B = H.block_lanczos(q, 30)
+    finally:
+        qcm.set_global_parameterC("block_Lanczos_QR", False)
+    Ar, Br = oracle.Model(wl.L, wl.terms()).block_lanczos(4, 3, q, 30, qr=qr)
     k = min(len(A), len(Ar), 8)
     assert np.allclose(A[:k], Ar[:k], atol=1e-9)
-    assert np.allclose(np.abs(B[:k]), np.abs(Br[:k]), atol=1e-9)
+    if qr:
+        assert np.allclose(np.abs(B[:k]), np.abs(Br[:k]), atol=1e-9)
+    else:
+        assert np.allclose(B[:k], Br[:k], atol=1e-9)
+        assert np.allclose(B[:k], B[:k].transpose(0, 2, 1), atol=1e-12)
 
 
 def test_krylov_outputs_longer_than_cap_fail_with_the_size_needed():

@@ -350,15 +350,55 @@ def test_block_lanczos_mcf_matches_lehmann(oracle):
         n = np.linalg.norm(q[l])
         Wm[l, l] = n
         q[l] /= n
-    # accur_bl = 0: recursion runs until the Krylov space is exhausted -> exact G
-    A, B = M.block_lanczos(4, 3, q, M0=200, accur_bl=0.0)
-    A = A - E0 * np.eye(p)[None]
-    G = oracle.mcf_eval(A, B, Wm, z)
+    # accur_bl = 0: recursion runs until the Krylov space is exhausted -> exact G;
+    # both the QR form and the polar form (symmetric B) of the recursion
     Hp = dense_hamiltonian(L, wl.terms(), 4, 3)
     ep, Up = np.linalg.eigh(Hp)
     C = np.array([Up.T @ (dense_c(L, a, 3, 3, True) @ gs) for a in orbs])
     Gl = np.einsum("ai,bi,zi->zab", C, C, 1.0 / (z[:, None] - (ep - E0)[None, :]))
-    assert np.allclose(G, Gl, atol=1e-8)
+    for qr in (True, False):
+        A, B = M.block_lanczos(4, 3, q, M0=200, accur_bl=0.0, qr=qr)
+        if not qr:
+            assert np.allclose(B, B.transpose(0, 2, 1), atol=1e-12)
+            assert np.all(np.linalg.eigvalsh(B[:-1]) > -1e-12)
+        A = A - E0 * np.eye(p)[None]
+        G = oracle.mcf_eval(A, B, Wm, z)
+        assert np.allclose(G, Gl, atol=1e-8), qr
+
+
+def _block_tridiagonal(A, B, k):
+    p = A.shape[1]
+    T = np.zeros((k * p, k * p))
+    for j in range(k):
+        T[j * p:(j + 1) * p, j * p:(j + 1) * p] = A[j]
+        if j + 1 < k:
+            T[(j + 1) * p:(j + 2) * p, j * p:(j + 1) * p] = B[j]
+            T[j * p:(j + 1) * p, (j + 1) * p:(j + 2) * p] = B[j].T
+    return T
+
+
+def test_block_lanczos_polar_and_qr_span_the_same_krylov_space(oracle):
+    """The two forms differ by an orthogonal change of basis inside each block:
+    their projected block-tridiagonal matrices have the same spectrum, floor by
+    floor, and the first floor A_0 is identical."""
+    wl = W.hubbard_chain(8, U=4.0, mu=2.0)
+    M = oracle.Model(wl.L, wl.terms())
+    rng = np.random.default_rng(4)
+    dim = M.dim(5, 4)
+    q = np.linalg.qr(rng.normal(size=(dim, 3)))[0].T.copy()
+    Aq, Bq = M.block_lanczos(5, 4, q, M0=12, accur_bl=0.0, qr=True)
+    Ap, Bp = M.block_lanczos(5, 4, q, M0=12, accur_bl=0.0, qr=False)
+    assert np.allclose(Aq[0], Ap[0], atol=1e-12)
+    for k in (2, 5, 12):
+        eq = np.linalg.eigvalsh(_block_tridiagonal(Aq, Bq, k))
+        ep = np.linalg.eigvalsh(_block_tridiagonal(Ap, Bp, k))
+        assert np.allclose(eq, ep, atol=1e-9), k
+    # the polar B_j is the symmetric square root of the residual's Gram matrix
+    # Z^T Z = B_qr^T B_qr: at the first floor exactly, beyond it up to the
+    # rotation inside the block (same spectrum)
+    assert np.allclose(Bp[0] @ Bp[0], Bq[0].T @ Bq[0], atol=1e-9)
+    for j in range(1, 6):
+        assert np.allclose(np.linalg.eigvalsh(Bp[j] @ Bp[j]), np.linalg.eigvalsh(Bq[j].T @ Bq[j]), atol=1e-9)
 
 
 def test_streamline_merges_degenerate_poles(oracle):
