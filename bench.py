@@ -77,20 +77,21 @@ def cpu_hv_baseline(wl, Nup=8, Ndw=8, nrep=5, csr=False):
     return per, orc.openmp_threads(), dim
 
 
-def batch_gpu(nx, ny, npts=512, nz=256, reps=3):
+def batch_gpu(nx, ny, npts=512, nz=256, reps=3, n_bath=0):
     """Batched VCA sweep: npts parameter points (U, Neel field M, mu from
-    default_rng(4)) of the nx x ny cluster; ground state over the five target
+    default_rng(4)) of the nx x ny cluster (with n_bath bath orbitals: the
+    2x3 + 6 bath model at N = 6, 48 400 states); ground state over the target
     sectors and continued-fraction G on nz real frequencies (both spins) per
     point, all points in one solve_batch.  Median over reps after a warm-up."""
     import numpy as np
     from pyqcm_amd import qcm, workloads as W
     qcm.set_global_parameterC("GF_method", "F")
-    wl = W.vca_cluster(nx, ny)
+    wl = W.vca_cluster_bath(nx, ny, n_bath) if n_bath else W.vca_cluster(nx, ny)
     name = "bench_" + wl.name
     qcm.new_model(name, wl.n_sites, wl.n_bath)
     for op, (typ, el) in wl.operators.items():
         qcm.new_operator(name, op, typ, [(r + 1, c + 1, v) for (r, c, v) in el])
-    pts = W.vca_points(npts, seed=4)
+    pts = [{**wl.values, **v} for v in W.vca_points(npts, seed=4)]
     z = W.real_axis(nz, -8.0, 8.0, 0.1)
     times, E0 = [], None
     label = 8000
@@ -544,6 +545,21 @@ def _batch_leg(args):
                       "cpu_sample": "first 16 points, oracle pipeline",
                       "speedup": round((512 / tg) * tc, 1), "dE0_rel_first_point": abs(E0g - E0c) / abs(E0c)})
         res["%dx%d" % (nx, ny)] = r
+    # the global-memory workgroups: one 48 400-state sector per point
+    npts = 64
+    wl, pts, tg, E0g = batch_gpu(2, 3, npts=npts, n_bath=6)
+    r = {"workload": "2x3 Hubbard cluster + 6 bath orbitals (default_rng(7)), sector N=6 S=0 (220 x 220 = 48 400 "
+                     "states; fractions in 14 520 / 108 900-state sectors), %d points U~U[2,10] M~U[0,0.3] "
+                     "mu~U[-1,1] (default_rng(4)), continued-fraction G on 256 real frequencies, both spins" % npts,
+         "gpu_points_per_s": round(npts / tg, 2), "gpu_s_per_batch": round(tg, 4),
+         "timing": "median of 3 batched solves after one warm-up"}
+    if not args.no_cpu:
+        E0c, tc = batch_cpu(wl, pts, nsample=2)
+        import oracle.oracle as orc
+        r.update({"cpu_points_per_s": round(1.0 / tc, 3), "cpu_cores": orc.openmp_threads(),
+                  "cpu_sample": "first 2 points, oracle pipeline",
+                  "speedup": round((npts / tg) * tc, 1), "dE0_rel_first_point": abs(E0g - E0c) / abs(E0c)})
+    res["2x3_6bath"] = r
     return res
 
 

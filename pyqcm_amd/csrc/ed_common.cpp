@@ -51,12 +51,14 @@ Params::Params() {
   i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
   i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
   i["reduction_blocks_per_cu"] = 1;
-  i["vector_block"] = 1024;        // threads per workgroup of the Krylov vector kernels: 256, 512 or 1024  // grid of the Krylov vector kernels (multi-dot, axpy, combine)
+  i["vector_block"] = 1024;       // threads per workgroup of the Krylov vector kernels (multi-dot, axpy, combine): 256, 512 or 1024
   i["hv_block"] = 0;              // row-kernel block size: 0 auto, 256 or 1024 threads
   i["hv_row_order"] = 1;          // fused row kernel: 0 label order, 1 breadth-first order of the dn rows
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
+  i["lanczos_block"] = 6;           // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
+  b["lanczos_graph"] = true;        // replay the block as a captured hipGraph
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
   d["lanczos_basis_budget_GB"] = -1.0;  // negative: 90 % of the available device memory less 4 GB

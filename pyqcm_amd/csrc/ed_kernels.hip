@@ -857,89 +857,143 @@ __global__ void k_vdvh(int L, int M, const double *e, const double *V, int nz, c
 }
 
 constexpr int kMcfMax = 8;
-__device__ bool cinv_small(int n, cplx *a, cplx *inv) {
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) inv[i + j * n] = {i == j ? 1.0 : 0.0, 0.0};
-  for (int c = 0; c < n; ++c) {
-    int pr = c;
-    double best = hypot(a[c + c * n].re, a[c + c * n].im);
-    for (int r = c + 1; r < n; ++r) {
-      const double m = hypot(a[r + c * n].re, a[r + c * n].im);
-      if (m > best) { best = m; pr = r; }
-    }
-    if (best == 0.0) return false;
-    if (pr != c)
-      for (int k = 0; k < n; ++k) {
-        cplx t = a[c + k * n]; a[c + k * n] = a[pr + k * n]; a[pr + k * n] = t;
-        t = inv[c + k * n]; inv[c + k * n] = inv[pr + k * n]; inv[pr + k * n] = t;
-      }
-    const cplx dinv = cdiv(cplx{1.0, 0.0}, a[c + c * n]);
-    for (int k = 0; k < n; ++k) {
-      cplx v = a[c + k * n];
-      a[c + k * n] = {v.re * dinv.re - v.im * dinv.im, v.re * dinv.im + v.im * dinv.re};
-      v = inv[c + k * n];
-      inv[c + k * n] = {v.re * dinv.re - v.im * dinv.im, v.re * dinv.im + v.im * dinv.re};
-    }
-    for (int r = 0; r < n; ++r) {
-      if (r == c) continue;
-      const cplx f = a[r + c * n];
-      if (f.re == 0.0 && f.im == 0.0) continue;
-      for (int k = 0; k < n; ++k) {
-        const cplx u = a[c + k * n], w = inv[c + k * n];
-        a[r + k * n].re -= f.re * u.re - f.im * u.im;
-        a[r + k * n].im -= f.re * u.im + f.im * u.re;
-        inv[r + k * n].re -= f.re * w.re - f.im * w.im;
-        inv[r + k * n].im -= f.re * w.im + f.im * w.re;
-      }
-    }
-  }
-  return true;
-}
-
-__global__ void k_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz,
-                           const double *z, double *G) {
-  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-  if (iz >= nz) return;
-  cplx F[kMcfMax * kMcfMax], D[kMcfMax * kMcfMax], T[kMcfMax * kMcfMax];
+// Matrix continued fraction on a frequency grid.  One 64-lane workgroup holds
+// 64 / p^2 frequencies; each frequency's p x p complex matrices (F, D, T and
+// the inverse) live in LDS, one lane per entry (r, c).  Level j (from the
+// last): D = z - A_j - B_j^T F B_j, F = D^-1 by Gauss-Jordan with partial
+// pivoting (first largest modulus, as a serial sweep picks it).
+__global__ __launch_bounds__(64) void k_mcf_eval(int p, int nfl, const double *A, const double *B, const double *W,
+                                                 int q, int nz, const double *z, double *G) {
+  constexpr int M = kMcfMax * kMcfMax;
+  __shared__ cplx sF[M], sD[M], sT[M], sI[M];
+  __shared__ double sA[M], sB[M];
+  __shared__ int sfail[M];
   const int pp = p * p;
-  for (int k = 0; k < pp; ++k) F[k] = {0.0, 0.0};
-  const double zr = z[2 * iz], zi = z[2 * iz + 1];
-  for (int j = nfl - 1; j >= 0; --j) {
-    const double *Aj = A + (size_t)j * pp, *Bj = B + (size_t)j * pp;
-    for (int k = 0; k < pp; ++k) D[k] = {-Aj[k], 0.0};
-    for (int k = 0; k < p; ++k) { D[k + k * p].re += zr; D[k + k * p].im += zi; }
-    if (j < nfl - 1) {
-      for (int c = 0; c < p; ++c)
-        for (int r = 0; r < p; ++r) {
-          cplx s{0.0, 0.0};
-          for (int k = 0; k < p; ++k) { s.re += F[r + k * p].re * Bj[k + c * p]; s.im += F[r + k * p].im * Bj[k + c * p]; }
-          T[r + c * p] = s;
-        }
-      for (int c = 0; c < p; ++c)
-        for (int r = 0; r < p; ++r) {
-          cplx s{0.0, 0.0};
-          for (int k = 0; k < p; ++k) { s.re += Bj[k + r * p] * T[k + c * p].re; s.im += Bj[k + r * p] * T[k + c * p].im; }
-          D[r + c * p].re -= s.re;
-          D[r + c * p].im -= s.im;
-        }
-    }
-    if (!cinv_small(p, D, F)) {
-      for (int k = 0; k < q * q; ++k) { G[2 * ((size_t)iz * q * q + k)] = NAN; G[2 * ((size_t)iz * q * q + k) + 1] = NAN; }
-      return;
-    }
+  const int per = 64 / pp;                  // frequencies per workgroup
+  const int g = threadIdx.x / pp;           // this lane's frequency slot
+  const int e = threadIdx.x - g * pp;       // entry r + c p
+  const int r = e % p, c = e / p;
+  const bool lane_on = g < per;
+  const int iz = blockIdx.x * per + g;
+  const bool live = lane_on && iz < nz;
+  const int o = g * pp;                     // slot's offset in the LDS matrices
+  double zr = 0.0, zi = 0.0;
+  if (live) {
+    zr = z[2 * iz];
+    zi = z[2 * iz + 1];
   }
-  for (int c = 0; c < q; ++c)
-    for (int r = 0; r < q; ++r) {
-      double sr = 0.0, si = 0.0;
-      for (int k = 0; k < p; ++k)
-        for (int l = 0; l < p; ++l) {
-          const double w = W[k + r * p] * W[l + c * p];
-          sr += w * F[k + l * p].re;
-          si += w * F[k + l * p].im;
-        }
-      G[2 * ((size_t)iz * q * q + r + c * q)] = sr;
-      G[2 * ((size_t)iz * q * q + r + c * q) + 1] = si;
+  if (lane_on) sF[o + e] = {0.0, 0.0};
+  if (threadIdx.x < per) sfail[threadIdx.x] = 0;
+  for (int j = nfl - 1; j >= 0; --j) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < pp; k += 64) {
+      sA[k] = A[(size_t)j * pp + k];
+      sB[k] = B[(size_t)j * pp + k];
     }
+    __syncthreads();
+    cplx d{0.0, 0.0};
+    if (lane_on) {
+      d = {-sA[e], 0.0};
+      if (r == c) { d.re += zr; d.im += zi; }
+    }
+    if (j < nfl - 1) {
+      if (lane_on) {
+        cplx t{0.0, 0.0};
+        for (int k = 0; k < p; ++k) {
+          const cplx f = sF[o + r + k * p];
+          const double bb = sB[k + c * p];
+          t.re += f.re * bb;
+          t.im += f.im * bb;
+        }
+        sT[o + e] = t;
+      }
+      __syncthreads();
+      if (lane_on) {
+        cplx t{0.0, 0.0};
+        for (int k = 0; k < p; ++k) {
+          const double bb = sB[k + r * p];
+          const cplx u = sT[o + k + c * p];
+          t.re += bb * u.re;
+          t.im += bb * u.im;
+        }
+        d.re -= t.re;
+        d.im -= t.im;
+      }
+    }
+    // Gauss-Jordan: sD -> identity, sI -> D^-1
+    if (lane_on) {
+      sD[o + e] = d;
+      sI[o + e] = {r == c ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    for (int k = 0; k < p; ++k) {
+      int pr = k;
+      double best = 0.0;
+      if (lane_on) {
+        best = hypot(sD[o + k + k * p].re, sD[o + k + k * p].im);
+        for (int rr = k + 1; rr < p; ++rr) {
+          const double m = hypot(sD[o + rr + k * p].re, sD[o + rr + k * p].im);
+          if (m > best) { best = m; pr = rr; }
+        }
+        if (best == 0.0 && e == 0) sfail[g] = 1;
+      }
+      // row exchange k <-> pr, then row k scaled by 1 / D[k][k]
+      cplx va{0.0, 0.0}, vi{0.0, 0.0};
+      if (lane_on && best != 0.0) {
+        const int src = (r == k) ? pr : (r == pr ? k : r);
+        va = sD[o + src + c * p];
+        vi = sI[o + src + c * p];
+      }
+      __syncthreads();
+      if (lane_on && best != 0.0) {
+        if (r == k) {
+          const cplx piv = sD[o + pr + k * p];  // not yet overwritten: row pr is written below only by lanes r == pr
+          const cplx dinv = cdiv(cplx{1.0, 0.0}, piv);
+          va = {va.re * dinv.re - va.im * dinv.im, va.re * dinv.im + va.im * dinv.re};
+          vi = {vi.re * dinv.re - vi.im * dinv.im, vi.re * dinv.im + vi.im * dinv.re};
+        }
+      }
+      __syncthreads();
+      if (lane_on && best != 0.0) {
+        sD[o + e] = va;
+        sI[o + e] = vi;
+      }
+      __syncthreads();
+      // elimination of column k from the other rows
+      cplx f{0.0, 0.0}, u{0.0, 0.0}, w{0.0, 0.0};
+      if (lane_on && best != 0.0 && r != k) {
+        f = sD[o + r + k * p];
+        u = sD[o + k + c * p];
+        w = sI[o + k + c * p];
+      }
+      __syncthreads();
+      if (lane_on && best != 0.0 && r != k && (f.re != 0.0 || f.im != 0.0)) {
+        sD[o + e].re -= f.re * u.re - f.im * u.im;
+        sD[o + e].im -= f.re * u.im + f.im * u.re;
+        sI[o + e].re -= f.re * w.re - f.im * w.im;
+        sI[o + e].im -= f.re * w.im + f.im * w.re;
+      }
+      __syncthreads();
+    }
+    if (lane_on) sF[o + e] = sI[o + e];
+  }
+  __syncthreads();
+  if (!live || r >= q || c >= q) return;
+  double *out = G + 2 * ((size_t)iz * q * q + r + c * q);
+  if (sfail[g]) {
+    out[0] = NAN;
+    out[1] = NAN;
+    return;
+  }
+  double sr = 0.0, si = 0.0;
+  for (int k = 0; k < p; ++k)
+    for (int l = 0; l < p; ++l) {
+      const double ww = W[k + r * p] * W[l + c * p];
+      sr += ww * sF[o + k + l * p].re;
+      si += ww * sF[o + k + l * p].im;
+    }
+  out[0] = sr;
+  out[1] = si;
 }
 
 int g_ncu = 0;
@@ -1361,6 +1415,74 @@ void set_unit(double *x, size_t i, hipStream_t s) {
 }
 
 namespace {
+// device-driven Lanczos step kernels; step k = *base + j.  Each workgroup
+// folds the previous kernel's per-block partials itself (the same values in
+// the same order in every workgroup, so every workgroup sees the same scalar)
+// and workgroup 0 stores the scalar; the grids stay at one workgroup per CU so
+// that the partial sets stay short.
+constexpr int kLzBlock = 1024;
+__device__ __forceinline__ double lz_fold(const double *part, int nb, double *sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kLzBlock) s += part[i];
+  s = block_sum<kLzBlock>(s, sh);
+  if (threadIdx.x == 0) sh[kLzBlock / kWave] = s;
+  __syncthreads();
+  return sh[kLzBlock / kWave];
+}
+// a[k] = <W, C> from the product's partials; W -= a[k] C + b[k-1] P; norm partials
+__global__ __launch_bounds__(kLzBlock) void k_lz_update(size_t n, double *W, const double *C, const double *P,
+                                                        double *coef, int cap, const int *base, int j,
+                                                        const double *dpart, int dnb, double *npart) {
+  __shared__ double red[kLzBlock / kWave + 1];
+  const int k = *base + j;
+  const double a = lz_fold(dpart, dnb, red);
+  if (blockIdx.x == 0 && threadIdx.x == 0) coef[k] = a;
+  const double bp = k > 0 ? coef[cap + k - 1] : 0.0;
+  double s = 0.0;
+  const size_t stride = (size_t)gridDim.x * kLzBlock;
+  if (k > 0) {
+    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
+      const double w = W[i] - a * C[i] - bp * P[i];
+      W[i] = w;
+      s += w * w;
+    }
+  } else {
+    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
+      const double w = W[i] - a * C[i];
+      W[i] = w;
+      s += w * w;
+    }
+  }
+  s = block_sum<kLzBlock>(s, red);
+  if (threadIdx.x == 0) npart[blockIdx.x] = s;
+}
+// b[k] = |W| from the update's partials; W /= b[k] (and copied to slots[k + 1]).
+// b[k] = 0 (an exhausted Krylov space) leaves non-finite values, never read:
+// the host's stop rule ends the recurrence at that step
+__global__ __launch_bounds__(kLzBlock) void k_lz_advance(size_t n, double *W, double *coef, int cap,
+                                                         const int *base, int j, const double *npart, int nnb,
+                                                         double *const *slots, int nslots) {
+  __shared__ double red[kLzBlock / kWave + 1];
+  const int k = *base + j;
+  const double b = sqrt(lz_fold(npart, nnb, red));
+  if (blockIdx.x == 0 && threadIdx.x == 0) coef[cap + k] = b;
+  const double inv = 1.0 / b;
+  double *dst = (slots && k + 1 < nslots) ? slots[k + 1] : nullptr;
+  const size_t stride = (size_t)gridDim.x * kLzBlock;
+  if (dst) {
+    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
+      const double w = W[i] * inv;
+      W[i] = w;
+      dst[i] = w;
+    }
+  } else {
+    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) W[i] *= inv;
+  }
+}
+__global__ void k_lz_next(int *base, int steps) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *base += steps;
+}
+
 __global__ __launch_bounds__(256) void k_group_project(ProjArgs A, const double *x, double *y) {
   const size_t n = (size_t)A.du * A.dd;
   for (size_t J = (size_t)blockIdx.x * 256 + threadIdx.x; J < n; J += (size_t)gridDim.x * 256) {
@@ -1375,6 +1497,26 @@ __global__ __launch_bounds__(256) void k_group_project(ProjArgs A, const double 
   }
 }
 }  // namespace
+
+static int lz_grid(size_t n) {
+  const size_t g = (n + kLzBlock - 1) / kLzBlock;
+  return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)num_cu()));
+}
+void lz_update(const LanczosDev &L, double *W, const double *C, const double *P, int j, int dnb, hipStream_t s) {
+  hipLaunchKernelGGL(k_lz_update, dim3(lz_grid(L.n)), dim3(kLzBlock), 0, s, L.n, W, C, P, L.coef, L.cap,
+                     (const int *)L.base, j, (const double *)L.part, dnb, L.part + kPartialsCap);
+  ED_HIP(hipGetLastError());
+}
+void lz_advance(const LanczosDev &L, double *W, int j, hipStream_t s) {
+  hipLaunchKernelGGL(k_lz_advance, dim3(lz_grid(L.n)), dim3(kLzBlock), 0, s, L.n, W, L.coef, L.cap,
+                     (const int *)L.base, j, (const double *)(L.part + kPartialsCap), lz_grid(L.n), L.slots,
+                     L.nslots);
+  ED_HIP(hipGetLastError());
+}
+void lz_next(const LanczosDev &L, int steps, hipStream_t s) {
+  hipLaunchKernelGGL(k_lz_next, dim3(1), dim3(64), 0, s, L.base, steps);
+  ED_HIP(hipGetLastError());
+}
 
 void group_project(const ProjArgs &A, const double *x, double *y, hipStream_t s) {
   const size_t n = (size_t)A.du * A.dd;
@@ -1450,7 +1592,9 @@ void mcf_eval(int p, int nfl, const double *A, const double *B, const double *W,
               double *G, hipStream_t s) {
   if (p > kMcfMax) fail(ED_ERR_UNSUPPORTED, "matrix continued fraction with block size > 8 on device");
   if (!nz) return;
-  hipLaunchKernelGGL(k_mcf_eval, dim3((nz + 63) / 64), dim3(64), 0, s, p, nfl, A, B, W, q, nz, z, G);
+  if (p < 1 || q < 0 || q > p) fail(ED_ERR_INVALID, "matrix continued fraction: bad block sizes");
+  const int per = 64 / (p * p);
+  hipLaunchKernelGGL(k_mcf_eval, dim3((nz + per - 1) / per), dim3(64), 0, s, p, nfl, A, B, W, q, nz, z, G);
   ED_HIP(hipGetLastError());
 }
 

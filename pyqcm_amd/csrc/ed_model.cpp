@@ -984,6 +984,9 @@ struct HamFactorized : Hamiltonian {
     if (part) pending_nb = nb;
     return nb;
   }
+  void prepare() override {
+    if (variant == HV_AUTO && !tuned) autotune();
+  }
   int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *aux,
                   size_t aux_cap) override {
     int v = variant;

@@ -142,6 +142,9 @@ struct Hamiltonian {
   // tuned, else the default one), larger partial sets in aux (aux_cap doubles)
   virtual int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *aux,
                           size_t aux_cap) = 0;
+  // settle the product's schedule (autotuning where the plan has it) before
+  // apply_fixed is queued or captured
+  virtual void prepare() {}
   // diagonal of H on device
   virtual void diag(double *d) = 0;
   double dot_result(int nb);  // sum of the partials of the last apply (deterministic order)

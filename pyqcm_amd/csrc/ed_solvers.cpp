@@ -72,6 +72,165 @@ double comb(size_t n, double *t, double st, const std::vector<const double *> &b
   return nn;
 }
 
+// ---------------------------------------------------------------------------
+// Device-driven Lanczos recurrence over normalized vectors (kernels lz_*): the
+// scalars of a step are folded, used and stored on the device, so a block of
+// steps runs without a host wait; the block is queued once under stream
+// capture and replayed as a hipGraph.  The host reads the new coefficients
+// once per launch and applies the caller's stop rule to them in step order, so
+// the recurrence it keeps is the one a per-step check would keep (the steps
+// queued past the stop are discarded).  Steps rotate over three buffers (step
+// k: C = R[k % 3], P = R[(k + 2) % 3], W = R[(k + 1) % 3]), hence blocks of a
+// multiple of 3 steps.  With slots, step k also copies v_{k+1} into a vector
+// of its own (the Krylov basis kept in HBM), allocated by the host before the
+// block that needs it, within a byte budget.
+// ---------------------------------------------------------------------------
+class LanczosDeviceRun {
+ public:
+  LanczosDeviceRun(Hamiltonian &H, const double *d_start, int max_steps, bool keep_basis, double budget)
+      : H(H), n(H.dim) {
+    Params &P = params();
+    K = (int)P.geti("lanczos_block");
+    graph = P.getb("lanczos_graph");
+    cap = ((max_steps + K - 1) / K + 1) * K;
+    for (auto &r : R) r.alloc(n);
+    coef.alloc(2 * (size_t)cap);
+    part.alloc(2 * (size_t)partials_cap());
+    base.alloc(1);
+    d2d(R[0].p, d_start, n);
+    const int zero = 0;
+    ED_HIP(hipMemcpyAsync(base.p, &zero, sizeof(int), hipMemcpyHostToDevice, stream()));
+    D.n = n;
+    D.cap = cap;
+    D.coef = coef.p;
+    D.base = base.p;
+    D.part = part.p;
+    if (keep_basis) {
+      slot_budget = budget;
+      table.alloc((size_t)cap + 1);
+      host_table.assign((size_t)cap + 1, nullptr);
+      basis.emplace_back(n);
+      d2d(basis[0].p, d_start, n);
+      host_table[0] = basis[0].p;
+      D.slots = table.p;
+      D.nslots = cap + 1;
+    }
+    H.prepare();
+    sync();
+  }
+  ~LanczosDeviceRun() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+  int block() const { return K; }
+  int queued() const { return done; }
+  // true while every queued step also stored its vector
+  bool basis_complete() const { return !basis.empty() && !basis_full; }
+  const double *vec(int k) const { return basis[(size_t)k].p; }
+  // queue nblk blocks (fewer near the coefficient capacity), wait, and append
+  // the new coefficients to a and b; false once the capacity is used up
+  bool run(int nblk, std::vector<double> &a, std::vector<double> &b) {
+    nblk = std::min(nblk, (cap - done) / K);
+    if (nblk <= 0) return false;
+    grow_basis(done + nblk * K);
+    for (int r = 0; r < nblk; ++r) {
+      if (!graph) {
+        enqueue();
+      } else if (exec) {
+        ED_HIP(hipGraphLaunch(exec, stream()));
+      } else if (done + r * K == 0) {
+        enqueue();  // the first block runs eagerly: lazy set-up happens outside the capture
+      } else {
+        capture();
+        ED_HIP(hipGraphLaunch(exec, stream()));
+      }
+    }
+    std::vector<double> h(2 * (size_t)nblk * K);
+    const size_t m = (size_t)nblk * K;
+    ED_HIP(hipMemcpyAsync(h.data(), coef.p + done, m * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    ED_HIP(hipMemcpyAsync(h.data() + m, coef.p + cap + done, m * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    sync();
+    a.insert(a.end(), h.begin(), h.begin() + m);
+    b.insert(b.end(), h.begin() + m, h.end());
+    done += (int)m;
+    return true;
+  }
+
+ private:
+  void enqueue() {
+    for (int j = 0; j < K; ++j) {
+      double *C = R[j % 3].p, *Pv = R[(j + 2) % 3].p, *W = R[(j + 1) % 3].p;
+      const int nb = H.apply_fixed(C, W, 1.0, 0.0, part.p, nullptr, 0);
+      lz_update(D, W, C, Pv, j, nb, stream());
+      lz_advance(D, W, j, stream());
+    }
+    lz_next(D, K, stream());
+  }
+  void capture() {
+    hipGraph_t g = nullptr;
+    ED_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(stream(), &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    ED_HIP(hipStreamEndCapture(stream(), &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    ED_HIP(e);
+  }
+  // vectors for slots 1 .. upto (step k writes slot k + 1) within the budget
+  void grow_basis(int upto) {
+    if (basis.empty() || basis_full) return;
+    const size_t first = basis.size();
+    while ((int)basis.size() <= upto) {
+      if ((double)(basis.size() + 1) * n * 8.0 > slot_budget) {
+        basis_full = true;
+        break;
+      }
+      basis.emplace_back(n);
+      host_table[basis.size() - 1] = basis.back().p;
+    }
+    if (basis.size() > first)
+      ED_HIP(hipMemcpyAsync(table.p + first, host_table.data() + first, (basis.size() - first) * sizeof(double *),
+                            hipMemcpyHostToDevice, stream()));
+  }
+  Hamiltonian &H;
+  size_t n;
+  int K = 6, cap = 0, done = 0;
+  bool graph = true, basis_full = false;
+  double slot_budget = 0.0;
+  DevBuf<double> R[3], coef, part;
+  DevBuf<int> base;
+  DevBuf<double *> table;
+  std::vector<double *> host_table;
+  std::vector<DevBuf<double>> basis;
+  LanczosDev D;
+  hipGraphExec_t exec = nullptr;
+};
+
+// whether the device-driven recurrence applies (a block length that is a
+// positive multiple of 3, a non-empty space)
+bool device_lanczos(size_t n) {
+  const long K = params().geti("lanczos_block");
+  if (K == 0 || n == 0) return false;
+  if (K < 0 || K % 3 != 0 || K > 96) fail(ED_ERR_INVALID, "lanczos_block must be 0 or a multiple of 3 up to 96");
+  return true;
+}
+
+// blocks to queue before the next host check: the steps the last six
+// log-decrements of a decreasing quantity predict to reach the target
+int predict_blocks(const std::vector<double> &logq, double target, int K) {
+  const size_t m = logq.size();
+  if (m < 7) return 2;
+  const double rate = (logq[m - 1] - logq[m - 7]) / 6.0;
+  if (!(rate < 0.0)) return 2;
+  const double need = (target - logq[m - 1]) / rate;
+  if (!(need > 0.0)) return 1;
+  return (int)std::min(8.0, std::max(1.0, std::ceil(need / K)));
+}
+
 }  // namespace
 
 double ddot(size_t n, const double *a, const double *b) {
@@ -89,6 +248,76 @@ double ddot(size_t n, const double *a, const double *b) {
 // eigenvector is one combination pass; otherwise (or when the HBM budget is
 // exceeded) the recurrence is replayed as in the reference.
 // ---------------------------------------------------------------------------
+// the same recurrence, device-driven: the stop rule (Ritz residual of the
+// lowest eigenvalue) is applied to the coefficients in step order; the
+// eigenvector is combined from the kept basis, or replayed when the basis did
+// not fit the HBM budget
+static LanczosGS lanczos_gs_device(Hamiltonian &H, const double *d_start, double *d_gs, bool store,
+                                   double budget) {
+  Params &P = params();
+  const int max_iter = (int)P.geti("max_iter_lanczos");
+  const double accur = P.getd("accur_lanczos");
+  const size_t n = H.dim;
+  LanczosDeviceRun run(H, d_start, max_iter + 1, store, budget);
+  LanczosGS R;
+  std::vector<double> a, b, logr;
+  int nblk = std::max(1, 24 / run.block());
+  bool conv = false;
+  while (!conv) {
+    check_cancel();
+    const size_t k0 = a.size();
+    if (!run.run(nblk, a, b))
+      fail(ED_ERR_NOT_CONVERGED, "Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+    for (size_t k = k0; k < a.size(); ++k) {
+      R.alpha.push_back(a[k]);
+      R.beta.push_back(b[k]);
+      double e0, zl;
+      tridiag_lowest((int)R.alpha.size(), R.alpha.data(), R.beta.data(), e0, zl,
+                     R.alpha.size() > 1 ? R.E0 : std::numeric_limits<double>::quiet_NaN());
+      R.E0 = e0;
+      const double ritz = std::fabs(zl * b[k]);
+      logr.push_back(std::log(std::max(ritz, 1e-300)));
+      if ((int)R.alpha.size() > max_iter)
+        fail(ED_ERR_NOT_CONVERGED, "Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+      if (!(ritz > accur)) {
+        conv = true;
+        break;
+      }
+    }
+    nblk = predict_blocks(logr, std::log(accur), run.block());
+  }
+  const int kk = (int)R.alpha.size();
+  R.niter = kk;
+  if (verbose()) vlog("device Lanczos: %d steps kept of %d queued", kk, run.queued());
+  std::vector<double> z;
+  double e0;
+  tridiag_eigvec0(kk, R.alpha.data(), R.beta.data(), e0, z);
+  if (run.basis_complete()) {
+    std::vector<const double *> src;
+    for (int j = 0; j < kk; ++j) src.push_back(run.vec(j));
+    comb(n, d_gs, 0.0, src, z, false);
+  } else {
+    // LanczosEigenvector replay (Lanczos.hpp:109-143); buffer A holds
+    // beta_{j-2} v_{j-1} (norm bA), B the vector before it (norm bB)
+    DevBuf<double> A(n), B(n);
+    double bA = 1.0, bB = 1.0;
+    d2d(A.p, d_start, n);
+    d2d(d_gs, d_start, n);
+    scale(n, d_gs, z[0], stream());
+    for (int j = 1; j < kk; ++j) {
+      check_cancel();
+      const double cb = (j > 1) ? -R.beta[j - 2] / bB : 0.0;
+      H.apply(A.p, B.p, 1.0 / bA, cb, false);
+      replay_step(n, B.p, A.p, d_gs, -R.alpha[j - 1] / bA, z[j] / R.beta[j - 1], stream());
+      std::swap(A.p, B.p);
+      bB = bA;
+      bA = R.beta[j - 1];
+    }
+  }
+  sync();
+  return R;
+}
+
 LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs, const GroupProjector *proj) {
   Params &P = params();
   const int max_iter = (int)P.geti("max_iter_lanczos");
@@ -99,6 +328,7 @@ LanczosGS lanczos_gs(Hamiltonian &H, const double *d_start, double *d_gs, const 
   const bool reorth = P.getb("lanczos_reorth");
   // a projected recurrence keeps its basis (the replay would have to project too)
   bool store = P.getb("store_lanczos_basis") || reorth || proj;
+  if (!reorth && !proj && device_lanczos(n)) return lanczos_gs_device(H, d_start, d_gs, store, budget);
   LanczosGS R;
   std::vector<DevBuf<double>> basis;  // basis[k] = s_k v_k
   std::vector<double> sc;             // s_k
@@ -244,6 +474,32 @@ void lanczos_green(Hamiltonian &H, const double *d_psi, std::vector<double> &a, 
   const bool reorth = P.getb("lanczos_reorth");
   a.clear();
   b.assign(1, 1.0);
+  if (!reorth && max_cf > 0 && device_lanczos(n)) {
+    // device-driven: the stop rule applied to the coefficients in step order
+    LanczosDeviceRun run(H, d_psi, (int)std::min<size_t>(max_cf, 1u << 20), false, 0.0);
+    std::vector<double> na, nbv, logp;
+    double prod = 1.0;
+    int nblk = std::max(1, 12 / run.block());
+    bool stop = false;
+    while (!stop) {
+      check_cancel();
+      const size_t k0 = na.size();
+      if (!run.run(nblk, na, nbv)) break;
+      for (size_t k = k0; k < na.size(); ++k) {
+        if (!(prod > acc && std::fabs(b.back()) > acc && a.size() < max_cf)) {
+          stop = true;
+          break;
+        }
+        a.push_back(na[k]);
+        b.push_back(nbv[k]);
+        prod *= std::fabs(nbv[k]);
+        logp.push_back(std::log(std::max(std::min(prod, std::fabs(nbv[k])), 1e-300)));
+      }
+      if (!(prod > acc && std::fabs(b.back()) > acc && a.size() < max_cf)) stop = true;
+      nblk = predict_blocks(logp, std::log(acc), run.block());
+    }
+    return;
+  }
   DevBuf<double> A(n), B(n);
   d2d(A.p, d_psi, n);
   std::vector<DevBuf<double>> basis;

@@ -43,7 +43,7 @@ _PARAM_TYPES = {
     "verb_ED": _BOOL, "verb_Hilbert": _BOOL, "verb_warning": _BOOL, "parallel_sectors": _BOOL,
     "merge_states": _BOOL, "combine_mcf": _BOOL, "block_Lanczos_QR": _BOOL, "store_lanczos_basis": _BOOL,
     "sell16": _BOOL, "lanczos_reorth": _BOOL, "host_random_start": _BOOL,
-    "poll_reductions": _BOOL,
+    "poll_reductions": _BOOL, "lanczos_graph": _BOOL,
     "accur_band_lanczos": _DOUBLE, "accur_continued_fraction": _DOUBLE, "accur_Davidson": _DOUBLE,
     "accur_deflation": _DOUBLE, "accur_lanczos": _DOUBLE, "accur_Q_matrix": _DOUBLE,
     "band_lanczos_minimum_gap": _DOUBLE, "minimum_weight": _DOUBLE, "Qmatrix_wtol": _DOUBLE,
@@ -51,6 +51,7 @@ _PARAM_TYPES = {
     "n_states": _INT, "max_dim_full": _INT, "max_dim_print": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
     "sell_sort_window": _INT, "hv_block": _INT, "hv_row_order": _INT, "vector_block": _INT, "reduction_blocks_per_cu": _INT,
+    "lanczos_block": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 
@@ -511,6 +512,55 @@ def combined_mcf(label=0, spin_down=0):
     A = [A[j * P * P:(j + 1) * P * P].reshape(P, P).T for j in range(nfl.value)]
     B = [B[j * P * P:(j + 1) * P * P].reshape(P, P).T for j in range(nfl.value)]
     return W, A, B
+
+
+def continued_fractions(label=0, spin_down=0, state=0):
+    """continued fractions of low-energy state `state` of a solved instance
+    (GF_method 'F'; continued_fraction_set, src_ed/continued_fraction_set.cpp:12-98):
+    (a, b, lengths), fraction f = 2 q + eh for the orbital pair q = o1 (o1 + 1) / 2 + o2
+    (o1 >= o2), eh 0 electron / 1 hole; a, b are (nfrac, max length) arrays as
+    evaluated (a_n shifted by the state energy, b_n squared, b_0 the weighted
+    norm), zero past each fraction's length."""
+    args = (int(label), int(spin_down), int(state))
+    nfr = C.c_int()
+    check(lib().ed_continued_fractions(*args, 0, None, None, None, C.byref(nfr)))
+    nf = nfr.value
+    ln = np.zeros(nf, dtype=np.int32)
+    one = np.zeros(1)
+    check(lib().ed_continued_fractions(*args, 0, _dptr(one), _dptr(one), ln.ctypes.data_as(C.POINTER(C.c_int)),
+                                       C.byref(nfr)))
+    cap = max(1, int(ln.max(initial=0)))
+    a = np.zeros(nf * cap)
+    b = np.zeros(nf * cap)
+    check(lib().ed_continued_fractions(*args, cap, _dptr(a), _dptr(b), ln.ctypes.data_as(C.POINTER(C.c_int)),
+                                       C.byref(nfr)))
+    return a.reshape(nf, cap), b.reshape(nf, cap), ln
+
+
+def continued_fractions_pairs(label, pairs, ncf, spin_down=0):
+    """the first ncf levels of the fractions of the orbital pairs (o1, o2)
+    (0-based) from the ground state of instance `label`: (a, b, lengths) with
+    fraction 2 k + eh for pair k (layout of continued_fractions)."""
+    n = len(pairs)
+    o1 = (C.c_int * n)(*[int(x) for (x, y) in pairs])
+    o2 = (C.c_int * n)(*[int(y) for (x, y) in pairs])
+    a = np.zeros((2 * n, ncf))
+    b = np.zeros((2 * n, ncf))
+    ln = np.zeros(2 * n, dtype=np.int32)
+    check(lib().ed_continued_fractions_pairs(int(label), int(spin_down), n, o1, o2, int(ncf), _dptr(a), _dptr(b),
+                                             ln.ctypes.data_as(C.POINTER(C.c_int))))
+    return a, b, ln
+
+
+def set_continued_fractions(label, E0, sector, spin_down, a, b, lengths):
+    """instance `label` becomes solved with one state of energy E0 in `sector`
+    whose spin block holds these fractions (layout of continued_fractions);
+    call once per spin block."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    ln = np.ascontiguousarray(lengths, dtype=np.int32)
+    check(lib().ed_set_continued_fractions(int(label), float(E0), sector.encode(), int(spin_down), a.shape[0],
+                                           a.shape[1], _dptr(a), _dptr(b), ln.ctypes.data_as(C.POINTER(C.c_int))))
 
 
 def states(label=0):

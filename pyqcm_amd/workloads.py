@@ -218,6 +218,34 @@ def vca_cluster(nx=2, ny=2):
                     list(range(L)))
 
 
+def vca_cluster_bath(nx=2, ny=3, n_bath=6, seed=7, N=None):
+    """vca_cluster(nx, ny) with n_bath bath orbitals: energies eps_b ~ U[-2, 2]
+    and hybridisations V_b ~ U[0.2, 0.8] from default_rng(seed) (eps first),
+    bath b coupled to cluster site b mod (nx ny); one target sector N (default
+    nx ny, i.e. quarter filling of the ns + n_bath orbitals), S = 0.  The 2x3
+    cluster with 6 bath orbitals at N = 6 has 220 x 220 = 48 400 states."""
+    base = vca_cluster(nx, ny)
+    ns = nx * ny
+    L = ns + n_bath
+    N = ns if N is None else N
+    rng = np.random.default_rng(seed)
+    eps = rng.uniform(-2.0, 2.0, n_bath)
+    hyb = rng.uniform(0.2, 0.8, n_bath)
+
+    def widen(el):  # spin-down orbitals move from index ns + i to L + i
+        return [((r if r < ns else r - ns + L), (c if c < ns else c - ns + L), v) for (r, c, v) in el]
+    ops = {k: (typ, widen(el)) for k, (typ, el) in base.operators.items()}
+    values = dict(base.values)
+    for b in range(n_bath):
+        o = ns + b
+        ops["eb%d" % (b + 1)] = ("one-body", [(o, o, 1.0), (o + L, o + L, 1.0)])
+        s = b % ns
+        ops["tb%d" % (b + 1)] = ("one-body", [(s, o, 1.0), (s + L, o + L, 1.0)])
+        values["eb%d" % (b + 1)] = float(eps[b])
+        values["tb%d" % (b + 1)] = float(hyb[b])
+    return Workload("vca%dx%d_%db" % (nx, ny, n_bath), ns, n_bath, ops, values, _sector(N), list(range(ns)))
+
+
 def vca_points(n=512, seed=4):
     """n parameter points of the sweep: U ~ U[2, 10], M ~ U[0, 0.3], mu ~ U[-1, 1]
     (numpy default_rng(seed)), t = 1."""

@@ -106,6 +106,27 @@ def test_batch_large_sector_four_wave_kernel():
              dense=False)
 
 
+def test_batch_48400_state_sector_global_memory(oracle):
+    """The 2x3 cluster with 6 bath orbitals at N = 6 (220 x 220 = 48 400
+    states; the fractions run in the N +- 1 sectors of 14 520 and 108 900
+    states): every item goes through the global-memory workgroups.  E0 against
+    the oracle's Lanczos, E0 and G against the per-instance path."""
+    wl = W.vca_cluster_bath(2, 3, 6)
+    pts = [{**wl.values, **v} for v in W.vca_points(2, seed=4)]   # bath values kept
+    _compare(wl, pts, "F", W.matsubara(16, 20.0), dense=False)
+    for vals in pts:
+        w = dataclasses.replace(wl, values=vals)
+        M = oracle.Model(w.L, w.terms())
+        x = np.random.default_rng(0).normal(size=M.dim(3, 3))
+        E0r = M.lanczos_gs(3, 3, x)[0]
+        lab = _new_label()
+        qcm.new_model_instanceC(_model(wl), vals, wl.sector, lab)
+        qcm.solve_batch([lab])
+        E0 = qcm.ground_state_solve(lab)[0]
+        assert abs(E0 - E0r) <= 1e-10 * abs(E0r), (vals, E0, E0r)
+        qcm.erase_model_instance(lab)
+
+
 def test_batch_mixed_formats_and_empty():
     """An empty batch is a no-op; instances built with the CSR format ('S')
     are solved one by one inside the same call, next to batched 'G' ones, and

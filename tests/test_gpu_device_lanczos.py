@@ -1,0 +1,127 @@
+"""The device-driven Lanczos recurrence (scalars folded and used on the device,
+blocks of steps replayed as a hipGraph, one host check per launch) against the
+host-driven one (a host wait per step, lanczos_block = 0): the same stop step,
+the same ground-state energy and vector, the same continued fractions, with the
+Krylov basis kept in HBM or replayed, and on sectors whose Krylov space runs
+out (b_k -> 0) before the stop rule.  Both are checked against the oracle's
+Lanczos and the dense spectrum."""
+import numpy as np
+import pytest
+
+from pyqcm_amd import _lib, qcm, workloads as W
+from tests.dense_ref import dense_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+_label = [9600]
+_MODES = [(6, True), (6, False), (3, True), (12, True), (24, True)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    _lib.require_device()
+    qcm.set_global_parameterC("Hamiltonian_format", "G")
+    qcm.set_global_parameterC("GF_method", "L")
+    yield
+    qcm.set_global_parameterC("lanczos_block", 6)
+    qcm.set_global_parameterC("lanczos_graph", True)
+    qcm.set_global_parameterC("store_lanczos_basis", True)
+
+
+def _mode(block, graph):
+    qcm.set_global_parameterC("lanczos_block", block)
+    qcm.set_global_parameterC("lanczos_graph", graph)
+
+
+def _ham(wl, sector):
+    _label[0] += 1
+    W.setup(wl, label=_label[0], sectors=sector, model_name="dlz_%d" % _label[0])
+    return qcm.Hamiltonian(_label[0], sector, "G")
+
+
+@pytest.mark.parametrize("store", [True, False])
+def test_ground_state_same_as_host_driven(oracle, store):
+    wl = W.plaquette_bath(n_bath=4)
+    H = _ham(wl, "R0:N8:S0")
+    x = np.random.default_rng(11).normal(size=H.dim)
+    qcm.set_global_parameterC("store_lanczos_basis", store)
+    try:
+        _mode(0, False)
+        E0h, gsh, nh = H.lanczos_gs(x)
+        for block, graph in _MODES:
+            _mode(block, graph)
+            E0, gs, n = H.lanczos_gs(x)
+            print("block", block, "graph", graph, "steps", n, "host", nh, flush=True)
+            assert abs(n - nh) <= 1, (block, graph, n, nh)
+            assert abs(E0 - E0h) <= 1e-12 * abs(E0h), (block, graph, E0, E0h)
+            assert abs(abs(gs @ gsh) / (np.linalg.norm(gs) * np.linalg.norm(gsh)) - 1.0) < 1e-10
+    finally:
+        _mode(6, True)
+        qcm.set_global_parameterC("store_lanczos_basis", True)
+    E0r, gsr, _, _ = oracle.Model(wl.L, wl.terms()).lanczos_gs(4, 4, x)
+    assert abs(E0h - E0r) <= 1e-10 * abs(E0r)
+
+
+def test_continued_fractions_same_as_host_driven():
+    """60 levels of the fraction of a random vector (3920 states)"""
+    wl = W.plaquette_bath(n_bath=4)
+    H = _ham(wl, "R0:N7:S1")
+    psi = np.random.default_rng(12).normal(size=H.dim)
+    psi /= np.linalg.norm(psi)
+    qcm.set_global_parameterC("max_iter_CF", 60)
+    try:
+        _mode(0, False)
+        ah, bh = H.lanczos_green(psi)
+        assert len(ah) == 60
+        for block, graph in _MODES:
+            _mode(block, graph)
+            a, b = H.lanczos_green(psi)
+            assert len(a) == len(ah) and len(b) == len(bh), (block, graph, len(a), len(ah))
+            assert np.abs(a - ah).max() < 1e-9 and np.abs(b - bh).max() < 1e-9, (block, graph)
+    finally:
+        _mode(6, True)
+        qcm.set_global_parameterC("max_iter_CF", 400)
+
+
+@pytest.mark.parametrize("N,S", [(0, 0), (1, 1), (11, 1)])
+def test_exhausted_krylov_space(N, S):
+    """a 6-site chain's sectors of 1 and 6 states (empty, one electron, one
+    hole): the recurrence runs into b_k ~ 0 (the device steps queued past it
+    are discarded); the fractions end at the same level as the host-driven
+    ones, the ground state is the dense one, and the fraction's poles are
+    eigenvalues of H"""
+    wl = W.hubbard_chain(6, U=4.0, mu=2.0)
+    sec = "R0:N%d:S%d" % (N, S)
+    H = _ham(wl, sec)
+    Nup, Ndw = (N + S) // 2, (N - S) // 2
+    ev = np.linalg.eigvalsh(dense_hamiltonian(wl.L, wl.terms(), Nup, Ndw))
+    x = np.random.default_rng(13).normal(size=H.dim)
+    xn = x / np.linalg.norm(x)
+    try:
+        _mode(0, False)
+        ah, bh = H.lanczos_green(xn)
+        assert len(ah) <= H.dim
+        for block, graph in [(0, False)] + _MODES:
+            _mode(block, graph)
+            E0, gs, n = H.lanczos_gs(x)
+            assert abs(E0 - ev[0]) <= 1e-10 * max(1.0, abs(ev[0])), (block, graph, E0, ev[0])
+            a, b = H.lanczos_green(xn)
+            assert len(a) == len(ah), (block, graph, len(a), len(ah))
+            assert np.abs(a - ah).max() < 1e-9, (block, graph)
+            T = np.diag(a) + np.diag(b[1:len(a)], 1) + np.diag(b[1:len(a)], -1)
+            poles = np.linalg.eigvalsh(T)
+            assert np.abs(poles[:, None] - ev[None, :]).min(axis=1).max() < 1e-8, (block, graph)
+    finally:
+        _mode(6, True)
+
+
+def test_invalid_block_is_an_error():
+    wl = W.hubbard_chain(4, U=4.0, mu=2.0)
+    H = _ham(wl, "R0:N4:S0")
+    x = np.random.default_rng(14).normal(size=H.dim)
+    try:
+        _mode(5, True)
+        with pytest.raises(_lib.EDError):
+            H.lanczos_gs(x)
+    finally:
+        _mode(6, True)
