@@ -473,6 +473,8 @@ def main():
             out["batch"] = _batch_leg(args)
         if not args.no_lattice:
             out["lattice"] = _lattice_leg(args)
+        if not args.no_surrogate:
+            out["surrogate"] = _surrogate_leg(args)
         print(json.dumps(out))
         return
     H, wl, ms, wall = bench_hv(args.steps, args.warmup, args.variant)

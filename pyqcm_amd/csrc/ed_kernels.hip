@@ -749,6 +749,62 @@ __global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, 
   }
 }
 
+// One band-Lanczos step's vector work in one pass: x scaled, the NA vectors
+// ahead updated from it, t combined from the NB vectors behind and x (the same
+// operations, in the same order, as k_multi_axpy_src followed by k_combine),
+// and the dots of the next step's vector c (a_0, or t when NA == 0) with the
+// vectors of its window, so that step needs no separate multi-dot pass.
+template <int BS, int NA, int NB>
+__global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs, MPtrPack a, CoefPack ca, double *t,
+                                                  double st, PtrPack w, CoefPack cb, double cx, double *part) {
+  constexpr int ND = (NA > 0 ? 1 + NA : 1) + NB + 1;
+  __shared__ double red[ND][BS / kWave];
+  double acc[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+    const double xv = x[i] * xs;
+    x[i] = xv;
+    double av[NA > 0 ? NA : 1];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+      av[k] = a.p[k][i] + ca.c[k] * xv;
+      a.p[k][i] = av[k];
+    }
+    double wv[NB > 0 ? NB : 1];
+    double tv = st * t[i];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      wv[b] = w.p[b][i];
+      tv += cb.c[b] * wv[b];
+    }
+    tv += cx * xv;
+    t[i] = tv;
+    const double c = NA > 0 ? av[0] : tv;
+    acc[0] += c * c;
+    int r = 1;
+#pragma unroll
+    for (int k = 1; k < NA; ++k) acc[r++] += c * av[k];
+    if (NA > 0) acc[r++] += c * tv;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[r++] += c * wv[b];
+    acc[r] += c * xv;
+  }
+  const int lane = threadIdx.x & (kWave - 1), wv_ = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const double sm = wave_sum(acc[k]);
+    if (lane == 0) red[k][wv_] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x < ND) {
+    double sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < BS / kWave; ++q) sm += red[threadIdx.x][q];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sm;
+  }
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_scale(size_t n, double *x, double a) {
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] *= a;
@@ -1347,6 +1403,30 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
     case 512: return multi_axpy_bs<512>(n, x, m, pk, cp, part, s, xs);
     default: return multi_axpy_bs<256>(n, x, m, pk, cp, part, s, xs);
   }
+}
+
+int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
+              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s) {
+  constexpr int BS = 1024;
+  if (na < 0 || na > kBandStepMax || nw < 0 || nw > kBandStepMax || vector_block() != BS || n == 0) return 0;
+  MPtrPack ap{};
+  CoefPack cap{}, cbp{};
+  PtrPack wp{};
+  for (int k = 0; k < na; ++k) { ap.p[k] = a[k]; cap.c[k] = ca[k]; }
+  for (int k = 0; k < nw; ++k) { wp.p[k] = w[k]; cbp.c[k] = cb[k]; }
+  const int g = grid_for(n, BS, reduction_blocks_per_cu());
+#define ED_BS(NA, NB)                                                                                         \
+  if (na == NA && nw == NB) {                                                                                \
+    hipLaunchKernelGGL((k_band_step<BS, NA, NB>), dim3(g), dim3(BS), 0, s, n, x, xs, ap, cap, t, st, wp, cbp, \
+                       cx, part);                                                                            \
+    ED_HIP(hipGetLastError());                                                                               \
+    return g;                                                                                                \
+  }
+#define ED_BS_ROW(NA) ED_BS(NA, 0) ED_BS(NA, 1) ED_BS(NA, 2) ED_BS(NA, 3) ED_BS(NA, 4)
+  ED_BS_ROW(0) ED_BS_ROW(1) ED_BS_ROW(2) ED_BS_ROW(3) ED_BS_ROW(4)
+#undef ED_BS_ROW
+#undef ED_BS
+  return 0;
 }
 
 template <int BS>

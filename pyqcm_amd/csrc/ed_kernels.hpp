@@ -128,6 +128,15 @@ int multi_dot(size_t n, const double *a, int m, const double *const *b, double *
 // norm_partials, partials of ||y_0||^2 after the update. returns nb
 int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c, double *norm_partials,
                    hipStream_t s, double xs = 1.0);
+// One band-Lanczos step's vector updates in one pass (no deflated vectors):
+//   x *= xs;  a_k += ca_k x (k < na);  t = st t + sum_b cb_b w_b + cx x;
+// and the dots the next step needs with c = a_0 (t when na == 0), as partials
+// (row r at part + r * nb): <c, c>, <c, a_k> (1 <= k < na), <c, t> (na > 0),
+// <c, w_b> (b < nw), <c, x>.  na, nw <= kBandStepMax; returns nb (0 when the
+// counts or the vector block size are outside what the kernel is built for)
+constexpr int kBandStepMax = 4;
+int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
+              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s);
 // t = st * t + sum_k c_k b_k ; optional partials of ||t||^2 and <t, w>. returns nb
 int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *norm_partials,
             const double *w, double *dot_partials, hipStream_t s);

@@ -570,30 +570,77 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
   }
   int nd = 0, j;
   double ev_old = 1e12, ev_change = 0.0;
+  // dots of the current v_j computed by the previous step's fused update
+  // (band_step): partials slot, block count, rows, and the layout of the rows
+  double *pre = nullptr;
+  int pre_nb = 0, pre_m = 0, pre_na = 0, pre_nw = 0, pre_skip = 0;
+  // H v_j queued ahead by the previous step (pre_h), with its partials count
+  bool pre_h = false;
+  int pre_hnb = 0;
+  // a queued product or fused dots are drained before the vectors go away
+  auto drain = [&]() {
+    if (pre_h) H.dot_result(pre_hnb);
+    if (pre) sum_partials(pre, pre_nb, pre_m, nullptr);
+    pre_h = false;
+    pre = nullptr;
+  };
   for (j = 0; j < M0; ++j) {
-    check_cancel();
+    try {
+      check_cancel();
+    } catch (...) {
+      drain();
+      throw;
+    }
     // one multi-dot per step: ||v_j||^2, <v_j, phi_k> (P matrix), the step (6)
     // overlaps <v_j, v_k> (k = j+1 .. j+pc-1) and <v_j, v_k> for the vectors
     // step (8) subtracts (k0 <= k < j), all against the unnormalized v_j
     const int jm = j + pc;
     const int k0 = (j > pc) ? j - pc : 0;
-    std::vector<const double *> db{v[j]->p};
-    for (int k = 0; k < p; ++k) db.push_back(phi[k]);
-    for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
-    for (int k = k0; k < j; ++k) db.push_back(v[k]->p);
-    // multi-dot and H v_j (on the unnormalized v_j) are queued together; the
-    // host then waits once for both results
-    std::vector<const double *> db1(db.begin(), db.begin() + std::min<size_t>(db.size(), 16));
-    double *pdot = partials_acquire();
-    const int dnb = multi_dot(dim, v[j]->p, (int)db1.size(), db1.data(), pdot, stream());
-    v[j + pc].reset(new DevBuf<double>(dim));
-    const int hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
+    // the P matrix enters Q only through its first p columns (build_Q_matrix,
+    // Hamiltonian_base.hpp:244, the product over phi.size() columns), so
+    // <v_j, phi_k> is computed for j < p only
+    const int np = (j < p) ? p : 0;
     std::vector<double> dv;
-    sum_partials(pdot, dnb, (int)db1.size(), &dv);
-    if (db.size() > db1.size()) {  // more than 16 vectors (p > 5): remaining dots
-      std::vector<const double *> rest(db.begin() + 16, db.end());
-      const std::vector<double> dr = dots(dim, v[j]->p, rest);
-      dv.insert(dv.end(), dr.begin(), dr.end());
+    int hnb;
+    if (pre) {
+      // the dots came with the previous step's update (and H v_j was queued
+      // there too, unless that step ended the range): wait once for both
+      if (pre_h) {
+        hnb = pre_hnb;
+        pre_h = false;
+      } else {
+        v[j + pc].reset(new DevBuf<double>(dim));
+        hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
+      }
+      std::vector<double> raw;
+      sum_partials(pre, pre_nb, pre_m, &raw);
+      pre = nullptr;
+      // raw: <c,c>, the window ahead (pre_na), the vectors behind (pre_nw), <c, v_{j-1}>
+      dv.push_back(raw[0]);
+      for (int r = 1; r <= pre_na; ++r) dv.push_back(raw[r]);
+      for (int r = 1 + pre_na + pre_skip; r <= pre_na + pre_nw + 1; ++r) dv.push_back(raw[r]);
+      if ((int)dv.size() != 1 + np + (jm - j - 1) + (j - k0)) {
+        H.dot_result(hnb);
+        fail(ED_ERR_INTERNAL, "band Lanczos: fused dots do not match the step's window");
+      }
+    } else {
+      std::vector<const double *> db{v[j]->p};
+      for (int k = 0; k < np; ++k) db.push_back(phi[k]);
+      for (int k = j + 1; k < jm; ++k) db.push_back(v[k]->p);
+      for (int k = k0; k < j; ++k) db.push_back(v[k]->p);
+      // multi-dot and H v_j (on the unnormalized v_j) are queued together; the
+      // host then waits once for both results
+      std::vector<const double *> db1(db.begin(), db.begin() + std::min<size_t>(db.size(), 16));
+      double *pdot = partials_acquire();
+      const int dnb = multi_dot(dim, v[j]->p, (int)db1.size(), db1.data(), pdot, stream());
+      v[j + pc].reset(new DevBuf<double>(dim));
+      hnb = H.apply(v[j]->p, v[j + pc]->p, 1.0, 0.0, true);
+      sum_partials(pdot, dnb, (int)db1.size(), &dv);
+      if (db.size() > db1.size()) {  // more than 16 vectors (p > 5): remaining dots
+        std::vector<const double *> rest(db.begin() + 16, db.end());
+        const std::vector<double> dr = dots(dim, v[j]->p, rest);
+        dv.insert(dv.end(), dr.begin(), dr.end());
+      }
     }
     const double hd_raw = H.dot_result(hnb);  // <H v_j, v_j> before normalization
     const double z = std::sqrt(std::max(dv[0], 0.0));
@@ -615,28 +662,18 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
     }
     const double iz = 1.0 / z;
     if (j >= pc) T(j, j - pc) = z;
-    for (int k = 0; k < p; ++k) Pmat[(size_t)k + (size_t)j * p] = dv[1 + k] * iz;
-    if (jm > j + 1) {
-      std::vector<double *> ys;
-      std::vector<double> cs;
-      for (int k = j + 1; k < jm; ++k) {
-        const double zz = dv[1 + p + (k - j - 1)] * iz;
-        ys.push_back(v[k]->p);
-        cs.push_back(-zz);
-        if (k >= pc) T(j, k - pc) = zz;
-      }
-      // v_j /= z in the same pass (only the first chunk scales)
-      for (size_t s0 = 0; s0 < ys.size(); s0 += 16) {
-        const int m = (int)std::min<size_t>(16, ys.size() - s0);
-        multi_axpy_src(dim, v[j]->p, m, ys.data() + s0, cs.data() + s0, nullptr, stream(), s0 == 0 ? iz : 1.0);
-      }
-    } else {
-      scale(dim, v[j]->p, iz, stream());
+    for (int k = 0; k < np; ++k) Pmat[(size_t)k + (size_t)j * p] = dv[1 + k] * iz;
+    std::vector<double *> ys;
+    std::vector<double> cs;
+    for (int k = j + 1; k < jm; ++k) {
+      const double zz = dv[1 + np + (k - j - 1)] * iz;
+      ys.push_back(v[k]->p);
+      cs.push_back(-zz);
+      if (k >= pc) T(j, k - pc) = zz;
     }
     // step (7): v[j+pc] = H v_j for the normalized v_j: the product above
     // times 1/z, applied in the combination below
     const double hd = hd_raw * iz * iz;
-    if (j > pc && def[k0 - 1] == 0) v[k0 - 1].reset();
     // step (8): v[j+pc] -= sum_{k0 <= k < j} t(k, j) v_k
     std::vector<const double *> src;
     std::vector<double> cf;
@@ -645,8 +682,52 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
       T(k, j) = T(j, k);
       src.push_back(v[k]->p);
       cf.push_back(-T(k, j));
-      wdot -= T(k, j) * dv[1 + p + (jm - j - 1) + (k - k0)] * iz;
+      wdot -= T(k, j) * dv[1 + np + (jm - j - 1) + (k - k0)] * iz;
     }
+    // steps (6) - (8) in one pass, with the next step's dots (no deflated
+    // vectors, windows of at most kBandStepMax vectors); the next step uses
+    // the dots when it needs no P-matrix column
+    if (nd == 0 && (int)ys.size() <= kBandStepMax && (int)src.size() <= kBandStepMax) {
+      double *slot = partials_acquire();
+      const int nb = band_step(dim, v[j]->p, iz, (int)ys.size(), ys.data(), cs.data(), v[j + pc]->p, iz,
+                               (int)src.size(), src.data(), cf.data(), -wdot, slot, stream());
+      if (nb > 0) {
+        const int na = (int)ys.size(), nw = (int)src.size();
+        const int m = (na > 0 ? 1 + na : 1) + nw + 1;
+        if (j + 1 >= p) {
+          pre = slot;
+          pre_nb = nb;
+          pre_m = m;
+          pre_na = na;
+          pre_nw = nw;
+          pre_skip = ((j + 1 > pc) ? j + 1 - pc : 0) - k0;
+          // queue the next step's product now, so that the host's
+          // convergence check below overlaps it (wasted if this step ends
+          // the run; drained then)
+          if (j + 1 < M0) {
+            v[j + 1 + pc].reset(new DevBuf<double>(dim));
+            pre_hnb = H.apply(v[j + 1]->p, v[j + 1 + pc]->p, 1.0, 0.0, true);
+            pre_h = true;
+          }
+        } else {
+          sum_partials(slot, nb, m, nullptr);  // release the slot
+        }
+        if (j > pc && def[k0 - 1] == 0) v[k0 - 1].reset();
+        T(j, j) = wdot;
+        goto converge;
+      }
+      sum_partials(slot, 0, 0, nullptr);
+    }
+    if (jm > j + 1) {
+      // v_j /= z in the same pass (only the first chunk scales)
+      for (size_t s0 = 0; s0 < ys.size(); s0 += 16) {
+        const int m = (int)std::min<size_t>(16, ys.size() - s0);
+        multi_axpy_src(dim, v[j]->p, m, ys.data() + s0, cs.data() + s0, nullptr, stream(), s0 == 0 ? iz : 1.0);
+      }
+    } else {
+      scale(dim, v[j]->p, iz, stream());
+    }
+    if (j > pc && def[k0 - 1] == 0) v[k0 - 1].reset();
     if (nd == 0) {
       // t(j, j) = <v[j+pc], v_j> after step (8), from the dots above; the
       // scaling, the subtraction and the projection on v_j are one pass
@@ -669,6 +750,7 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
     }
     T(j, j) = wdot;
     for (int k = 0; k < nd; ++k) T(j, Idef[k]) = T(Idef[k], j);
+  converge:
     // step (12): convergence of the lowest eigenvalue of the projected matrix
     if (j >= 4 * p && j % p == 0) {
       double e0, e1 = 0.0;
@@ -695,12 +777,15 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
       const double test = std::fabs(e0 - ev_old);
       if (ev_old < 1e11) ev_change = test;
       ev_old = e0;
-      if (no_deg && e1 - e0 < min_gap)
+      if (no_deg && e1 - e0 < min_gap) {
+        drain();
         fail(ED_ERR_DEGENERATE, "band Lanczos: the gap between the first two eigenvalues is smaller than " +
                                     std::to_string(min_gap));
+      }
       if (test < accur_bl) break;
     }
   }
+  drain();  // the product and dots of a step not taken
   const int M = j;
   vlog("band Lanczos: %d steps, %d deflations, dim %zu", M, nd, dim);
   BandLanczosRun out;

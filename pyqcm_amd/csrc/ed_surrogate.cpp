@@ -154,9 +154,12 @@ void Surrogate::ensure_batch(int B) {
   part.alloc(1024);
   // split-k partials of the weight gradients: S slabs of the largest layer
   split_S = std::max(1, std::min(32, B / 128));
-  size_t wmax = 0;
-  for (const Layer &L : layers) wmax = std::max(wmax, (size_t)L.out_pad * L.in_pad);
-  split.alloc((size_t)split_S * wmax);
+  size_t wmax = 0, omax = 0;
+  for (const Layer &L : layers) {
+    wmax = std::max(wmax, (size_t)L.out_pad * L.in_pad);
+    omax = std::max(omax, (size_t)L.out_pad);
+  }
+  split.alloc(std::max((size_t)split_S * wmax, (size_t)mlp_colsum_chunks(B) * omax));
   cap_B = B;
 }
 
@@ -218,7 +221,8 @@ int Surrogate::backward(int B, float *partials) {
     g.split_stride = (long)L.out_pad * L.in_pad;
     mlp_gemm(g, EPI_STORE, stream());
     mlp_split_reduce(split_S, L.out, L.in, L.in_pad, g.split_stride, split.p, L.gW.p, stream());
-    mlp_colsum(B, L.out, ldcur, dcur, L.gb.p, stream());
+    // the split partials are folded: their buffer holds the column-sum chunks
+    mlp_colsum(B, L.out, ldcur, dcur, L.gb.p, split.p, stream());
     if (l == 0) break;
     // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
     GemmArgs h;

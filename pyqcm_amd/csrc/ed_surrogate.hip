@@ -143,16 +143,26 @@ __global__ __launch_bounds__(256) void k_loss_grad(int B, int n_out, int ld, con
   if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// 64 columns per block, 4 row groups, fixed-order sums
-__global__ __launch_bounds__(256) void k_colsum(int M, int N, int ld, const bf16_t *dz, float *db) {
+// column sums in two fixed-order stages: block (x, r) sums rows [r rc, (r+1) rc)
+// of 64 columns (4 row groups) into part[r N + n]; k_colsum_fold adds the R
+// row chunks in order
+__global__ __launch_bounds__(256) void k_colsum(int M, int N, int ld, int rc, const bf16_t *dz, float *part) {
   __shared__ float red[4][64];
   const int c = threadIdx.x & 63, rg = threadIdx.x >> 6, n = blockIdx.x * 64 + c;
+  const int m0 = blockIdx.y * rc, m1 = min(M, m0 + rc);
   float s = 0.f;
   if (n < N)
-    for (int m = rg; m < M; m += 4) s += (float)dz[(long)m * ld + n];
+    for (int m = m0 + rg; m < m1; m += 4) s += (float)dz[(long)m * ld + n];
   red[rg][c] = s;
   __syncthreads();
-  if (rg == 0 && n < N) db[n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+  if (rg == 0 && n < N) part[(long)blockIdx.y * N + n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+}
+__global__ __launch_bounds__(256) void k_colsum_fold(int R, int N, const float *part, float *db) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int r = 0; r < R; ++r) s += part[(long)r * N + n];
+  db[n] = s;
 }
 
 // only the R x C logical entries are written: the padding of out keeps its zeros
@@ -237,9 +247,15 @@ int mlp_loss_grad(int B, int n_out, int ld, const float *y, const float *t, cons
   return g;
 }
 
-void mlp_colsum(int M, int N, int ld, const bf16_t *dz, float *db, hipStream_t s) {
+int mlp_colsum_chunks(int M) { return std::max(1, std::min(64, (M + 63) / 64)); }
+
+void mlp_colsum(int M, int N, int ld, const bf16_t *dz, float *db, float *part, hipStream_t s) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(256), 0, s, M, N, ld, dz, db);
+  const int R = mlp_colsum_chunks(M);
+  const int rc = (M + R - 1) / R;
+  hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, R), dim3(256), 0, s, M, N, ld, rc, dz, part);
+  ED_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_colsum_fold, dim3((N + 255) / 256), dim3(256), 0, s, R, N, (const float *)part, db);
   ED_HIP(hipGetLastError());
 }
 

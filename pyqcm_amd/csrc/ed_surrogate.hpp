@@ -50,8 +50,10 @@ void mlp_to_bf16(size_t n, const float *in, bf16_t *out, hipStream_t s);
 // dy = bf16(2 w_o r / B) (zero in the padding), per-block partials of sum w_o r^2
 int mlp_loss_grad(int B, int n_out, int ld, const float *y, const float *t, const float *w, bf16_t *dy,
                   float *partials, hipStream_t s);
-// db[n] = sum_m dz[m * ld + n], n < N (fixed order)
-void mlp_colsum(int M, int N, int ld, const bf16_t *dz, float *db, hipStream_t s);
+// db[n] = sum_m dz[m * ld + n], n < N (fixed order, through part: at least
+// mlp_colsum_chunks(M) x N floats of scratch)
+int mlp_colsum_chunks(int M);
+void mlp_colsum(int M, int N, int ld, const bf16_t *dz, float *db, float *part, hipStream_t s);
 // out[r * ld + c] = sum_z part[z * stride + r * ld + c] for r < R, c < C (fixed
 // order); the padding columns of out are left as they are
 void mlp_split_reduce(int S, int R, int C, int ld, long stride, const float *part, float *out, hipStream_t s);

@@ -509,7 +509,11 @@ def load_corpus(paths):
 
 def main(argv=None):
     """corpus: python -m pyqcm_amd.vca_surrogate corpus N START OUT.npz
-    train:  python -m pyqcm_amd.vca_surrogate train OUT_WEIGHTS.npz CHUNK.npz ..."""
+    train:  python -m pyqcm_amd.vca_surrogate train OUT_WEIGHTS.npz CHUNK.npz ...
+    time:   python -m pyqcm_amd.vca_surrogate time WEIGHTS.npz [BATCH]
+            (forward and forward + backward of the trained network on BATCH
+            resident rows, and 20 training steps on random rows, for kernel
+            statistics)"""
     import sys
     import time
     a = list(sys.argv[1:] if argv is None else argv)
@@ -529,6 +533,17 @@ def main(argv=None):
                               "train_s": wall})
         print("trained on %d samples: loss %.4g -> %.4g, %.3f ms per step, %.1f s" %
               (X.shape[0], hist[0], hist[-1], ms, wall), flush=True)
+    elif a[0] == "time":
+        net, _ = load(a[1])
+        B = int(a[2]) if len(a) > 2 else 4096
+        ms_f, ms_fb = net.time(B, nrep=20)
+        ff, ffb = net.flops(B)
+        rng = np.random.default_rng(0)
+        X = rng.normal(size=(B * 20, net.n_in))
+        Y = rng.normal(size=(B * 20, net.n_out))
+        _, ms_step = net.train(X, Y, 1, B, seed=1)
+        print("batch %d: forward %.4f ms (%.1f TFLOP/s), forward + backward %.4f ms (%.1f TFLOP/s), "
+              "training step %.4f ms" % (B, ms_f, ff / ms_f / 1e9, ms_fb, ffb / ms_fb / 1e9, ms_step), flush=True)
     else:
         raise SystemExit(main.__doc__)
 
