@@ -58,6 +58,7 @@ Params::Params() {
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
   i["lanczos_block"] = 6;           // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
+  i["band_lanczos_block"] = 0;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
   b["lanczos_graph"] = true;        // replay the block as a captured hipGraph
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings

@@ -118,37 +118,38 @@ double band_lowest_eigenvalue(int n, int b, const double *A, int lda, double hin
   return 0.5 * (lo + hi);
 }
 
-// two-sided Givens rotation on rows/cols (p, q) of the symmetric matrix M
-// (column-major full storage), restricted to indices in [j0, j1)
-static inline void rot2(int n, double *M, int p, int q, double c, double s, int j0, int j1) {
-  for (int j = j0; j < j1; ++j) {
-    if (j == p || j == q) continue;
-    double &ap = M[p + (size_t)j * n];
-    double &aq = M[q + (size_t)j * n];
-    const double x = ap, y = aq;
-    ap = c * x + s * y;
-    aq = -s * x + c * y;
-    M[j + (size_t)p * n] = ap;
-    M[j + (size_t)q * n] = aq;
-  }
-  const double app = M[p + (size_t)p * n], aqq = M[q + (size_t)q * n], apq = M[p + (size_t)q * n];
-  const double npp = c * c * app + 2 * c * s * apq + s * s * aqq;
-  const double nqq = s * s * app - 2 * c * s * apq + c * c * aqq;
-  const double npq = (c * c - s * s) * apq + c * s * (aqq - app);
-  M[p + (size_t)p * n] = npp;
-  M[q + (size_t)q * n] = nqq;
-  M[p + (size_t)q * n] = npq;
-  M[q + (size_t)p * n] = npq;
-}
-
 void band_eig_first_rows(int n, int b, const double *Ain, int p, std::vector<double> &w, std::vector<double> &R) {
-  std::vector<double> M(Ain, Ain + (size_t)n * n);
+  // The reduction keeps every nonzero within distance b + 1 of the diagonal
+  // (the bulge), so the symmetric matrix is held as its lower band of width
+  // W = b + 2: Lb[i * W + d] = M[i][i - d].  The whole working set then stays
+  // in cache (a dense n x n copy made every rotation a strided row walk).
+  const int W = b + 2;
+  std::vector<double> Lb((size_t)n * W, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d <= std::min(b, i); ++d) Lb[(size_t)i * W + d] = Ain[i + (size_t)(i - d) * n];
+  auto at = [&](int r, int c) -> double & {
+    return r >= c ? Lb[(size_t)r * W + (r - c)] : Lb[(size_t)c * W + (c - r)];
+  };
   // R: p x n (row-major) = first p rows of the accumulated transform
   R.assign((size_t)p * n, 0.0);
   for (int i = 0; i < std::min(p, n); ++i) R[(size_t)i * n + i] = 1.0;
-  auto apply_rot = [&](int pp, int qq, double c, double s) {
-    const int j0 = std::max(0, pp - b - 1), j1 = std::min(n, qq + b + 2);
-    rot2(n, M.data(), pp, qq, c, s, j0, j1);
+  // two-sided rotation of rows / columns (pp, pp + 1)
+  auto apply_rot = [&](int pp, double c, double s) {
+    const int qq = pp + 1;
+    const int j0 = std::max(0, qq - W + 1), j1 = std::min(n, pp + W);
+    for (int j = j0; j < j1; ++j) {
+      if (j == pp || j == qq) continue;
+      double &ap = at(pp, j);
+      double &aq = at(qq, j);
+      const double x = ap, y = aq;
+      ap = c * x + s * y;
+      aq = -s * x + c * y;
+    }
+    double &app = at(pp, pp), &aqq = at(qq, qq), &apq = at(qq, pp);
+    const double xpp = app, xqq = aqq, xpq = apq;
+    app = c * c * xpp + 2 * c * s * xpq + s * s * xqq;
+    aqq = s * s * xpp - 2 * c * s * xpq + c * c * xqq;
+    apq = (c * c - s * s) * xpq + c * s * (xqq - xpp);
     for (int k = 0; k < p; ++k) {
       double &rp = R[(size_t)k * n + pp];
       double &rq = R[(size_t)k * n + qq];
@@ -162,31 +163,31 @@ void band_eig_first_rows(int n, int b, const double *Ain, int p, std::vector<dou
       for (int l = std::min(b, n - 1 - k); l >= 2; --l) {
         int i = k + l;
         // annihilate M[i][k] with rows (i-1, i)
-        double x = M[(i - 1) + (size_t)k * n], y = M[i + (size_t)k * n];
+        double x = at(i - 1, k), y = at(i, k);
         if (y == 0.0) continue;
         double r = std::hypot(x, y), c = x / r, s = y / r;
-        apply_rot(i - 1, i, c, s);
-        M[i + (size_t)k * n] = M[k + (size_t)i * n] = 0.0;
+        apply_rot(i - 1, c, s);
+        at(i, k) = 0.0;
         // chase the bulge created at (i + b, i - 1)
         int j = i;
         while (j + b < n) {
           const int row = j + b, col = j - 1;
-          x = M[(row - 1) + (size_t)col * n];
-          y = M[row + (size_t)col * n];
+          x = at(row - 1, col);
+          y = at(row, col);
           if (y == 0.0) break;
           r = std::hypot(x, y);
           c = x / r;
           s = y / r;
-          apply_rot(row - 1, row, c, s);
-          M[row + (size_t)col * n] = M[col + (size_t)row * n] = 0.0;
+          apply_rot(row - 1, c, s);
+          at(row, col) = 0.0;
           j = row;
         }
       }
     }
   }
   std::vector<double> d(n), e(n, 0.0);
-  for (int i = 0; i < n; ++i) d[i] = M[i + (size_t)i * n];
-  for (int i = 0; i + 1 < n; ++i) e[i] = M[(i + 1) + (size_t)i * n];
+  for (int i = 0; i < n; ++i) d[i] = at(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = at(i + 1, i);
   ql_rows(n, d.data(), e.data(), R.data(), p);
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);

@@ -756,9 +756,23 @@ __global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, 
 // vectors of its window, so that step needs no separate multi-dot pass.
 template <int BS, int NA, int NB>
 __global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs, MPtrPack a, CoefPack ca, double *t,
-                                                  double st, PtrPack w, CoefPack cb, double cx, double *part) {
+                                                  double st, PtrPack w, CoefPack cb, double cx, double *part,
+                                                  const double *dcoef, const int *stop) {
   constexpr int ND = (NA > 0 ? 1 + NA : 1) + NB + 1;
   __shared__ double red[ND][BS / kWave];
+  // device-driven steps: the scalars come from k_band_coef (xs, st, cx, ca[4],
+  // cb[4]); a set stop flag (a deflation found on the device) makes the step
+  // a no-op
+  if (stop && *stop) return;
+  if (dcoef) {
+    xs = dcoef[0];
+    st = dcoef[1];
+    cx = dcoef[2];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) ca.c[k] = dcoef[3 + k];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) cb.c[b] = dcoef[3 + kBandStepMax + b];
+  }
   double acc[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) acc[k] = 0.0;
@@ -803,6 +817,63 @@ __global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs
     for (int q = 0; q < BS / kWave; ++q) sm += red[threadIdx.x][q];
     part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sm;
   }
+}
+
+// The scalars of one device-driven band-Lanczos step j (no deflated vectors;
+// the window has na vectors ahead and nw behind, k0 = the first behind):
+// folds the dots of v_j (rows of dnb partials: <v_j, v_j>, then the na ahead,
+// then the nw behind from row rb) and the product's partials of <H v_j, v_j>,
+// and writes the coefficients of k_band_step (xs, st, cx, ca[], cb[]) and the
+// host's record (z, zz[0..3], t(j, j), deflated).  |v_j| < accur_deflation
+// sets *stop and leaves the rest to the host.
+__global__ __launch_bounds__(256) void k_band_coef(const double *dpart, int dnb, int na, int rb, int nw,
+                                                   const double *hpart, int hnb, int j, int pc, int k0,
+                                                   double accur_deflation, double *coef, double *rec, int *stop) {
+  __shared__ double red[4];
+  __shared__ double dv[1 + 2 * kBandStepMax + 1];
+  if (*stop) return;
+  auto fold = [&](const double *row, int nb) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += row[i];
+    return block_sum<256>(s, red);
+  };
+  double r = fold(dpart, dnb);
+  if (threadIdx.x == 0) dv[0] = r;
+  for (int k = 0; k < na; ++k) {
+    r = fold(dpart + (size_t)(1 + k) * dnb, dnb);
+    if (threadIdx.x == 0) dv[1 + k] = r;
+  }
+  for (int b = 0; b < nw; ++b) {
+    r = fold(dpart + (size_t)(rb + b) * dnb, dnb);
+    if (threadIdx.x == 0) dv[1 + kBandStepMax + b] = r;
+  }
+  const double hd_raw = fold(hpart, hnb);
+  if (threadIdx.x != 0) return;
+  const double z = sqrt(fmax(dv[0], 0.0));
+  if (z < accur_deflation) {
+    *stop = 1;
+    rec[6] = 1.0;
+    return;
+  }
+  const double iz = 1.0 / z;
+  double zz[kBandStepMax];
+  for (int k = 0; k < na; ++k) zz[k] = dv[1 + k] * iz;
+  double wdot = hd_raw * iz * iz;
+  double tb[kBandStepMax];
+  for (int b = 0; b < nw; ++b) {
+    const int k = k0 + b;
+    tb[b] = (k == j - pc) ? z : zz[k + pc - j - 1];
+    wdot -= tb[b] * dv[1 + kBandStepMax + b] * iz;
+  }
+  coef[0] = iz;
+  coef[1] = iz;
+  coef[2] = -wdot;
+  for (int k = 0; k < kBandStepMax; ++k) coef[3 + k] = k < na ? -zz[k] : 0.0;
+  for (int b = 0; b < kBandStepMax; ++b) coef[3 + kBandStepMax + b] = b < nw ? -tb[b] : 0.0;
+  rec[0] = z;
+  for (int k = 0; k < kBandStepMax; ++k) rec[1 + k] = k < na ? zz[k] : 0.0;
+  rec[5] = wdot;
+  rec[6] = 0.0;
 }
 
 template <int BS>
@@ -1405,20 +1476,29 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
   }
 }
 
+void band_coef(const double *dpart, int dnb, int na, int rb, int nw, const double *hpart, int hnb, int j, int pc,
+               int k0, double accur_deflation, double *coef, double *rec, int *stop, hipStream_t s) {
+  if (na > kBandStepMax || nw > kBandStepMax) fail(ED_ERR_INTERNAL, "band_coef: window larger than the kernel's");
+  hipLaunchKernelGGL(k_band_coef, dim3(1), dim3(256), 0, s, dpart, dnb, na, rb, nw, hpart, hnb, j, pc, k0,
+                     accur_deflation, coef, rec, stop);
+  ED_HIP(hipGetLastError());
+}
+
 int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
-              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s) {
+              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s,
+              const double *dcoef, const int *stop) {
   constexpr int BS = 1024;
   if (na < 0 || na > kBandStepMax || nw < 0 || nw > kBandStepMax || vector_block() != BS || n == 0) return 0;
   MPtrPack ap{};
   CoefPack cap{}, cbp{};
   PtrPack wp{};
-  for (int k = 0; k < na; ++k) { ap.p[k] = a[k]; cap.c[k] = ca[k]; }
-  for (int k = 0; k < nw; ++k) { wp.p[k] = w[k]; cbp.c[k] = cb[k]; }
+  for (int k = 0; k < na; ++k) { ap.p[k] = a[k]; cap.c[k] = ca ? ca[k] : 0.0; }
+  for (int k = 0; k < nw; ++k) { wp.p[k] = w[k]; cbp.c[k] = cb ? cb[k] : 0.0; }
   const int g = grid_for(n, BS, reduction_blocks_per_cu());
 #define ED_BS(NA, NB)                                                                                         \
   if (na == NA && nw == NB) {                                                                                \
     hipLaunchKernelGGL((k_band_step<BS, NA, NB>), dim3(g), dim3(BS), 0, s, n, x, xs, ap, cap, t, st, wp, cbp, \
-                       cx, part);                                                                            \
+                       cx, part, dcoef, stop);                                                               \
     ED_HIP(hipGetLastError());                                                                               \
     return g;                                                                                                \
   }

@@ -134,9 +134,20 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
 // (row r at part + r * nb): <c, c>, <c, a_k> (1 <= k < na), <c, t> (na > 0),
 // <c, w_b> (b < nw), <c, x>.  na, nw <= kBandStepMax; returns nb (0 when the
 // counts or the vector block size are outside what the kernel is built for)
+// With dcoef (device-driven steps) the scalars are read from there (layout of
+// band_coef) and a set *stop makes the launch a no-op.
 constexpr int kBandStepMax = 4;
 int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
-              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s);
+              int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s,
+              const double *dcoef = nullptr, const int *stop = nullptr);
+// The scalars of a device-driven band-Lanczos step j from device partials: the
+// dots of v_j (rows of dnb: <v_j, v_j>, na ahead, nw behind from row rb) and
+// <H v_j, v_j> (hnb partials).  coef (3 + 2 kBandStepMax doubles): xs = st =
+// 1/|v_j|, cx = -t(j, j), ca = -zz (the ahead coefficients), cb = -t(k, j)
+// (behind, k = k0 ..); rec (7 doubles): |v_j|, zz[0..3], t(j, j), deflated.
+// |v_j| < accur_deflation sets *stop (and rec[6]); a set *stop skips the step.
+void band_coef(const double *dpart, int dnb, int na, int rb, int nw, const double *hpart, int hnb, int j, int pc,
+               int k0, double accur_deflation, double *coef, double *rec, int *stop, hipStream_t s);
 // t = st * t + sum_k c_k b_k ; optional partials of ||t||^2 and <t, w>. returns nb
 int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *norm_partials,
             const double *w, double *dot_partials, hipStream_t s);

@@ -1271,6 +1271,15 @@ std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &s
 // implicit QL on a symmetric tridiagonal matrix; rotations accumulated into the
 // rows of Z given by zrows (n_z rows of length n, row-major) -- the full matrix
 // (zrows = n, identity start) or only the last row (for the Ritz residual).
+// sqrt(x^2 + y^2) without std::hypot's scaling when the squares cannot
+// overflow or underflow (the common case here; std::hypot dominated the QL)
+static inline double fast_hypot(double x, double y) {
+  const double ax = std::fabs(x), ay = std::fabs(y);
+  const double m = ax > ay ? ax : ay;
+  if (m < 1e150 && m > 1e-150) return std::sqrt(x * x + y * y);
+  return std::hypot(x, y);
+}
+
 void ql_rows(int n, double *d, double *e, double *Z, int nz) {
   if (n <= 0) return;
   e[n - 1] = 0.0;
@@ -1284,13 +1293,13 @@ void ql_rows(int n, double *d, double *e, double *Z, int nz) {
       if (m != l) {
         if (iter++ == 60) fail(ED_ERR_NOT_CONVERGED, "tridiagonal eigensolver did not converge");
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = std::hypot(g, 1.0);
+        double r = fast_hypot(g, 1.0);
         g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
         double s = 1.0, c = 1.0, p = 0.0;
         int i;
         for (i = m - 1; i >= l; --i) {
           double f = s * e[i], b = c * e[i];
-          r = std::hypot(f, g);
+          r = fast_hypot(f, g);
           e[i + 1] = r;
           if (r == 0.0) { d[i + 1] -= p; e[m] = 0.0; break; }
           s = f / r;

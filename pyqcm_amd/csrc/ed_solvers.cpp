@@ -584,12 +584,143 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
     pre_h = false;
     pre = nullptr;
   };
+  // step (12): convergence of the lowest eigenvalue of the projected matrix,
+  // checked after step j (every p steps)
+  auto converged_at = [&](int j) -> bool {
+    if (!(j >= 4 * p && j % p == 0)) return false;
+    double e0, e1 = 0.0;
+    if (nd == 0 && !no_deg) {
+      // the leading j x j block of t in place; the previous check's value
+      // bounds this one from above (interlacing) and the last change sizes
+      // the bracket below it
+      e0 = band_lowest_eigenvalue(j, p, t.data(), M0, ev_old < 1e11 ? ev_old : HUGE_VAL, 2.0 * ev_change);
+    } else if (nd == 0) {
+      std::vector<double> Hp((size_t)j * j);
+      for (int c = 0; c < j; ++c)
+        for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
+      e0 = band_kth_eigenvalue(j, p, Hp.data(), 0);
+      e1 = band_kth_eigenvalue(j, p, Hp.data(), 1);
+    } else {
+      std::vector<double> Hp((size_t)j * j);
+      for (int c = 0; c < j; ++c)
+        for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
+      std::vector<double> ev(j);
+      symeig(j, Hp.data(), ev.data(), nullptr);
+      e0 = ev[0];
+      e1 = j > 1 ? ev[1] : ev[0];
+    }
+    const double test = std::fabs(e0 - ev_old);
+    if (ev_old < 1e11) ev_change = test;
+    ev_old = e0;
+    if (no_deg && e1 - e0 < min_gap) {
+      drain();
+      fail(ED_ERR_DEGENERATE, "band Lanczos: the gap between the first two eigenvalues is smaller than " +
+                                  std::to_string(min_gap));
+    }
+    return test < accur_bl;
+  };
+  // Device-driven steps (no deflated vectors, windows of at most
+  // kBandStepMax vectors, no P-matrix column): each step's scalars are
+  // computed on the device (band_coef) from the dots of the previous fused
+  // update and the product's partials, so `dev_block` steps are queued without
+  // a host wait; the host then fills t from the steps' records and applies the
+  // convergence check in step order.  A deflation found on the device stops
+  // the block (the remaining launches do nothing) and the host redoes that
+  // step; a convergence ends the run at the same step as the host-driven loop.
+  const int dev_block = (int)Pm.geti("band_lanczos_block");
+  const bool dev_on = dev_block >= 2 && Pm.geti("vector_block") == 1024;
+  auto dev_ok = [&](int jj) {
+    return dev_on && nd == 0 && jj >= p && pc - 1 <= kBandStepMax && pc <= kBandStepMax && jj + 1 < M0;
+  };
+  DevBuf<double> dring, hring, cring, rring;
+  DevBuf<int> dstop;
+  const size_t drow = (size_t)(2 * kBandStepMax + 2) * (size_t)partials_cap();
+  const size_t hcap = (size_t)partials_cap();
+  bool dev_converged = false;
+  // runs device blocks from step j0 (dots of v_j0 in the pinned slot `pre`);
+  // returns the step the host loop continues with (its dots not available)
+  auto run_device = [&](int j0) -> int {
+    if (!dring.p) {
+      dring.alloc((size_t)(dev_block + 1) * drow);
+      hring.alloc((size_t)dev_block * hcap);
+      cring.alloc((size_t)dev_block * 16);
+      rring.alloc((size_t)dev_block * 8);
+      dstop.alloc(1);
+    }
+    const double *din = pre;
+    int din_nb = pre_nb, din_rb = 1 + pre_na + pre_skip;
+    double *pinned = pre;
+    const int pinned_nb = pre_nb, pinned_m = pre_m;
+    pre = nullptr;
+    int jcur = j0;
+    while (true) {
+      const int K = std::min(dev_block, M0 - jcur);
+      ED_HIP(hipMemsetAsync(dstop.p, 0, sizeof(int), stream()));
+      int dslot = 0;
+      for (int s2 = 0; s2 < K; ++s2) {
+        const int js = jcur + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
+        const int na = jm2 - js - 1, nw = js - k02;
+        v[js + pc].reset(new DevBuf<double>(dim));
+        const int hnb2 = H.apply_fixed(v[js]->p, v[js + pc]->p, 1.0, 0.0, hring.p + s2 * hcap, nullptr, 0);
+        band_coef(din, din_nb, na, din_rb, nw, hring.p + s2 * hcap, hnb2, js, pc, k02, accur_deflation,
+                  cring.p + s2 * 16, rring.p + s2 * 8, dstop.p, stream());
+        std::vector<double *> ys;
+        std::vector<const double *> ws;
+        for (int k = js + 1; k < jm2; ++k) ys.push_back(v[k]->p);
+        for (int k = k02; k < js; ++k) ws.push_back(v[k]->p);
+        // the block's dots go to ring slots 0 .. K-1, avoiding the slot the
+        // block's first step reads (the previous block's last one)
+        dslot = (din == dring.p + (size_t)s2 * drow) ? dev_block : s2;
+        double *dout = dring.p + (size_t)dslot * drow;
+        const int nb = band_step(dim, v[js]->p, 0.0, na, ys.data(), nullptr, v[js + pc]->p, 0.0, nw, ws.data(),
+                                 nullptr, 0.0, dout, stream(), cring.p + s2 * 16, dstop.p);
+        if (nb <= 0) {
+          sync();
+          fail(ED_ERR_INTERNAL, "band Lanczos: device step outside the fused kernel's range");
+        }
+        const int k0n = (js + 1 > pc) ? js + 1 - pc : 0;
+        din = dout;
+        din_nb = nb;
+        din_rb = 1 + na + (k0n - k02);
+      }
+      std::vector<double> rec((size_t)K * 8);
+      ED_HIP(hipMemcpyAsync(rec.data(), rring.p, rec.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+      sync();
+      if (pinned) {
+        sum_partials(pinned, pinned_nb, pinned_m, nullptr);  // release the slot the first block read
+        pinned = nullptr;
+      }
+      for (int s2 = 0; s2 < K; ++s2) {
+        const int js = jcur + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
+        const double *r = rec.data() + (size_t)s2 * 8;
+        if (r[6] != 0.0) return js;  // deflation: the host redoes step js
+        if (js >= pc) T(js, js - pc) = r[0];
+        for (int k = js + 1; k < jm2; ++k)
+          if (k >= pc) T(js, k - pc) = r[1 + (k - js - 1)];
+        for (int k = k02; k < js; ++k) T(k, js) = T(js, k);
+        T(js, js) = r[5];
+        if (js > pc && def[k02 - 1] == 0) v[k02 - 1].reset();
+        if (converged_at(js)) {
+          dev_converged = true;
+          return js;
+        }
+      }
+      jcur += K;
+      if (!dev_ok(jcur)) return jcur;
+      check_cancel();
+    }
+  };
   for (j = 0; j < M0; ++j) {
     try {
       check_cancel();
     } catch (...) {
       drain();
       throw;
+    }
+    if (pre && !pre_h && dev_ok(j)) {
+      j = run_device(j);
+      if (dev_converged) break;
+      if (j >= M0) break;
     }
     // one multi-dot per step: ||v_j||^2, <v_j, phi_k> (P matrix), the step (6)
     // overlaps <v_j, v_k> (k = j+1 .. j+pc-1) and <v_j, v_k> for the vectors
@@ -703,8 +834,8 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
           pre_skip = ((j + 1 > pc) ? j + 1 - pc : 0) - k0;
           // queue the next step's product now, so that the host's
           // convergence check below overlaps it (wasted if this step ends
-          // the run; drained then)
-          if (j + 1 < M0) {
+          // the run; drained then), unless the next steps run device-driven
+          if (j + 1 < M0 && !dev_ok(j + 1)) {
             v[j + 1 + pc].reset(new DevBuf<double>(dim));
             pre_hnb = H.apply(v[j + 1]->p, v[j + 1 + pc]->p, 1.0, 0.0, true);
             pre_h = true;
@@ -751,39 +882,7 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
     T(j, j) = wdot;
     for (int k = 0; k < nd; ++k) T(j, Idef[k]) = T(Idef[k], j);
   converge:
-    // step (12): convergence of the lowest eigenvalue of the projected matrix
-    if (j >= 4 * p && j % p == 0) {
-      double e0, e1 = 0.0;
-      if (nd == 0 && !no_deg) {
-        // the leading j x j block of t in place; the previous check's value
-        // bounds this one from above (interlacing) and the last change sizes
-        // the bracket below it
-        e0 = band_lowest_eigenvalue(j, p, t.data(), M0, ev_old < 1e11 ? ev_old : HUGE_VAL, 2.0 * ev_change);
-      } else if (nd == 0) {
-        std::vector<double> Hp((size_t)j * j);
-        for (int c = 0; c < j; ++c)
-          for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
-        e0 = band_kth_eigenvalue(j, p, Hp.data(), 0);
-        e1 = band_kth_eigenvalue(j, p, Hp.data(), 1);
-      } else {
-        std::vector<double> Hp((size_t)j * j);
-        for (int c = 0; c < j; ++c)
-          for (int r = 0; r < j; ++r) Hp[(size_t)r + (size_t)c * j] = T(r, c);
-        std::vector<double> ev(j);
-        symeig(j, Hp.data(), ev.data(), nullptr);
-        e0 = ev[0];
-        e1 = j > 1 ? ev[1] : ev[0];
-      }
-      const double test = std::fabs(e0 - ev_old);
-      if (ev_old < 1e11) ev_change = test;
-      ev_old = e0;
-      if (no_deg && e1 - e0 < min_gap) {
-        drain();
-        fail(ED_ERR_DEGENERATE, "band Lanczos: the gap between the first two eigenvalues is smaller than " +
-                                    std::to_string(min_gap));
-      }
-      if (test < accur_bl) break;
-    }
+    if (converged_at(j)) break;
   }
   drain();  // the product and dots of a step not taken
   const int M = j;

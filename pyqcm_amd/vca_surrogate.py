@@ -510,6 +510,9 @@ def load_corpus(paths):
 def main(argv=None):
     """corpus: python -m pyqcm_amd.vca_surrogate corpus N START OUT.npz
     train:  python -m pyqcm_amd.vca_surrogate train OUT_WEIGHTS.npz CHUNK.npz ...
+            [--epochs E1,E2,E3] [--wE0 W] [--holdout N]
+            (epochs at learning rates 1e-3, 3e-4, 1e-4; the E0 loss weight;
+            the last N rows kept out of training and reported)
     time:   python -m pyqcm_amd.vca_surrogate time WEIGHTS.npz [BATCH]
             (forward and forward + backward of the trained network on BATCH
             resident rows, and 20 training steps on random rows, for kernel
@@ -526,13 +529,34 @@ def main(argv=None):
         _save_chunk(out, X, Y, lens)
         print("corpus rows %d..%d: %.1f s, %.2f ms per solve" % (start, start + n, dt, 1e3 * dt / n), flush=True)
     elif a[0] == "train":
-        out, chunks = a[1], a[2:]
+        opts = {"--epochs": "200,100,100", "--wE0": "64", "--holdout": "0"}
+        rest = []
+        i = 1
+        while i < len(a):
+            if a[i] in opts:
+                opts[a[i]] = a[i + 1]
+                i += 2
+            else:
+                rest.append(a[i])
+                i += 1
+        out, chunks = rest[0], rest[1:]
         X, Y = load_corpus(chunks)
-        net, hist, ms, wall = train(X, Y)
-        save(out, net, X, Y, {"samples": int(X.shape[0]), "final_loss": float(hist[-1]), "ms_per_step": ms,
-                              "train_s": wall})
-        print("trained on %d samples: loss %.4g -> %.4g, %.3f ms per step, %.1f s" %
-              (X.shape[0], hist[0], hist[-1], ms, wall), flush=True)
+        nh = int(opts["--holdout"])
+        Xh, Yh = X[len(X) - nh:], Y[len(Y) - nh:]
+        X, Y = X[:len(X) - nh], Y[:len(Y) - nh]
+        epochs = tuple(int(v) for v in opts["--epochs"].split(","))
+        w_E0 = float(opts["--wE0"])
+        net, hist, ms, wall = train(X, Y, epochs=epochs, w_E0=w_E0)
+        meta = {"samples": int(X.shape[0]), "final_loss": float(hist[-1]), "ms_per_step": ms, "train_s": wall,
+                "epochs": list(epochs), "w_E0": w_E0}
+        if nh:
+            err = np.abs(net.predict(Xh) - Yh)
+            meta["holdout"] = nh
+            meta["holdout_E0_mae"] = float(err[:, 0].mean())
+            meta["holdout_cf_mae_level0"] = float(err[:, 1:].reshape(nh, -1, 2, NCF)[:, :, :, 0].mean())
+        save(out, net, X, Y, meta)
+        print("trained on %d samples: loss %.4g -> %.4g, %.3f ms per step, %.1f s; %s" %
+              (X.shape[0], hist[0], hist[-1], ms, wall, meta), flush=True)
     elif a[0] == "time":
         net, _ = load(a[1])
         B = int(a[2]) if len(a) > 2 else 4096

@@ -23,6 +23,7 @@ def _device():
     qcm.set_global_parameterC("Hamiltonian_format", "G")
     qcm.set_global_parameterC("GF_method", "L")
     yield
+    qcm.set_global_parameterC("band_lanczos_block", 8)
     qcm.set_global_parameterC("lanczos_block", 6)
     qcm.set_global_parameterC("lanczos_graph", True)
     qcm.set_global_parameterC("store_lanczos_basis", True)
@@ -125,3 +126,39 @@ def test_invalid_block_is_an_error():
             H.lanczos_gs(x)
     finally:
         _mode(6, True)
+
+
+def _lehmann(e, Q, z):
+    return np.einsum("ic,jc,zc->zij", Q, Q, 1.0 / (z[:, None] - e[None, :]))
+
+
+@pytest.mark.parametrize("case", ["plaquette", "exhausted"])
+def test_band_lanczos_device_steps_same_as_host(case):
+    """band Lanczos with its steps' scalars computed on the device (blocks of
+    band_lanczos_block steps) against the host-driven loop (a host wait per
+    step): the same number of steps, eigenvalues and Lehmann weights.  The
+    plaquette + 4 bath sector (3920 states, p = 4) runs to the convergence
+    test; the 6-site chain's 300-state sector exhausts its Krylov space, so a
+    deflation is found on the device and the host redoes that step."""
+    if case == "plaquette":
+        wl, sec = W.plaquette_bath(n_bath=4), "R0:N9:S1"
+    else:
+        wl, sec = W.hubbard_chain(6, U=4.0, mu=2.0), "R0:N7:S1"
+    H = _ham(wl, sec)
+    phi = np.random.default_rng(21).normal(size=(4, H.dim))
+    z = W.matsubara(32, 20.0)
+    try:
+        qcm.set_global_parameterC("band_lanczos_block", 0)
+        eh, Qh = H.band_lanczos(phi)
+        Gh = _lehmann(eh, Qh, z)
+        for blk in (2, 8, 32):
+            qcm.set_global_parameterC("band_lanczos_block", blk)
+            e, Q = H.band_lanczos(phi)
+            print(case, "block", blk, "steps", len(e), "host", len(eh), flush=True)
+            # the convergence test (every p steps) decides at the same step
+            # unless rounding moves its value across the threshold
+            assert abs(len(e) - len(eh)) <= 4, (blk, len(e), len(eh))
+            assert np.abs(e[:4] - eh[:4]).max() < 1e-9 * max(1.0, np.abs(eh).max()), blk
+            assert np.abs(_lehmann(e, Q, z) - Gh).max() < 1e-8, blk
+    finally:
+        qcm.set_global_parameterC("band_lanczos_block", 8)
