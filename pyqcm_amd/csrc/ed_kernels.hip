@@ -826,29 +826,40 @@ __global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs
 // and writes the coefficients of k_band_step (xs, st, cx, ca[], cb[]) and the
 // host's record (z, zz[0..3], t(j, j), deflated).  |v_j| < accur_deflation
 // sets *stop and leaves the rest to the host.
-__global__ __launch_bounds__(256) void k_band_coef(const double *dpart, int dnb, int na, int rb, int nw,
-                                                   const double *hpart, int hnb, int j, int pc, int k0,
-                                                   double accur_deflation, double *coef, double *rec, int *stop) {
-  __shared__ double red[4];
-  __shared__ double dv[1 + 2 * kBandStepMax + 1];
+__global__ __launch_bounds__(1024) void k_band_coef(const double *dpart, int dnb, int na, int rb, int nw,
+                                                    const double *hpart, int hnb, int j, int pc, int k0,
+                                                    double accur_deflation, double *coef, double *rec, int *stop) {
+  // one wave per folded row: wave 0 <v_j, v_j>, waves 1 .. na the ahead dots,
+  // then the nw behind dots, then <H v_j, v_j>; each in a fixed order
+  __shared__ double dv[2 + 2 * kBandStepMax + 1];
   if (*stop) return;
-  auto fold = [&](const double *row, int nb) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int nrows = 1 + na + nw + 1;
+  if (w < nrows) {
+    const double *row;
+    int nb = dnb, slot;
+    if (w == 0) {
+      row = dpart;
+      slot = 0;
+    } else if (w <= na) {
+      row = dpart + (size_t)w * dnb;
+      slot = w;
+    } else if (w <= na + nw) {
+      row = dpart + (size_t)(rb + w - 1 - na) * dnb;
+      slot = 1 + kBandStepMax + (w - 1 - na);
+    } else {
+      row = hpart;
+      nb = hnb;
+      slot = 1 + 2 * kBandStepMax;
+    }
     double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) s += row[i];
-    return block_sum<256>(s, red);
-  };
-  double r = fold(dpart, dnb);
-  if (threadIdx.x == 0) dv[0] = r;
-  for (int k = 0; k < na; ++k) {
-    r = fold(dpart + (size_t)(1 + k) * dnb, dnb);
-    if (threadIdx.x == 0) dv[1 + k] = r;
+    for (int i = lane; i < nb; i += kWave) s += row[i];
+    s = wave_sum(s);
+    if (lane == 0) dv[slot] = s;
   }
-  for (int b = 0; b < nw; ++b) {
-    r = fold(dpart + (size_t)(rb + b) * dnb, dnb);
-    if (threadIdx.x == 0) dv[1 + kBandStepMax + b] = r;
-  }
-  const double hd_raw = fold(hpart, hnb);
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  const double hd_raw = dv[1 + 2 * kBandStepMax];
   const double z = sqrt(fmax(dv[0], 0.0));
   if (z < accur_deflation) {
     *stop = 1;
@@ -1479,7 +1490,7 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
 void band_coef(const double *dpart, int dnb, int na, int rb, int nw, const double *hpart, int hnb, int j, int pc,
                int k0, double accur_deflation, double *coef, double *rec, int *stop, hipStream_t s) {
   if (na > kBandStepMax || nw > kBandStepMax) fail(ED_ERR_INTERNAL, "band_coef: window larger than the kernel's");
-  hipLaunchKernelGGL(k_band_coef, dim3(1), dim3(256), 0, s, dpart, dnb, na, rb, nw, hpart, hnb, j, pc, k0,
+  hipLaunchKernelGGL(k_band_coef, dim3(1), dim3(1024), 0, s, dpart, dnb, na, rb, nw, hpart, hnb, j, pc, k0,
                      accur_deflation, coef, rec, stop);
   ED_HIP(hipGetLastError());
 }
