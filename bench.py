@@ -302,7 +302,7 @@ def _large_leg():
                         "1e-10); large_solve_configured.cpu_estimate is measured in this run"}
 
 
-def ed_solve_gpu(nz=1024, beta=50.0, reps=5):
+def ed_solve_gpu(nz=1024, beta=50.0, reps=5, windows=False):
     import numpy as np
     from pyqcm_amd import qcm, workloads as W
     wl = W.plaquette_bath(seed=2025)
@@ -318,9 +318,15 @@ def ed_solve_gpu(nz=1024, beta=50.0, reps=5):
             qcm.new_model_instanceC(name, dict(wl.values), wl.sector, lab)
         t0 = time.perf_counter()
         E0, _ = qcm.ground_state_solve(lab)
+        tg = time.perf_counter()
         qcm.Green_function_solveC(lab)
         G = qcm.Green_function_grid(z, label=lab)
         t1 = time.perf_counter()
+        if windows:
+            # the host clock of these stamps is the one of rocprofv3's kernel
+            # timestamps: a trace can be cut into solves and phases with them
+            print("solve %d window %.6f %.6f %.6f (ground state %.2f ms, Green function %.2f ms)" %
+                  (r, t0, tg, t1, 1e3 * (tg - t0), 1e3 * (t1 - tg)), file=sys.stderr, flush=True)
         assert np.all(np.isfinite(G))
         if r > 0:  # the first solve is the warm-up (allocator, code objects)
             times.append(t1 - t0)
@@ -452,6 +458,8 @@ def main():
     ap.add_argument("--no-lattice", action="store_true", help="skip the CPT / Potthoff-functional leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 4x4 ground-state solve")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--solve-windows", action="store_true",
+                    help="print each ED solve's host-clock window (and its phases) to stderr, for cutting a trace")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
@@ -566,7 +574,7 @@ def _batch_leg(args):
 
 
 def _solve_leg(args):
-    wl2, E0g, tg = ed_solve_gpu()
+    wl2, E0g, tg = ed_solve_gpu(windows=args.solve_windows)
     solve = {"workload": "2x2 plaquette t=1 t'=-0.3 U=8 mu=4 + 8 bath (seed 2025), sector N=12 S=0, "
                          "GS + band-Lanczos G on 1024 Matsubara points (beta=50)",
              "gpu_solves_per_s": round(1.0 / tg, 3), "gpu_s_per_solve": round(tg, 4), "E0_gpu": E0g,

@@ -742,7 +742,10 @@ int ed_hamiltonian_autotune(int64_t handle, int cap, double *ms, int *n, int *be
 }
 
 int ed_hamiltonian_set_variant(int64_t handle, int variant) {
-  return guard([&] { get_h(handle).variant = variant; });
+  return guard([&] {
+    need(variant >= 0 && variant <= 3, "ed_hamiltonian_set_variant: variant 0 .. 3");
+    get_h(handle).variant = variant;
+  });
 }
 
 int ed_hamiltonian_time(int64_t handle, int nrep, double *ms, int variant) {

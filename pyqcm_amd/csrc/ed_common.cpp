@@ -48,7 +48,7 @@ Params::Params() {
   c["Ground_state_method"] = 'L';
   c["GF_method"] = 'L';
   // device-side knobs (no reference counterpart)
-  i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass
+  i["hv_variant"] = 0;            // 0 auto-tuned, 1 fused row kernel, 2 two-pass, 3 two-pass with LDS slabs
   i["sell_sort_window"] = 256;    // two-pass up table: rows sorted by length within windows of this many states
   i["reduction_blocks_per_cu"] = 1;
   i["vector_block"] = 1024;       // threads per workgroup of the Krylov vector kernels (multi-dot, axpy, combine): 256, 512 or 1024
@@ -58,7 +58,7 @@ Params::Params() {
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
   i["lanczos_block"] = 6;           // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
-  i["band_lanczos_block"] = 0;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
+  i["band_lanczos_block"] = 8;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
   b["lanczos_graph"] = true;        // replay the block as a captured hipGraph
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings

@@ -534,6 +534,51 @@ __global__ __launch_bounds__(SW *RL) void k_hv_down_slab(SlabParams P) {
   }
 }
 
+// Down pass of the two-pass schedule for sectors whose column slab fits in
+// LDS (dim_dn x W doubles): workgroup b stages columns [b W, b W + W) of every
+// row of x once, then y[d][u] += ax sum_e K_dn[d, c_e] x[c_e][u] for all rows d
+// from LDS (W lanes per row: consecutive columns, conflict-free), with the
+// partials of <y, x>.  Each x element is read from memory once instead of once
+// per neighbour row (the fused kernel's ~17 re-reads, served by the Infinity
+// Cache on the small sectors where this schedule is chosen).
+template <int W>
+__global__ __launch_bounds__(256) void k_hv_down_lds(SlabParams P) {
+  extern __shared__ double sx[];
+  __shared__ double vtd[kVtabMax];
+  __shared__ double red[256 / kWave];
+  const FactArgs &a = P.a;
+  const uint32_t du = a.up.dim, dd = a.dn.dim;
+  for (int i = threadIdx.x; i < P.ntab_dn; i += 256) vtd[i] = a.dn.val[i];
+  const double *vtab_d = P.ntab_dn ? vtd : a.dn.val;
+  const uint32_t c0 = blockIdx.x * W;
+  const uint32_t n = dd * W;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const uint32_t d = i / W, c = i - d * W;
+    sx[i] = (c0 + c < du) ? P.x[(size_t)d * du + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const uint32_t d = i / W, c = i - d * W;
+    if (c0 + c >= du) continue;
+    double acc = 0.0;
+    const uint32_t e1 = a.dn.ptr[d + 1];
+    for (uint32_t e = a.dn.ptr[d]; e < e1; ++e) {
+      uint32_t col;
+      const double v = half_entry(a.dn, vtab_d, e, col);
+      acc += v * sx[col * W + c];
+    }
+    const size_t I = (size_t)d * du + c0 + c;
+    const double out = P.y[I] + P.ax * acc;
+    P.y[I] = out;
+    dot += out * sx[i];
+  }
+  if (P.part) {
+    const double s = block_sum<256>(dot, red);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = s;
+  }
+}
+
 // partial sums -> kPartialsCap partials (deterministic, fixed order)
 template <int BS>
 __global__ __launch_bounds__(BS) void k_fold_partials(const double *in, size_t n, double *out) {
@@ -757,21 +802,78 @@ __global__ __launch_bounds__(BS) void k_combine(size_t n, double *t, double st, 
 template <int BS, int NA, int NB>
 __global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs, MPtrPack a, CoefPack ca, double *t,
                                                   double st, PtrPack w, CoefPack cb, double cx, double *part,
-                                                  const double *dcoef, const int *stop) {
+                                                  BandFold F) {
   constexpr int ND = (NA > 0 ? 1 + NA : 1) + NB + 1;
   __shared__ double red[ND][BS / kWave];
-  // device-driven steps: the scalars come from k_band_coef (xs, st, cx, ca[4],
-  // cb[4]); a set stop flag (a deflation found on the device) makes the step
-  // a no-op
-  if (stop && *stop) return;
-  if (dcoef) {
-    xs = dcoef[0];
-    st = dcoef[1];
-    cx = dcoef[2];
+  __shared__ double dv[NA + NB + 2];
+  __shared__ double sc[3 + NA + NB + 1];
+  if (F.stop && *F.stop) return;  // a deflation found by an earlier step
+  if (F.dpart) {
+    // device-driven step: every workgroup folds the dots of v_j (rows of
+    // F.dnb partials: <v_j, v_j>, the NA ahead, the NB behind from row F.rb)
+    // and <H v_j, v_j> (F.hnb partials), one wave per row in a fixed order,
+    // so all of them derive the same scalars; workgroup 0 stores the record
+    const int lane = threadIdx.x & (kWave - 1), wq = threadIdx.x / kWave;
+    if (wq < NA + NB + 2) {
+      const double *row;
+      int nb = F.dnb;
+      if (wq == 0) row = F.dpart;
+      else if (wq <= NA) row = F.dpart + (size_t)wq * F.dnb;
+      else if (wq <= NA + NB) row = F.dpart + (size_t)(F.rb + wq - 1 - NA) * F.dnb;
+      else {
+        row = F.hpart;
+        nb = F.hnb;
+      }
+      double sm = 0.0;
+      for (int i = lane; i < nb; i += kWave) sm += row[i];
+      sm = wave_sum(sm);
+      if (lane == 0) dv[wq] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double z = sqrt(fmax(dv[0], 0.0));
+      if (z < F.accur) {
+        sc[3 + NA + NB] = 1.0;
+        if (blockIdx.x == 0) {
+          *F.stop = 1;
+          F.rec[6] = 1.0;
+        }
+      } else {
+        const double iz = 1.0 / z;
+        double zz[NA > 0 ? NA : 1];
 #pragma unroll
-    for (int k = 0; k < NA; ++k) ca.c[k] = dcoef[3 + k];
+        for (int k = 0; k < NA; ++k) zz[k] = dv[1 + k] * iz;
+        double wdot = dv[1 + NA + NB] * iz * iz;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) cb.c[b] = dcoef[3 + kBandStepMax + b];
+        for (int b = 0; b < NB; ++b) {
+          const int k = F.k0 + b;
+          const double tb = (k == F.j - F.pc) ? z : zz[k + F.pc - F.j - 1];
+          wdot -= tb * dv[1 + NA + b] * iz;
+          sc[3 + NA + b] = -tb;
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) sc[3 + k] = -zz[k];
+        sc[0] = iz;
+        sc[1] = iz;
+        sc[2] = -wdot;
+        sc[3 + NA + NB] = 0.0;
+        if (blockIdx.x == 0) {
+          F.rec[0] = z;
+          for (int k = 0; k < kBandStepMax; ++k) F.rec[1 + k] = k < NA ? zz[k] : 0.0;
+          F.rec[5] = wdot;
+          F.rec[6] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (sc[3 + NA + NB] != 0.0) return;
+    xs = sc[0];
+    st = sc[1];
+    cx = sc[2];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) ca.c[k] = sc[3 + k];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) cb.c[b] = sc[3 + NA + b];
   }
   double acc[ND];
 #pragma unroll
@@ -817,74 +919,6 @@ __global__ __launch_bounds__(BS) void k_band_step(size_t n, double *x, double xs
     for (int q = 0; q < BS / kWave; ++q) sm += red[threadIdx.x][q];
     part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sm;
   }
-}
-
-// The scalars of one device-driven band-Lanczos step j (no deflated vectors;
-// the window has na vectors ahead and nw behind, k0 = the first behind):
-// folds the dots of v_j (rows of dnb partials: <v_j, v_j>, then the na ahead,
-// then the nw behind from row rb) and the product's partials of <H v_j, v_j>,
-// and writes the coefficients of k_band_step (xs, st, cx, ca[], cb[]) and the
-// host's record (z, zz[0..3], t(j, j), deflated).  |v_j| < accur_deflation
-// sets *stop and leaves the rest to the host.
-__global__ __launch_bounds__(1024) void k_band_coef(const double *dpart, int dnb, int na, int rb, int nw,
-                                                    const double *hpart, int hnb, int j, int pc, int k0,
-                                                    double accur_deflation, double *coef, double *rec, int *stop) {
-  // one wave per folded row: wave 0 <v_j, v_j>, waves 1 .. na the ahead dots,
-  // then the nw behind dots, then <H v_j, v_j>; each in a fixed order
-  __shared__ double dv[2 + 2 * kBandStepMax + 1];
-  if (*stop) return;
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int nrows = 1 + na + nw + 1;
-  if (w < nrows) {
-    const double *row;
-    int nb = dnb, slot;
-    if (w == 0) {
-      row = dpart;
-      slot = 0;
-    } else if (w <= na) {
-      row = dpart + (size_t)w * dnb;
-      slot = w;
-    } else if (w <= na + nw) {
-      row = dpart + (size_t)(rb + w - 1 - na) * dnb;
-      slot = 1 + kBandStepMax + (w - 1 - na);
-    } else {
-      row = hpart;
-      nb = hnb;
-      slot = 1 + 2 * kBandStepMax;
-    }
-    double s = 0.0;
-    for (int i = lane; i < nb; i += kWave) s += row[i];
-    s = wave_sum(s);
-    if (lane == 0) dv[slot] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double hd_raw = dv[1 + 2 * kBandStepMax];
-  const double z = sqrt(fmax(dv[0], 0.0));
-  if (z < accur_deflation) {
-    *stop = 1;
-    rec[6] = 1.0;
-    return;
-  }
-  const double iz = 1.0 / z;
-  double zz[kBandStepMax];
-  for (int k = 0; k < na; ++k) zz[k] = dv[1 + k] * iz;
-  double wdot = hd_raw * iz * iz;
-  double tb[kBandStepMax];
-  for (int b = 0; b < nw; ++b) {
-    const int k = k0 + b;
-    tb[b] = (k == j - pc) ? z : zz[k + pc - j - 1];
-    wdot -= tb[b] * dv[1 + kBandStepMax + b] * iz;
-  }
-  coef[0] = iz;
-  coef[1] = iz;
-  coef[2] = -wdot;
-  for (int k = 0; k < kBandStepMax; ++k) coef[3 + k] = k < na ? -zz[k] : 0.0;
-  for (int b = 0; b < kBandStepMax; ++b) coef[3 + kBandStepMax + b] = b < nw ? -tb[b] : 0.0;
-  rec[0] = z;
-  for (int k = 0; k < kBandStepMax; ++k) rec[1 + k] = k < na ? zz[k] : 0.0;
-  rec[5] = wdot;
-  rec[6] = 0.0;
 }
 
 template <int BS>
@@ -1267,9 +1301,18 @@ static int fused_grid(const FactArgs &a, uint32_t R) {
 int hv_factorized_max_partials(const FactArgs &a, int variant) { return kPartialsCap; }
 
 int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, double beta, double *part, int variant,
-                      const SlabConfig &sc, hipStream_t s, double *aux, size_t aux_cap) {
+                      const SlabConfig &sc_in, hipStream_t s, double *aux, size_t aux_cap) {
   const uint32_t du = a.up.dim, dd = a.dn.dim;
   if ((size_t)du * dd == 0) return 0;
+  SlabConfig sc = sc_in;
+  if (variant == HV_TWO_PASS_LDS) {
+    sc.lds = 1;
+    sc.sw = 0;
+    for (int w : {16, 8, 4})
+      if (!sc.sw && (size_t)dd * w * sizeof(double) <= kDownLdsMax) sc.sw = w;
+    if (!sc.sw) fail(ED_ERR_INVALID, "hv_variant 3: a slab of this sector's rows does not fit in LDS");
+    variant = HV_TWO_PASS;
+  }
   uint32_t R = rows_per_block(du);
   const bool stage = R > 0;
   if (!stage) R = 1;
@@ -1350,6 +1393,28 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   Q.x = x;
   Q.y = y;
   Q.ax = ax;
+  Q.ntab_dn = (a.dn.packed && a.dn.nval <= (uint32_t)kVtabMax) ? (int)a.dn.nval : 0;
+  if (sc.lds) {
+    // slabs resident in LDS (small sectors)
+    const int W = (sc.sw == 4 || sc.sw == 8 || sc.sw == 16) ? sc.sw : 8;
+    const size_t bytes = (size_t)dd * W * sizeof(double);
+    const size_t nb = (du + W - 1) / W;
+    if (bytes > kDownLdsMax || nb > (size_t)kPartialsCap)
+      fail(ED_ERR_INTERNAL, "LDS slab schedule chosen for a sector it does not fit");
+    Q.part = part;
+#define ED_DLDS(WV)                                                              \
+  do {                                                                           \
+    auto kern = k_hv_down_lds<WV>;                                               \
+    allow_lds(kern, bytes);                                                      \
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), bytes, s, Q);         \
+  } while (0)
+    if (W == 4) ED_DLDS(4);
+    else if (W == 16) ED_DLDS(16);
+    else ED_DLDS(8);
+#undef ED_DLDS
+    ED_HIP(hipGetLastError());
+    return (int)nb;
+  }
   Q.nxcd = (sc.nxcd == 1) ? 1 : 8;
   const int SW = (sc.sw == 8 || sc.sw == 16 || sc.sw == 64) ? sc.sw : 32;
   Q.nslab = (du + SW - 1) / SW;
@@ -1487,17 +1552,11 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
   }
 }
 
-void band_coef(const double *dpart, int dnb, int na, int rb, int nw, const double *hpart, int hnb, int j, int pc,
-               int k0, double accur_deflation, double *coef, double *rec, int *stop, hipStream_t s) {
-  if (na > kBandStepMax || nw > kBandStepMax) fail(ED_ERR_INTERNAL, "band_coef: window larger than the kernel's");
-  hipLaunchKernelGGL(k_band_coef, dim3(1), dim3(1024), 0, s, dpart, dnb, na, rb, nw, hpart, hnb, j, pc, k0,
-                     accur_deflation, coef, rec, stop);
-  ED_HIP(hipGetLastError());
-}
-
 int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
               int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s,
-              const double *dcoef, const int *stop) {
+              const BandFold *fold) {
+  const BandFold F = fold ? *fold : BandFold();
+  if (fold && (fold->stop == nullptr || fold->rec == nullptr)) fail(ED_ERR_INTERNAL, "band_step: fold without outputs");
   constexpr int BS = 1024;
   if (na < 0 || na > kBandStepMax || nw < 0 || nw > kBandStepMax || vector_block() != BS || n == 0) return 0;
   MPtrPack ap{};
@@ -1509,7 +1568,7 @@ int band_step(size_t n, double *x, double xs, int na, double *const *a, const do
 #define ED_BS(NA, NB)                                                                                         \
   if (na == NA && nw == NB) {                                                                                \
     hipLaunchKernelGGL((k_band_step<BS, NA, NB>), dim3(g), dim3(BS), 0, s, n, x, xs, ap, cap, t, st, wp, cbp, \
-                       cx, part, dcoef, stop);                                                               \
+                       cx, part, F);                                                                         \
     ED_HIP(hipGetLastError());                                                                               \
     return g;                                                                                                \
   }

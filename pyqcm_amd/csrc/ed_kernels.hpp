@@ -51,10 +51,13 @@ struct DevHalf {
 
 // slab configuration of the two-pass down kernel (autotuned per plan)
 struct SlabConfig {
-  int sw = 32;        // columns per slab (16, 32, 64)
+  int sw = 32;        // columns per slab (16, 32, 64; 4, 8, 16 with lds)
   int nt = 0;         // non-temporal y / own-x accesses (keep L2 for the slab)
   int nxcd = 8;       // 8: slabs partitioned over XCDs (block b -> XCD b % 8); 1: one shared sweep
+  int lds = 0;        // 1: each slab held in LDS (dim_dn x sw doubles <= kDownLdsMax)
 };
+// largest LDS slab of the down pass (two workgroups per CU)
+constexpr size_t kDownLdsMax = 64 * 1024;
 
 // Factorized H on X[i_dn][i_up]: H = D + (1 x K_up) + (K_dn x 1).
 struct FactArgs {
@@ -66,7 +69,9 @@ struct FactArgs {
   int row_order = 0;      // fused row kernel: 0 rows in label order, 1 in dn.order (neighbour rows reused sooner)
 };
 
-enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2 };
+// HV_TWO_PASS_LDS: the two-pass schedule with LDS-resident slabs (the widest
+// of 16, 8, 4 columns that fits); an error where no slab fits
+enum HvVariant { HV_AUTO = 0, HV_FUSED = 1, HV_TWO_PASS = 2, HV_TWO_PASS_LDS = 3 };
 // aux / aux_cap: device array for the two-pass schedule's per-slab-block
 // partials when they exceed partials_cap() (nullptr: a library-owned array,
 // allocated on first need; otherwise more than aux_cap of them is an error)
@@ -134,20 +139,27 @@ int multi_axpy_src(size_t n, double *x, int m, double *const *y, const double *c
 // (row r at part + r * nb): <c, c>, <c, a_k> (1 <= k < na), <c, t> (na > 0),
 // <c, w_b> (b < nw), <c, x>.  na, nw <= kBandStepMax; returns nb (0 when the
 // counts or the vector block size are outside what the kernel is built for)
-// With dcoef (device-driven steps) the scalars are read from there (layout of
-// band_coef) and a set *stop makes the launch a no-op.
+// Device-driven steps (fold given): the kernel derives the step's scalars
+// itself from partials on the device, in every workgroup: the dots of v_j
+// (rows of dnb: <v_j, v_j>, the na ahead, the nw behind from row rb) and
+// <H v_j, v_j> (hnb partials); step j's window starts at k0 (behind) with pc
+// vectors per band.  Workgroup 0 writes the host's record rec[7]: |v_j|,
+// zz[0 .. 3] (the ahead coefficients), t(j, j), deflated; |v_j| < accur
+// (a deflation) sets *stop, and a set *stop makes later steps no-ops.
 constexpr int kBandStepMax = 4;
+struct BandFold {
+  const double *dpart = nullptr;
+  int dnb = 0, rb = 0;
+  const double *hpart = nullptr;
+  int hnb = 0;
+  int j = 0, pc = 0, k0 = 0;
+  double accur = 0.0;
+  double *rec = nullptr;
+  int *stop = nullptr;
+};
 int band_step(size_t n, double *x, double xs, int na, double *const *a, const double *ca, double *t, double st,
               int nw, const double *const *w, const double *cb, double cx, double *part, hipStream_t s,
-              const double *dcoef = nullptr, const int *stop = nullptr);
-// The scalars of a device-driven band-Lanczos step j from device partials: the
-// dots of v_j (rows of dnb: <v_j, v_j>, na ahead, nw behind from row rb) and
-// <H v_j, v_j> (hnb partials).  coef (3 + 2 kBandStepMax doubles): xs = st =
-// 1/|v_j|, cx = -t(j, j), ca = -zz (the ahead coefficients), cb = -t(k, j)
-// (behind, k = k0 ..); rec (7 doubles): |v_j|, zz[0..3], t(j, j), deflated.
-// |v_j| < accur_deflation sets *stop (and rec[6]); a set *stop skips the step.
-void band_coef(const double *dpart, int dnb, int na, int rb, int nw, const double *hpart, int hnb, int j, int pc,
-               int k0, double accur_deflation, double *coef, double *rec, int *stop, hipStream_t s);
+              const BandFold *fold = nullptr);
 // t = st * t + sum_k c_k b_k ; optional partials of ||t||^2 and <t, w>. returns nb
 int combine(size_t n, double *t, double st, int m, const double *const *b, const double *c, double *norm_partials,
             const double *w, double *dot_partials, hipStream_t s);

@@ -3,12 +3,14 @@
 #include "ed_dense.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <unordered_map>
 
@@ -927,8 +929,10 @@ struct HamFactorized : Hamiltonian {
     args.block = (int)params().geti("hv_block");
     args.row_order = (int)params().geti("hv_row_order");
   }
-  // candidate schedules, in the order reported by ed_hamiltonian_autotune
-  static std::vector<std::pair<int, SlabConfig>> candidates() {
+  // candidate schedules, in the order reported by ed_hamiltonian_autotune:
+  // the fused kernel, the two-pass schedule with L2-resident slabs, and (when
+  // a slab of dd rows fits) with LDS-resident slabs
+  static std::vector<std::pair<int, SlabConfig>> candidates(uint32_t dd) {
     std::vector<std::pair<int, SlabConfig>> c;
     c.push_back({HV_FUSED, SlabConfig()});
     for (int sw : {8, 16, 32})
@@ -938,12 +942,33 @@ struct HamFactorized : Hamiltonian {
         s.nt = nt;
         c.push_back({HV_TWO_PASS, s});
       }
+    for (int sw : {4, 8, 16})
+      if ((size_t)dd * sw * sizeof(double) <= kDownLdsMax) {
+        SlabConfig s;
+        s.sw = sw;
+        s.lds = 1;
+        c.push_back({HV_TWO_PASS, s});
+      }
     return c;
   }
-  void autotune() {
+  // force: time the candidates even when the shape's result is cached
+  void autotune(bool force = false) {
     tuned = true;
     best_variant = HV_FUSED;
-    if (dim < ((size_t)1 << 20)) return;  // small sectors: one fused launch
+    const bool small = dim < ((size_t)1 << 20);
+    // small sectors: the fused kernel or an LDS-slab two-pass schedule, timed
+    // once per sector shape in the process (the tables' sizes key it)
+    static std::map<std::array<uint64_t, 4>, std::pair<int, SlabConfig>> small_cache;
+    const std::array<uint64_t, 4> key{args.up.dim, args.dn.dim, (uint64_t)up.host.ptr.back(),
+                                      (uint64_t)dn.host.ptr.back()};
+    if (small && !force) {
+      auto it = small_cache.find(key);
+      if (it != small_cache.end()) {
+        best_variant = it->second.first;
+        best_sc = it->second.second;
+        return;
+      }
+    }
     DevBuf<double> x(dim), y(dim);
     fill(dim, x.p, 1.0 / std::sqrt((double)dim), stream());
     fill(dim, y.p, 0.0, stream());
@@ -952,7 +977,11 @@ struct HamFactorized : Hamiltonian {
     ED_HIP(hipEventCreate(&e1));
     double best = 1e300;
     tune_ms.clear();
-    for (const auto &c : candidates()) {
+    for (const auto &c : candidates(args.dn.dim)) {
+      if (small && c.first == HV_TWO_PASS && !c.second.lds) {
+        tune_ms.push_back(-1.0);  // L2-slab schedules: for sectors beyond the LDS slabs
+        continue;
+      }
       hv_factorized_cfg(args, x.p, y.p, 1.0, 0.0, nullptr, c.first, c.second, stream());
       ED_HIP(hipEventRecord(e0, stream()));
       for (int r = 0; r < 3; ++r) hv_factorized_cfg(args, x.p, y.p, 1.0, 0.0, nullptr, c.first, c.second, stream());
@@ -970,6 +999,7 @@ struct HamFactorized : Hamiltonian {
     }
     ED_HIP(hipEventDestroy(e0));
     ED_HIP(hipEventDestroy(e1));
+    if (small) small_cache[key] = {best_variant, best_sc};
   }
   int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
     int v = variant;
@@ -1195,12 +1225,12 @@ struct HamOTF : Hamiltonian {
 std::vector<double> hamiltonian_autotune(Hamiltonian &H, int &best) {
   auto *F = dynamic_cast<HamFactorized *>(&H);
   if (!F) fail(ED_ERR_UNSUPPORTED, "autotuning applies to the factorized format only");
-  F->autotune();
-  const auto cands = HamFactorized::candidates();
+  F->autotune(true);
+  const auto cands = HamFactorized::candidates(F->args.dn.dim);
   best = 0;
   for (size_t k = 0; k < cands.size(); ++k)
     if (cands[k].first == F->best_variant && cands[k].second.sw == F->best_sc.sw &&
-        cands[k].second.nt == F->best_sc.nt)
+        cands[k].second.nt == F->best_sc.nt && cands[k].second.lds == F->best_sc.lds)
       best = (int)k;
   return F->tune_ms;
 }

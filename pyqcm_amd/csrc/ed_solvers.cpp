@@ -620,60 +620,81 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
     return test < accur_bl;
   };
   // Device-driven steps (no deflated vectors, windows of at most
-  // kBandStepMax vectors, no P-matrix column): each step's scalars are
-  // computed on the device (band_coef) from the dots of the previous fused
-  // update and the product's partials, so `dev_block` steps are queued without
-  // a host wait; the host then fills t from the steps' records and applies the
-  // convergence check in step order.  A deflation found on the device stops
-  // the block (the remaining launches do nothing) and the host redoes that
-  // step; a convergence ends the run at the same step as the host-driven loop.
+  // kBandStepMax vectors, no P-matrix column): each fused step derives its own
+  // scalars on the device from the dots the previous one left and the
+  // product's partials (band_step with a BandFold), so blocks of
+  // `band_lanczos_block` steps are queued without a host wait.  The host reads
+  // a block's records (pinned, one event per block) while the next block runs,
+  // fills t and applies the convergence check in step order.  A deflation
+  // found on the device makes the remaining launches no-ops and the host
+  // redoes that step; a convergence ends the run at the same step as the
+  // host-driven loop (a block queued past it is discarded).
   const int dev_block = (int)Pm.geti("band_lanczos_block");
   const bool dev_on = dev_block >= 2 && Pm.geti("vector_block") == 1024;
   auto dev_ok = [&](int jj) {
     return dev_on && nd == 0 && jj >= p && pc - 1 <= kBandStepMax && pc <= kBandStepMax && jj + 1 < M0;
   };
-  DevBuf<double> dring, hring, cring, rring;
+  DevBuf<double> dring, hring, rring;
   DevBuf<int> dstop;
+  double *rhost = nullptr;  // pinned: two blocks of records
+  hipEvent_t rev[2] = {nullptr, nullptr};
   const size_t drow = (size_t)(2 * kBandStepMax + 2) * (size_t)partials_cap();
   const size_t hcap = (size_t)partials_cap();
   bool dev_converged = false;
+  struct DevCleanup {
+    double *&h;
+    hipEvent_t *ev;
+    ~DevCleanup() {
+      if (h) (void)hipHostFree(h);
+      for (int q = 0; q < 2; ++q)
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
+    }
+  } dev_cleanup{rhost, rev};
   // runs device blocks from step j0 (dots of v_j0 in the pinned slot `pre`);
   // returns the step the host loop continues with (its dots not available)
   auto run_device = [&](int j0) -> int {
     if (!dring.p) {
-      dring.alloc((size_t)(dev_block + 1) * drow);
-      hring.alloc((size_t)dev_block * hcap);
-      cring.alloc((size_t)dev_block * 16);
+      dring.alloc(2 * drow);
+      hring.alloc(2 * hcap);
       rring.alloc((size_t)dev_block * 8);
       dstop.alloc(1);
+      ED_HIP(hipHostMalloc((void **)&rhost, 2 * (size_t)dev_block * 8 * sizeof(double), hipHostMallocDefault));
+      for (int q = 0; q < 2; ++q) ED_HIP(hipEventCreateWithFlags(&rev[q], hipEventDisableTiming));
     }
+    // the incoming dots: the pinned slot the host-driven step left
     const double *din = pre;
     int din_nb = pre_nb, din_rb = 1 + pre_na + pre_skip;
     double *pinned = pre;
     const int pinned_nb = pre_nb, pinned_m = pre_m;
     pre = nullptr;
-    int jcur = j0;
-    while (true) {
-      const int K = std::min(dev_block, M0 - jcur);
-      ED_HIP(hipMemsetAsync(dstop.p, 0, sizeof(int), stream()));
-      int dslot = 0;
-      for (int s2 = 0; s2 < K; ++s2) {
-        const int js = jcur + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
+    ED_HIP(hipMemsetAsync(dstop.p, 0, sizeof(int), stream()));
+    int nstep = 0;  // steps queued so far (dots / partials slots alternate)
+    auto launch_block = [&](int jb, int K, int parity) {
+      for (int s2 = 0; s2 < K; ++s2, ++nstep) {
+        const int js = jb + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
         const int na = jm2 - js - 1, nw = js - k02;
         v[js + pc].reset(new DevBuf<double>(dim));
-        const int hnb2 = H.apply_fixed(v[js]->p, v[js + pc]->p, 1.0, 0.0, hring.p + s2 * hcap, nullptr, 0);
-        band_coef(din, din_nb, na, din_rb, nw, hring.p + s2 * hcap, hnb2, js, pc, k02, accur_deflation,
-                  cring.p + s2 * 16, rring.p + s2 * 8, dstop.p, stream());
+        double *hp = hring.p + (size_t)(nstep & 1) * hcap;
+        const int hnb2 = H.apply_fixed(v[js]->p, v[js + pc]->p, 1.0, 0.0, hp, nullptr, 0);
         std::vector<double *> ys;
         std::vector<const double *> ws;
         for (int k = js + 1; k < jm2; ++k) ys.push_back(v[k]->p);
         for (int k = k02; k < js; ++k) ws.push_back(v[k]->p);
-        // the block's dots go to ring slots 0 .. K-1, avoiding the slot the
-        // block's first step reads (the previous block's last one)
-        dslot = (din == dring.p + (size_t)s2 * drow) ? dev_block : s2;
-        double *dout = dring.p + (size_t)dslot * drow;
+        BandFold F;
+        F.dpart = din;
+        F.dnb = din_nb;
+        F.rb = din_rb;
+        F.hpart = hp;
+        F.hnb = hnb2;
+        F.j = js;
+        F.pc = pc;
+        F.k0 = k02;
+        F.accur = accur_deflation;
+        F.rec = rring.p + (size_t)s2 * 8;
+        F.stop = dstop.p;
+        double *dout = dring.p + (size_t)(nstep & 1) * drow;
         const int nb = band_step(dim, v[js]->p, 0.0, na, ys.data(), nullptr, v[js + pc]->p, 0.0, nw, ws.data(),
-                                 nullptr, 0.0, dout, stream(), cring.p + s2 * 16, dstop.p);
+                                 nullptr, 0.0, dout, stream(), &F);
         if (nb <= 0) {
           sync();
           fail(ED_ERR_INTERNAL, "band Lanczos: device step outside the fused kernel's range");
@@ -683,31 +704,59 @@ BandLanczosRun band_lanczos_run(Hamiltonian &H, const std::vector<const double *
         din_nb = nb;
         din_rb = 1 + na + (k0n - k02);
       }
-      std::vector<double> rec((size_t)K * 8);
-      ED_HIP(hipMemcpyAsync(rec.data(), rring.p, rec.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
-      sync();
+      ED_HIP(hipMemcpyAsync(rhost + (size_t)parity * dev_block * 8, rring.p, (size_t)K * 8 * sizeof(double),
+                            hipMemcpyDeviceToHost, stream()));
+      ED_HIP(hipEventRecord(rev[parity], stream()));
+    };
+    int jb = j0, K = std::min(dev_block, M0 - jb), parity = 0;
+    launch_block(jb, K, parity);
+    while (true) {
+      const int jn = jb + K;
+      const bool next = dev_ok(jn);
+      const int Kn = next ? std::min(dev_block, M0 - jn) : 0;
+      if (next) launch_block(jn, Kn, parity ^ 1);
+      ED_HIP(hipEventSynchronize(rev[parity]));
       if (pinned) {
-        sum_partials(pinned, pinned_nb, pinned_m, nullptr);  // release the slot the first block read
+        sum_partials(pinned, pinned_nb, pinned_m, nullptr);  // release the slot the first step read
         pinned = nullptr;
       }
+      const double *recs = rhost + (size_t)parity * dev_block * 8;
       for (int s2 = 0; s2 < K; ++s2) {
-        const int js = jcur + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
-        const double *r = rec.data() + (size_t)s2 * 8;
-        if (r[6] != 0.0) return js;  // deflation: the host redoes step js
+        const int js = jb + s2, jm2 = js + pc, k02 = (js > pc) ? js - pc : 0;
+        const double *r = recs + (size_t)s2 * 8;
+        if (r[6] != 0.0) {  // deflation: the host redoes step js
+          sync();
+          return js;
+        }
         if (js >= pc) T(js, js - pc) = r[0];
         for (int k = js + 1; k < jm2; ++k)
           if (k >= pc) T(js, k - pc) = r[1 + (k - js - 1)];
         for (int k = k02; k < js; ++k) T(k, js) = T(js, k);
         T(js, js) = r[5];
         if (js > pc && def[k02 - 1] == 0) v[k02 - 1].reset();
-        if (converged_at(js)) {
+        bool conv;
+        try {
+          conv = converged_at(js);
+        } catch (...) {
+          sync();
+          throw;
+        }
+        if (conv) {
+          sync();
           dev_converged = true;
           return js;
         }
       }
-      jcur += K;
-      if (!dev_ok(jcur)) return jcur;
-      check_cancel();
+      if (!next) return jn;
+      jb = jn;
+      K = Kn;
+      parity ^= 1;
+      try {
+        check_cancel();
+      } catch (...) {
+        sync();
+        throw;
+      }
     }
   };
   for (j = 0; j < M0; ++j) {

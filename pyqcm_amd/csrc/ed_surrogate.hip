@@ -157,12 +157,16 @@ __global__ __launch_bounds__(256) void k_colsum(int M, int N, int ld, int rc, co
   __syncthreads();
   if (rg == 0 && n < N) part[(long)blockIdx.y * N + n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
+// 64 columns per workgroup, 4 row groups over the R chunks, fixed order
 __global__ __launch_bounds__(256) void k_colsum_fold(int R, int N, const float *part, float *db) {
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= N) return;
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6, n = blockIdx.x * 64 + c;
   float s = 0.f;
-  for (int r = 0; r < R; ++r) s += part[(long)r * N + n];
-  db[n] = s;
+  if (n < N)
+    for (int r = rg; r < R; r += 4) s += part[(long)r * N + n];
+  red[rg][c] = s;
+  __syncthreads();
+  if (rg == 0 && n < N) db[n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
 // only the R x C logical entries are written: the padding of out keeps its zeros
@@ -255,7 +259,7 @@ void mlp_colsum(int M, int N, int ld, const bf16_t *dz, float *db, float *part, 
   const int rc = (M + R - 1) / R;
   hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, R), dim3(256), 0, s, M, N, ld, rc, dz, part);
   ED_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_colsum_fold, dim3((N + 255) / 256), dim3(256), 0, s, R, N, (const float *)part, db);
+  hipLaunchKernelGGL(k_colsum_fold, dim3((N + 63) / 64), dim3(256), 0, s, R, N, (const float *)part, db);
   ED_HIP(hipGetLastError());
 }
 

@@ -83,7 +83,7 @@ def test_csr_pattern_bit_exact(oracle):
     assert np.allclose(v, v2, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_factorized_variants_agree(oracle, variant):
     wl = W.plaquette_bath(n_bath=4)     # L = 8, dim(4,4) = 4900
     lab = _instance(wl, "R0:N8:S0")
@@ -98,7 +98,7 @@ def test_factorized_variants_agree(oracle, variant):
 @pytest.mark.parametrize("row_order", [0, 1])
 @pytest.mark.parametrize("block", [0, 1024])
 @pytest.mark.parametrize("sell16", [True, False])
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("kind", ["chain12", "plaquette8b"])
 def test_factorized_sliced_ell_modes(oracle, kind, variant, sell16, block, row_order):
     """Spin-up rows of >= 512 states take the sliced-ELL row kernel; a single
