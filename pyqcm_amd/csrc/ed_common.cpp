@@ -169,6 +169,37 @@ void dev_free(void *p, size_t bytes) {
   g_cached_bytes += b;
 }
 
+void h2d_staged(void *dst, const void *src, size_t bytes) {
+  // pinned staging ring: slot k is reused once the copy out of it has run
+  // (its event); the host copy into pinned memory is the only host wait, so
+  // the many small table uploads of a sector no longer each pay a pageable
+  // transfer's synchronous staging
+  constexpr int kSlots = 16;
+  constexpr size_t kSlot = size_t(512) << 10;
+  static std::mutex mu;
+  static char *pin = nullptr;
+  static hipEvent_t ev[kSlots];
+  static bool used[kSlots] = {};
+  static int next = 0;
+  if (!bytes) return;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!pin) {
+    ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&pin), kSlots * kSlot, hipHostMallocDefault));
+    for (int i = 0; i < kSlots; ++i) ED_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+  }
+  for (size_t off = 0; off < bytes; off += kSlot) {
+    const size_t m = std::min(kSlot, bytes - off);
+    const int k = next;
+    next = (next + 1) % kSlots;
+    if (used[k]) ED_HIP(hipEventSynchronize(ev[k]));
+    char *slot = pin + (size_t)k * kSlot;
+    std::memcpy(slot, static_cast<const char *>(src) + off, m);
+    ED_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, slot, m, hipMemcpyHostToDevice, stream()));
+    ED_HIP(hipEventRecord(ev[k], stream()));
+    used[k] = true;
+  }
+}
+
 void d2h_pageable(void *dst, const void *src, size_t bytes) {
   constexpr size_t kChunk = size_t(8) << 20;
   if (bytes <= kChunk) {

@@ -68,6 +68,9 @@ void dev_release_cache();
 size_t dev_available_bytes();
 hipStream_t stream();  // the library's compute stream (default priority)
 
+// host-to-device copy out of pageable memory through a pinned staging ring
+// (asynchronous on the library stream once the data is staged)
+void h2d_staged(void *dst, const void *src, size_t bytes);
 // device-to-host copy into pageable memory through pinned staging buffers
 // (transfers of one chunk overlap the host copy of the previous one);
 // synchronizes the library stream
@@ -117,7 +120,7 @@ struct DevBuf {
   }
   void upload(const T *h, size_t count) {
     if (count > n) alloc(count);
-    if (count) ED_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+    if (count) h2d_staged(p, h, count * sizeof(T));
   }
   void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
   void download(T *h, size_t count) const { d2h_pageable(h, p, count * sizeof(T)); }

@@ -99,7 +99,7 @@ class LanczosDeviceRun {
     base.alloc(1);
     d2d(R[0].p, d_start, n);
     const int zero = 0;
-    ED_HIP(hipMemcpyAsync(base.p, &zero, sizeof(int), hipMemcpyHostToDevice, stream()));
+    h2d_staged(base.p, &zero, sizeof(int));
     D.n = n;
     D.cap = cap;
     D.coef = coef.p;
@@ -193,8 +193,7 @@ class LanczosDeviceRun {
       host_table[basis.size() - 1] = basis.back().p;
     }
     if (basis.size() > first)
-      ED_HIP(hipMemcpyAsync(table.p + first, host_table.data() + first, (basis.size() - first) * sizeof(double *),
-                            hipMemcpyHostToDevice, stream()));
+      h2d_staged(table.p + first, host_table.data() + first, (basis.size() - first) * sizeof(double *));
   }
   Hamiltonian &H;
   size_t n;

@@ -153,7 +153,7 @@ void Surrogate::ensure_batch(int B) {
   dz2_b.alloc((size_t)B * width);
   part.alloc(1024);
   // split-k partials of the weight gradients: S slabs of the largest layer
-  split_S = std::max(1, std::min(32, B / 128));
+  split_S = std::max(1, std::min(8, B / 256));
   size_t wmax = 0, omax = 0;
   for (const Layer &L : layers) {
     wmax = std::max(wmax, (size_t)L.out_pad * L.in_pad);
