@@ -150,6 +150,14 @@ TermList Instance::terms() const {
   return T;
 }
 
+bool Instance::complex_hs() const { return terms_complex(terms()); }
+
+void Instance::need_real(const char *what) const {
+  if (complex_hs())
+    fail(ED_ERR_UNSUPPORTED, std::string(what) + " in a complex Hilbert space is not supported (instance " +
+                                 std::to_string(label) + ")");
+}
+
 std::string Instance::gs_string() const {
   std::map<Sector, double> ws;
   for (const auto &s : states) ws[s->sec] += s->weight;
@@ -266,15 +274,22 @@ std::pair<double, std::string> Instance::ground_state_solve() {
   char method = P.getc("Ground_state_method");
   const double acc_dav = P.getd("accur_Davidson");
   TermList T = terms();
+  const bool cx = terms_complex(T);
+  if (cx && model->group.g > 1)
+    fail(ED_ERR_UNSUPPORTED, "complex operators with a point group are not supported");
   states.clear();
   GS_energy = 1.0e12;
   for (const auto &s : targets) {
     check_cancel();
-    const size_t dim = sector_dim(model->L, s);
+    // vectors of a complex Hilbert space are split [Re | Im]
+    const size_t dim = sector_dim(model->L, s) * (cx ? 2 : 1);
     if (dim == 0) continue;
-    auto H = make_hamiltonian(T, s, format);
+    auto H = make_hamiltonian(T, s, format, cx);
     long ns = std::max<long>(1, std::min<long>(n_states, (long)dim));
     char meth = (method == 'L' && ns > 1) ? 'D' : method;
+    // each level of the split form appears twice (psi and i psi)
+    if (cx && meth == 'D')
+      fail(ED_ERR_UNSUPPORTED, "several states per sector (Davidson) in a complex Hilbert space are not supported");
     // a point group: the sector's states are those of the full (N, S) sector in
     // representation s.irrep
     std::unique_ptr<GroupProjector> proj;
@@ -402,13 +417,16 @@ const std::vector<std::tuple<std::string, double, double>> &Instance::cluster_av
   for (auto &x : states)
     if (!x->psi.p) fail(ED_ERR_INVALID, "instance " + std::to_string(label) + " holds no ground-state vector (read from a file)");
   averages.clear();
+  const bool cx = complex_hs();
   for (const auto &kv : value) {
     double ave = 0.0, var = 0.0;
     TermList one;
     one.L = model->L;
     one.terms.push_back({&model->ops.at(kv.first), 1.0});
     for (auto &x : states) {
-      auto Ho = make_hamiltonian(one, x->sec, format);
+      // <psi|O|psi> is the quadratic form of the split operator (real for a
+      // Hermitian O)
+      auto Ho = make_hamiltonian(one, x->sec, format, cx);
       DevBuf<double> y(Ho->dim);
       const int nb = Ho->apply(x->psi.p, y.p, 1.0, 0.0, true);
       const double e = Ho->dot_result(nb);
@@ -668,7 +686,161 @@ GFRep::MCF combine_mcf(const GFRep::MCF &e, const GFRep::MCF &h, int n, bool qr)
   return out;
 }
 
+// complex residues: Q_matrix::streamline and check_norm for HilbertField =
+// Complex (src_ed/Q_matrix.hpp:181-223, :236-260, streamlineQ :286-316); within
+// a group of poles closer than Qmatrix_wtol, sum_k V_k V_k^H is diagonalized
+// and its eigenvectors above Qmatrix_vtol, scaled by the root of their
+// eigenvalue, replace the group's residues
+static void streamline_c(int L, std::vector<double> &e, std::vector<cplx> &V) {
+  Params &P = params();
+  const double wtol = P.getd("Qmatrix_wtol"), vtol = P.getd("Qmatrix_vtol");
+  if (vtol < 1e-12) return;
+  const int M = (int)e.size();
+  std::vector<double> e2;
+  std::vector<cplx> v2;
+  for (int i = 0; i < M; ++i) {
+    int j;
+    for (j = i + 1; j < M; ++j)
+      if (std::fabs(e[j] - e[i]) > wtol) break;
+    const int n = j - i;
+    if (n == 1) {
+      double s = 0.0;
+      for (int a = 0; a < L; ++a) s += std::norm(V[a + (size_t)i * L]);
+      if (!(s < vtol)) {
+        e2.push_back(e[i]);
+        v2.insert(v2.end(), V.begin() + (size_t)i * L, V.begin() + (size_t)(i + 1) * L);
+      }
+    } else {
+      std::vector<cplx> QQ((size_t)L * L, 0.0), U((size_t)L * L);
+      std::vector<double> d(L);
+      for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+          cplx s = 0.0;
+          for (int k = 0; k < n; ++k) s += V[a + (size_t)(i + k) * L] * std::conj(V[b + (size_t)(i + k) * L]);
+          QQ[a + (size_t)b * L] = s;
+        }
+      hermeig(L, QQ.data(), d.data(), U.data());
+      for (int c = 0; c < L; ++c)
+        if (d[c] > vtol) {
+          e2.push_back(e[i]);
+          const double sq = std::sqrt(d[c]);
+          for (int a = 0; a < L; ++a) v2.push_back(U[a + (size_t)c * L] * sq);
+        }
+    }
+    i += n - 1;
+  }
+  e.swap(e2);
+  V.swap(v2);
+}
+
+static double qnorm_error_c(int L, const std::vector<cplx> &V) {
+  const size_t M = V.size() / std::max(L, 1);
+  double tot = 0.0;
+  for (int i = 0; i < L; ++i) {
+    double z = 0.0;
+    for (size_t k = 0; k < M; ++k) z += std::norm(V[i + k * L]);
+    tot += (z - 1.0) * (z - 1.0);
+    for (int j = 0; j < i; ++j) {
+      cplx zz = 0.0;
+      for (size_t k = 0; k < M; ++k) zz += V[i + k * L] * std::conj(V[j + k * L]);
+      tot += 2.0 * std::norm(zz);
+    }
+  }
+  return tot;
+}
+
+// Green function of a state of a complex Hilbert space (GF_method 'L' only;
+// build_qmatrix, src_ed/model_instance.hpp:711-818, with HilbertField = Complex).
+// On split vectors the plan is the real symmetric [[A, -B], [B, A]], which
+// commutes with K: [x_r | x_i] -> [-x_i | x_r] (multiplication by i).  The band
+// Lanczos starts from the 2 ns vectors [phi_r | phi_i] and K [phi_r | phi_i]
+// (phi_a = c^{pm}_a |Omega>), so its Krylov space is closed under K and each
+// level of H appears twice among its Ritz values.  For a Ritz vector w with
+// projections W on the start vectors, the complex vector it represents gives
+// <phi_a|n> = W_a + i W_{ns+a}; over the two copies of a level these sum to
+// twice the complex residue, hence the factor 1/sqrt(2), and streamlining
+// merges the copies.  As there, the electron part stores conj(<phi_a|n>) and
+// the hole part <chi_a|m>, so that G(r, c) = sum_i V(c, i) conj(V(r, i)) / (z - e_i).
+static void build_gf_complex(Instance &I, State &st, int spin) {
+  if (I.gf_method != 'L')
+    fail(ED_ERR_UNSUPPORTED, std::string("GF_method ") + I.gf_method +
+                                 " in a complex Hilbert space is not supported (use L)");
+  const int L = I.model->L, ns = I.model->n_sites;
+  const TermList T = I.terms();
+  const size_t ds = sector_dim(L, st.sec);
+  GFRep &G = st.gf[spin];
+  G = GFRep();
+  G.kind = 'L';
+  G.n = ns;
+  std::vector<double> ee, eh;
+  std::vector<cplx> Ve, Vh;
+  const double E = st.energy, r2 = 1.0 / std::sqrt(2.0);
+  for (int pm = -1; pm <= 1; pm += 2) {
+    check_cancel();
+    Sector tgt = shift_sector(st.sec, pm, spin);
+    if (!sector_valid(L, tgt)) continue;
+    const size_t dt = sector_dim(L, tgt);
+    if (dt == 0) continue;
+    auto H = make_hamiltonian(T, tgt, I.format, true);
+    std::vector<DevBuf<double>> phi;
+    for (int i = 0; i < ns; ++i) {
+      phi.emplace_back(2 * dt);
+      fill(2 * dt, phi.back().p, 0.0, stream());
+      cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p, 1.0, phi.back().p);
+      cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p + ds, 1.0, phi.back().p + dt);
+    }
+    for (int i = 0; i < ns; ++i) {
+      // K phi_i = [-phi_i,im | phi_i,re]
+      phi.emplace_back(2 * dt);
+      double *k = phi.back().p;
+      const double *x = phi[i].p;
+      ED_HIP(hipMemcpyAsync(k, x + dt, dt * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+      ED_HIP(hipMemcpyAsync(k + dt, x, dt * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+      scale(dt, k, -1.0, stream());
+    }
+    std::vector<const double *> ph;
+    for (auto &b : phi) ph.push_back(b.p);
+    vlog("GF %s (pm %+d, spin %d, complex): dim %zu, %d start vectors ready", tgt.name().c_str(), pm, spin, dt,
+         2 * ns);
+    BandLanczosRun run = band_lanczos_run(*H, ph);
+    std::vector<double> e, W;
+    run.finish(e, W);
+    const int p = 2 * ns, M = (int)e.size();
+    std::vector<cplx> V((size_t)ns * M);
+    for (int n = 0; n < M; ++n)
+      for (int a = 0; a < ns; ++a) {
+        const double re = W[a + (size_t)n * p], im = W[ns + a + (size_t)n * p];
+        V[a + (size_t)n * ns] = r2 * cplx(re, pm == 1 ? -im : im);
+      }
+    for (auto &x : e) x -= E;
+    if (pm == -1)
+      for (auto &x : e) x = -x;
+    streamline_c(ns, e, V);
+    vlog("band Lanczos (complex): %d Ritz values, %zu poles", M, e.size());
+    if (pm == -1) { eh.swap(e); Vh.swap(V); }
+    else { ee.swap(e); Ve.swap(V); }
+  }
+  std::vector<cplx> V = Vh;
+  V.insert(V.end(), Ve.begin(), Ve.end());
+  G.e = eh;
+  G.e.insert(G.e.end(), ee.begin(), ee.end());
+  const double err = qnorm_error_c(ns, V);
+  if (err > params().getd("accur_Q_matrix"))
+    fail(ED_ERR_QNORM, "Q-matrix does not satisfy unitary condition by " + std::to_string(err));
+  const double sw = std::sqrt(st.weight);
+  G.V.resize(V.size());
+  G.Vi.resize(V.size());
+  for (size_t k = 0; k < V.size(); ++k) {
+    G.V[k] = sw * V[k].real();
+    G.Vi[k] = sw * V[k].imag();
+  }
+}
+
 static void build_gf(Instance &I, State &st, int spin) {
+  if (I.complex_hs()) {
+    build_gf_complex(I, st, spin);
+    return;
+  }
   const int L = I.model->L, ns = I.model->n_sites;
   const TermList T = I.terms();
   const size_t dim0 = sector_dim(L, st.sec);

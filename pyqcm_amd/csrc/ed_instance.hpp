@@ -26,8 +26,10 @@ struct CfStore {
 struct GFRep {
   char kind = 0;  // 'L' Lehmann (Q-matrix), 'F' continued fractions, 'M' matrix continued fractions
   int n = 0;      // GF dimension (cluster orbitals)
-  // Lehmann: poles e[M], residues V (n x M, column-major), weight folded in
-  std::vector<double> e, V;
+  // Lehmann: poles e[M], residues V (n x M, column-major), weight folded in;
+  // in a complex Hilbert space Vi holds their imaginary parts (empty
+  // otherwise) and G(r, c) = sum_i V(c, i) conj(V(r, i)) / (z - e_i)
+  std::vector<double> e, V, Vi;
   // continued fractions per ordered pair (o1 >= o2) for electrons / holes; a, b
   // already transformed (continued_fraction ctor, src_ed/continued_fraction.cpp:33-42)
   std::vector<std::vector<double>> ea, eb, ha, hb;
@@ -65,6 +67,12 @@ struct Instance {
   bool averages_done = false;
 
   TermList terms() const;
+  // complex Hilbert space: a term defined with complex elements has a nonzero
+  // coefficient (ED::new_model_instance, src_ed/qcm_ED.cpp:145-157); state
+  // vectors are then split [Re | Im], twice the sector dimension
+  bool complex_hs() const;
+  // fails with ED_ERR_UNSUPPORTED for a complex Hilbert space
+  void need_real(const char *what) const;
   std::pair<double, std::string> ground_state_solve();
   // weights, filters and E0 of the candidate states in 'states' (each with its
   // sector, energy and normalized vector); marks the ground state solved
@@ -84,6 +92,8 @@ struct Instance {
   int gf_spin(bool spin_down) const;
   // one-body matrix of the spin block (L x L, row-major) from the instance's terms
   std::vector<double> one_body_matrix(bool spin_down) const;  // L x L, row-major, full matrix
+  // the same with the imaginary parts (Hermitian)
+  std::vector<cplx> one_body_matrix_c(bool spin_down) const;
   double sef_bath() const;
   // frequency integral of the Green function of the spin block, n x n
   // column-major (real Hamiltonians: real and symmetric)
@@ -105,6 +115,8 @@ struct Instance {
   // is not diagonal already (eigenvalues ascending, tcb -> tcb S)
   struct Hopping {
     std::vector<double> tc, tb, tcb;
+    // imaginary parts (complex Hilbert space; empty otherwise)
+    std::vector<double> tci, tbi, tcbi;
   };
   Hopping hopping_blocks(bool spin_down, bool diag) const;
   // sum over pairs of low-energy states in the same sector of

@@ -327,29 +327,36 @@ uint32_t PointGroup::apply(int k, uint32_t b, int &sign) const {
   return out;
 }
 
-// ED::new_operator (src_ed/qcm_ED.cpp:74-90) with the element handling of the
-// one_body_operator (one_body_operator.hpp:59-76) and interaction_operator
-// (interaction_operator.hpp:37-57) constructors.
+// ED::new_operator (src_ed/qcm_ED.cpp:74-90, complex elements :95-106) with the
+// element handling of the one_body_operator (one_body_operator.hpp:59-76,
+// check_Hermiticity :179-188) and interaction_operator (interaction_operator.hpp:37-57)
+// constructors.
 void new_operator(const std::string &model, const std::string &name, const std::string &type,
-                  const std::vector<Element> &el) {
+                  const std::vector<Element> &el, bool is_complex) {
   if (el.empty()) return;
   Model &M = get_model(model);
   if (M.closed) return;  // reference: warning, model closed for modifications
   const int L = M.L;
   Operator op;
   op.name = name;
+  op.is_complex = is_complex;
   for (const auto &x : el)
     if (x.r < 0 || x.c < 0 || x.r >= 2 * L || x.c >= 2 * L)
       fail(ED_ERR_INVALID, "element (" + std::to_string(x.r) + "," + std::to_string(x.c) + ") of operator " + name +
                                " is out of range for " + std::to_string(2 * L) + " spin-orbitals");
+  if (is_complex && type != "one-body")
+    fail(ED_ERR_UNSUPPORTED, "type of operator " + name + " (" + type +
+                                 ") with complex elements is not supported by this solver");
   if (type == "one-body") {
     op.kind = OP_ONE_BODY;
+    // the lower triangle is the conjugate of the upper one
     for (const auto &x : el) {
       if (x.r < x.c) {
-        op.el.push_back({x.r, x.c, x.v});
-        op.el.push_back({x.c, x.r, x.v});
+        op.el.push_back({x.r, x.c, x.v, x.vi});
+        op.el.push_back({x.c, x.r, x.v, -x.vi});
       } else if (x.r == x.c) {
-        op.el.push_back({x.r, x.r, x.v});
+        if (std::fabs(x.vi) > 1e-12) fail(ED_ERR_INVALID, "operator " + name + " is not Hermitian");
+        op.el.push_back({x.r, x.r, x.v, 0.0});
       }
     }
     for (const auto &x : op.el)
@@ -362,8 +369,8 @@ void new_operator(const std::string &model, const std::string &name, const std::
   }
   // spin symmetry (one_body_operator.hpp:187-205): up and down parts equal?
   {
-    std::map<std::pair<int, int>, double> m;
-    for (const auto &x : op.el) m[{x.r, x.c}] += x.v;
+    std::map<std::pair<int, int>, cplx> m;
+    for (const auto &x : op.el) m[{x.r, x.c}] += cplx(x.v, x.vi);
     for (const auto &kv : m) {
       int r = kv.first.first, c = kv.first.second;
       r = (r < L) ? r + L : r - L;
@@ -374,6 +381,20 @@ void new_operator(const std::string &model, const std::string &name, const std::
                                                    : (g != m.end() && g->second == kv.second);
       if (!ok) { op.spin_asym = true; break; }
     }
+  }
+  // imaginary parts: an antisymmetric one-body operator of their own, applied
+  // to the other half of the split [Re | Im] vectors
+  bool any_im = false;
+  for (const auto &x : op.el) any_im = any_im || x.vi != 0.0;
+  if (any_im) {
+    auto im = std::make_shared<Operator>();
+    im->name = name + " (imaginary part)";
+    im->kind = OP_ONE_BODY;
+    im->spin_flip = op.spin_flip;
+    im->spin_asym = op.spin_asym;
+    for (const auto &x : op.el)
+      if (x.vi != 0.0) im->el.push_back({x.r, x.c, x.vi});
+    op.imag = im;
   }
   std::lock_guard<std::mutex> l(g_model_mu);
   if (!M.ops.count(name)) M.order.push_back(name);
@@ -1245,6 +1266,7 @@ bool terms_factorizable(const TermList &T) { return factorizable(T); }
 
 FactorizedHost factorized_tables(const TermList &T, const Sector &sec) {
   if (!factorizable(T)) fail(ED_ERR_UNSUPPORTED, "the instance's operators do not factorize by spin");
+  if (terms_complex(T)) fail(ED_ERR_UNSUPPORTED, "the instance's operators have complex elements");
   if (sec.Nup() < 0 || sec.Ndw() < 0 || sec.Nup() > T.L || sec.Ndw() > T.L || sec.Nup() >= Sector::odd ||
       sec.Ndw() >= Sector::odd)
     fail(ED_ERR_SECTOR, "sector " + sec.name() + " has no factorized basis");
@@ -1269,7 +1291,92 @@ FactorizedHost factorized_tables(const TermList &T, const Sector &sec) {
   return F;
 }
 
-std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format) {
+bool terms_complex(const TermList &T) {
+  for (const auto &tc : T.terms)
+    if (tc.first->is_complex && tc.second != 0.0) return true;
+  return false;
+}
+
+static std::unique_ptr<Hamiltonian> make_plan(const TermList &T, const Sector &sec, char format);
+
+// Complex Hilbert space (model_instance<Complex>, src_ed/qcm_ED.cpp:127-158) on
+// split vectors x = [x_r | x_i] of 2 n doubles: H = A + i B with A the real
+// parts of every term (symmetric) and B the imaginary parts of the one-body
+// terms (antisymmetric), so that
+//   H x = [A x_r - B x_i | B x_r + A x_i],
+// the real symmetric matrix [[A, -B], [B, A]]: its levels are those of H, each
+// twice (psi and i psi).  A is the plan of the requested format; B, which has
+// no diagonal, is factorized by spin (on the fly where a term flips spin).
+// Every Krylov solver runs on it unchanged.
+struct HamComplex : Hamiltonian {
+  std::unique_ptr<Hamiltonian> A, B;  // B null when no term has imaginary parts
+  size_t n = 0;
+  HamComplex(const TermList &T, const Sector &s, char fmt) {
+    A = make_plan(T, s, fmt);
+    TermList Ti;
+    Ti.L = T.L;
+    for (const auto &tc : T.terms)
+      if (tc.first->imag && tc.second != 0.0) Ti.terms.push_back({tc.first->imag.get(), tc.second});
+    if (!Ti.terms.empty()) {
+      if (fmt != 'N' && factorizable(Ti)) B.reset(new HamFactorized(Ti, s));
+      else B.reset(new HamOTF(Ti, s));
+      B->sec = s;
+      B->terms = Ti;
+    }
+    sec = s;
+    L = T.L;
+    Nup = s.Nup();
+    Ndw = s.Ndw();
+    format = fmt;
+    n = A->dim;
+    dim = 2 * n;
+  }
+  void run(const double *x, double *y, double ax, double beta, double *aux, size_t aux_cap, bool fixed) {
+    auto one = [&](Hamiltonian &P, const double *xx, double *yy, double a, double b) {
+      if (fixed) P.apply_fixed(xx, yy, a, b, nullptr, aux, aux_cap);
+      else P.apply(xx, yy, a, b, false);
+    };
+    one(*A, x, y, ax, beta);
+    if (B) one(*B, x + n, y, -ax, 1.0);
+    one(*A, x + n, y + n, ax, beta);
+    if (B) one(*B, x, y + n, ax, 1.0);
+  }
+  int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
+    run(x, y, ax, beta, nullptr, 0, false);
+    if (!want_dot) return 0;
+    double *part = take_partials();
+    const double *b[1] = {x};
+    const int nb = multi_dot(dim, y, 1, b, part, stream());
+    pending_nb = nb;
+    return nb;
+  }
+  int apply_fixed(const double *x, double *y, double ax, double beta, double *part, double *aux,
+                  size_t aux_cap) override {
+    run(x, y, ax, beta, aux, aux_cap, true);
+    if (!part) return 0;
+    const double *b[1] = {x};
+    return multi_dot(dim, y, 1, b, part, stream());
+  }
+  void prepare() override {
+    A->prepare();
+    if (B) B->prepare();
+  }
+  void diag(double *d) override {
+    A->diag(d);
+    A->diag(d + n);
+  }
+};
+
+std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format, bool force_complex) {
+  if (!force_complex && !terms_complex(T)) return make_plan(T, sec, format);
+  vlog("Hamiltonian %s: complex Hilbert space", sec.name().c_str());
+  std::unique_ptr<Hamiltonian> H(new HamComplex(T, sec, format));
+  H->sec = sec;
+  H->terms = T;
+  return H;
+}
+
+static std::unique_ptr<Hamiltonian> make_plan(const TermList &T, const Sector &sec, char format) {
   if (sec.Nup() >= Sector::odd || sec.Ndw() >= Sector::odd)
     fail(ED_ERR_UNSUPPORTED, "sector " + sec.name() + ": only sectors with conserved N and S are supported");
   if (sec.Nup() < 0 || sec.Ndw() < 0 || sec.Nup() > T.L || sec.Ndw() > T.L)
@@ -1510,6 +1617,59 @@ void tridiag_lowest(int n, const double *d, const double *e, double &ev0, double
 }
 
 // Householder tridiagonalisation (row-major working copy) + implicit QL
+// Hermitian eigensystem by cyclic Jacobi sweeps.  For the pair (p, q) with
+// a_pq = b e^{i phi}, U = diag(1, e^{-i phi}) R with the real plane rotation R
+// of angle theta = atan2(2 b, a_qq - a_pp) / 2 makes (U^H A U)_pq zero:
+//   columns  A_p <- c A_p - s e^{-i phi} A_q,  A_q <- s A_p + c e^{-i phi} A_q,
+//   rows     A_p <- c A_p - s e^{i phi} A_q,   A_q <- s A_p + c e^{i phi} A_q.
+void hermeig(int n, const cplx *Ain, double *w, cplx *V) {
+  if (n <= 0) return;
+  std::vector<cplx> a(Ain, Ain + (size_t)n * n), v((size_t)n * n, 0.0);
+  auto A = [&](int i, int j) -> cplx & { return a[i + (size_t)j * n]; };
+  for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+  double fro = 0.0;
+  for (const auto &x : a) fro += std::norm(x);
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += std::norm(A(p, q));
+    if (!(off > 1e-32 * fro)) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double b = std::abs(A(p, q));
+        if (b == 0.0) continue;
+        const cplx ph = A(p, q) / b;
+        const double th = 0.5 * std::atan2(2.0 * b, A(q, q).real() - A(p, p).real());
+        const double c = std::cos(th), s = std::sin(th);
+        const cplx em = std::conj(ph);  // e^{-i phi}
+        for (int k = 0; k < n; ++k) {
+          const cplx xp = A(k, p), xq = A(k, q);
+          A(k, p) = c * xp - s * em * xq;
+          A(k, q) = s * xp + c * em * xq;
+          const cplx vp = v[k + (size_t)p * n], vq = v[k + (size_t)q * n];
+          v[k + (size_t)p * n] = c * vp - s * em * vq;
+          v[k + (size_t)q * n] = s * vp + c * em * vq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const cplx xp = A(p, k), xq = A(q, k);
+          A(p, k) = c * xp - s * ph * xq;
+          A(q, k) = s * xp + c * ph * xq;
+        }
+        A(p, q) = A(q, p) = 0.0;
+        A(p, p) = A(p, p).real();
+        A(q, q) = A(q, q).real();
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A(x, x).real() < A(y, y).real(); });
+  for (int i = 0; i < n; ++i) {
+    w[i] = A(idx[i], idx[i]).real();
+    if (V)
+      for (int k = 0; k < n; ++k) V[k + (size_t)i * n] = v[k + (size_t)idx[i] * n];
+  }
+}
+
 void symeig(int n, const double *Ain, double *w, double *V) {
   if (n <= 0) return;
   std::vector<double> a((size_t)n * n);

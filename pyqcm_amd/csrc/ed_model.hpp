@@ -50,6 +50,7 @@ std::shared_ptr<const HalfBasis> half_basis(int L, int N);
 struct Element {
   int r, c;
   double v;
+  double vi = 0.0;  // imaginary part (operators defined with complex elements)
 };
 
 enum OpKind { OP_ONE_BODY = 0, OP_INTERACTION = 1 };
@@ -60,6 +61,12 @@ struct Operator {
   std::vector<Element> el;   // Hermitian-completed (one-body) / ordered (interaction)
   bool spin_flip = false;    // one-body elements coupling up and down
   bool spin_asym = false;    // up and down parts differ (mixing up_down)
+  // defined with complex elements (one_body_operator<Complex>): an instance
+  // with a nonzero coefficient on it works in a complex Hilbert space
+  bool is_complex = false;
+  // the imaginary parts as an operator of their own (elements v = Im, an
+  // antisymmetric one-body matrix); null when every element is real
+  std::shared_ptr<Operator> imag;
 };
 
 // ---------------------------------------------------------------------------
@@ -113,7 +120,7 @@ Model &get_model(const std::string &name);
 void new_model(const std::string &name, int n_sites, int n_bath,
                const std::vector<std::vector<int>> &generators = {}, bool bath_irrep = false);
 void new_operator(const std::string &model, const std::string &name, const std::string &type,
-                  const std::vector<Element> &el);
+                  const std::vector<Element> &el, bool is_complex = false);
 
 // (operator, coefficient) pairs of an instance
 struct TermList {
@@ -155,7 +162,12 @@ struct Hamiltonian {
   int pending_nb = 0;
 };
 
-std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format);
+// a complex Hilbert space is used when a term with a nonzero coefficient was
+// defined with complex elements, or when force_complex is set: the plan then
+// acts on split vectors [Re | Im] of twice the sector dimension
+bool terms_complex(const TermList &T);
+std::unique_ptr<Hamiltonian> make_hamiltonian(const TermList &T, const Sector &sec, char format,
+                                              bool force_complex = false);
 
 // Explicit host form of the factorized Hamiltonian of one sector: per-spin
 // merged symmetric one-body tables (CSR with values; both triangles) and the
@@ -189,6 +201,9 @@ void tridiag_eigvec0(int n, const double *d, const double *e, double &ev0, std::
                      double guess = std::numeric_limits<double>::quiet_NaN());
 // dense symmetric eigensystem, A column-major n x n, eigenvalues ascending
 void symeig(int n, const double *A, double *w, double *V /* may be null */);
+// dense Hermitian eigensystem (cyclic Jacobi), A column-major n x n,
+// eigenvalues ascending, eigenvectors in the columns of V (may be null)
+void hermeig(int n, const cplx *A, double *w, cplx *V);
 
 // Per-block partials of a reduction go to a slot of pinned host memory
 // (partials_acquire: up to 16 rows of partials_cap() doubles); sum_partials
