@@ -36,7 +36,7 @@ enum {
   ED_ERR_OOM = 5,            /* device or host allocation failed */
   ED_ERR_HIP = 6,            /* HIP runtime error */
   ED_ERR_NOT_FOUND = 7,      /* unknown model / instance label */
-  ED_ERR_UNSUPPORTED = 8,    /* feature this library does not implement (point groups, complex operators) */
+  ED_ERR_UNSUPPORTED = 8,    /* feature this library does not implement (see each entry point) */
   ED_ERR_INTERNAL = 9,
   ED_ERR_DEGENERATE = 10,    /* band Lanczos degenerate gap with no_degenerate_BL (Lanczos.hpp:368-370) */
   ED_ERR_CANCELLED = 11      /* ed_request_cancel was called (check_signals, src_python/common_Py.cpp:29-33) */
@@ -73,6 +73,19 @@ int ed_new_model(const char *name, int n_sites, int n_bath, const int *generator
 /* ED::new_operator, real elements (src_ed/qcm_ED.cpp:74-90); type "one-body" or "interaction". */
 int ed_new_operator(const char *model, const char *name, const char *type, int n,
                     const int *r, const int *c, const double *v);
+/* ED::new_operator, complex elements (src_ed/qcm_ED.cpp:95-106; one_body_operator<Complex>,
+   one_body_operator.hpp:59-76): type "one-body" only (others ED_ERR_UNSUPPORTED); v holds n
+   complex values as (re, im) pairs, elements r <= c, the lower triangle their conjugates
+   (an imaginary diagonal element is ED_ERR_INVALID: not Hermitian).  An instance with a
+   nonzero value on such an operator works in a complex Hilbert space (qcm_ED.cpp:145-157):
+   ground state, cluster averages, band-Lanczos Green function (GF_method 'L'), Q-matrix,
+   self-energy, hybridization, Potthoff functional; Davidson (n_states > 1), point groups,
+   GF_method 'F' / 'M', susceptibilities, fidelity, density matrices, wavefunction printing,
+   continued-fraction export / import, batched solves and the HDF5 format return
+   ED_ERR_UNSUPPORTED there.  Hamiltonian handles of such an instance act on split vectors
+   [Re | Im] of twice the sector dimension. */
+int ed_new_operator_complex(const char *model, const char *name, const char *type, int n,
+                            const int *r, const int *c, const double *v);
 /* ED::new_model_instance (src_ed/qcm_ED.cpp:125-158); sectors is slash-separated. */
 int ed_new_model_instance(const char *model, int n_values, const char *const *names, const double *values,
                           const char *sectors, int label);
@@ -101,7 +114,8 @@ int ed_ground_state_energy(int label, double *E0);
    (same order as ed_instance_parameters), the ground-state average and variance. */
 int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n);
 /* ED::qmatrix (src_python/qcm_ED_wrap.hpp:339-349): the Lehmann representation of the
-   first state's Green function (band-Lanczos method); e[M], v[L x M] column-major. */
+   first state's Green function (band-Lanczos method); e[M], v[L x M] complex column-major
+   ((re, im) pairs), G(r, c) = sum_i v(c, i) conj(v(r, i)) / (z - e_i). */
 int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, int *M);
 /* number of low-energy states and their (energy, weight, sector) */
 int ed_states(int label, int cap, double *energy, double *weight, int *n);
@@ -240,7 +254,8 @@ int ed_potthoff_functional(int label, double *omega);
    order (names[k * name_len], values[k], k < min(*n, cap)) and its model's name. */
 int ed_instance_parameters(int label, int cap, int name_len, char *names, double *values, int *n, char *model,
                            int model_cap);
-/* ED::complex_HS (src_ed/qcm_ED.hpp:94): always 0 here (real operators only) */
+/* ED::complex_HS (src_ed/qcm_ED.hpp:94): 1 when the instance works in a complex Hilbert
+   space (a nonzero value on an operator defined with ed_new_operator_complex) */
 int ed_complex_hs(int label, int *is_complex);
 /* ED::hopping_matrix / hopping_matrix_full (src_ed/qcm_ED.cpp:269-284;
    model_instance.cpp:13-16, model_instance.hpp:1491-1522): the one-body matrix of the
@@ -257,7 +272,8 @@ int ed_interactions(int label, int cap, int *r, int *c, double *v, int *n);
    e[n_bath] and the cluster-bath matrix tcb (n_sites x n_bath complex column-major). */
 int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb);
 /* ED::matrix_elements (src_ed/qcm_ED.cpp:442-450): the operator's type ("one_body" or
-   "interaction") and its stored elements (0-based; one-body ones Hermitian-completed). */
+   "interaction") and its stored elements (0-based; one-body ones Hermitian-completed),
+   values complex: v[2 k], v[2 k + 1] = (re, im) of element k. */
 int ed_matrix_elements(const char *model, const char *op, char *type, int type_cap, int cap, int *r, int *c,
                        double *v, int *n);
 /* ED::density_matrix (src_ed/qcm_ED.cpp:205-216; density_matrix_mixed,

@@ -63,6 +63,7 @@ SIGNATURES = {
     "ed_get_global_int": (C.c_int, [C.c_char_p, C.POINTER(C.c_long)]),
     "ed_new_model": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _ip, C.c_int, C.c_int]),
     "ed_new_operator": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _ip, _ip, _dp]),
+    "ed_new_operator_complex": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _ip, _ip, _dp]),
     "ed_new_model_instance": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]),
     "ed_erase_model_instance": (C.c_int, [C.c_int]),
     "ed_ground_state_solve": (C.c_int, [C.c_int, _dp, C.c_char_p, C.c_int]),

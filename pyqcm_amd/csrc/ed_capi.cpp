@@ -158,6 +158,17 @@ int ed_new_operator(const char *model, const char *name, const char *type, int n
   });
 }
 
+int ed_new_operator_complex(const char *model, const char *name, const char *type, int n, const int *r, const int *c,
+                            const double *v) {
+  return guard([&] {
+    need(n >= 0, "negative element count");
+    need(n == 0 || (r && c && v), "ed_new_operator_complex: null element arrays");
+    std::vector<Element> el(n);
+    for (int k = 0; k < n; ++k) el[k] = {r[k], c[k], v[2 * k], v[2 * k + 1]};
+    new_operator(model, name, type, el, true);
+  });
+}
+
 int ed_new_model_instance(const char *model, int n_values, const char *const *names, const double *values,
                           const char *sectors, int label) {
   return guard([&] {
@@ -272,7 +283,10 @@ int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, 
     *M = (int)G.e.size();
     if (*M <= cap) {
       std::memcpy(e, G.e.data(), sizeof(double) * G.e.size());
-      std::memcpy(v, G.V.data(), sizeof(double) * G.V.size());
+      for (size_t k = 0; k < G.V.size(); ++k) {
+        v[2 * k] = G.V[k];
+        v[2 * k + 1] = G.Vi.empty() ? 0.0 : G.Vi[k];
+      }
     }
   });
 }
@@ -418,8 +432,7 @@ int ed_instance_parameters(int label, int cap, int name_len, char *names, double
 int ed_complex_hs(int label, int *is_complex) {
   return guard([&] {
     need(is_complex != nullptr, "ed_complex_hs: null output");
-    get_instance(label);
-    *is_complex = 0;  // complex operators are rejected at new_operator
+    *is_complex = get_instance(label).complex_hs() ? 1 : 0;
   });
 }
 
@@ -430,21 +443,31 @@ int ed_hopping_matrix(int label, int spin_down, int diag, int full, double *out)
     const int ns = I.model->n_sites, nb = I.model->n_bath;
     const Instance::Hopping H = I.hopping_blocks(spin_down != 0, full && diag);
     const int d = full ? ns + nb : ns;
-    std::vector<double> m((size_t)d * d, 0.0);
+    const bool cx = !H.tci.empty();
+    auto im = [cx](const std::vector<double> &x, size_t k) { return cx ? x[k] : 0.0; };
+    std::vector<cplx> m((size_t)d * d, 0.0);
     for (int a = 0; a < ns; ++a)
-      for (int b = 0; b < ns; ++b) m[a + (size_t)b * d] = H.tc[a + (size_t)b * ns];
+      for (int b = 0; b < ns; ++b) {
+        const size_t k = a + (size_t)b * ns;
+        m[a + (size_t)b * d] = cplx(H.tc[k], im(H.tci, k));
+      }
     if (full) {
       for (int i = 0; i < nb; ++i)
-        for (int j = 0; j < nb; ++j) m[ns + i + (size_t)(ns + j) * d] = H.tb[i + (size_t)j * nb];
+        for (int j = 0; j < nb; ++j) {
+          const size_t k = i + (size_t)j * nb;
+          m[ns + i + (size_t)(ns + j) * d] = cplx(H.tb[k], im(H.tbi, k));
+        }
       for (int a = 0; a < ns; ++a)
         for (int i = 0; i < nb; ++i) {
-          m[a + (size_t)(ns + i) * d] = H.tcb[a + (size_t)i * ns];
-          m[ns + i + (size_t)a * d] = H.tcb[a + (size_t)i * ns];
+          const size_t k = a + (size_t)i * ns;
+          const cplx t(H.tcb[k], im(H.tcbi, k));
+          m[a + (size_t)(ns + i) * d] = t;
+          m[ns + i + (size_t)a * d] = std::conj(t);
         }
     }
     for (size_t k = 0; k < m.size(); ++k) {
-      out[2 * k] = m[k];
-      out[2 * k + 1] = 0.0;
+      out[2 * k] = m[k].real();
+      out[2 * k + 1] = m[k].imag();
     }
   });
 }
@@ -480,7 +503,7 @@ int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb) {
     for (int i = 0; i < nb; ++i) e[i] = H.tb[i + (size_t)i * nb];
     for (size_t k = 0; k < (size_t)ns * nb; ++k) {
       tcb[2 * k] = H.tcb[k];
-      tcb[2 * k + 1] = 0.0;
+      tcb[2 * k + 1] = H.tcbi.empty() ? 0.0 : H.tcbi[k];
     }
   });
 }
@@ -503,7 +526,8 @@ int ed_matrix_elements(const char *model, const char *op, char *type, int type_c
     for (int k = 0; k < *n && k < cap; ++k) {
       r[k] = el[k].r;
       c[k] = el[k].c;
-      v[k] = el[k].v;
+      v[2 * k] = el[k].v;
+      v[2 * k + 1] = el[k].vi;
     }
   });
 }
@@ -568,10 +592,10 @@ int ed_green_function_average(int label, int spin_down, double *out) {
   return guard([&] {
     need(out != nullptr, "ed_green_function_average: null output");
     Instance &I = get_instance(label);
-    const std::vector<double> G = I.gf_average(spin_down != 0);
+    const std::vector<cplx> G = I.gf_average(spin_down != 0);
     for (size_t k = 0; k < G.size(); ++k) {
-      out[2 * k] = G[k];
-      out[2 * k + 1] = 0.0;
+      out[2 * k] = G[k].real();
+      out[2 * k + 1] = G[k].imag();
     }
   });
 }

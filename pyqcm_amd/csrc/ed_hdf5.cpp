@@ -459,6 +459,7 @@ std::string hdf5_version() {
 // ED::write_instance_hdf5 + model_instance<T>::write_hdf5 + state<T>::write_hdf5
 void write_instance_hdf5(const std::string &filename, int label, const std::string &group) {
   Instance &I = get_instance(label);
+  I.need_real("the HDF5 solution format");
   h5();
   if (!I.gf_solved) I.green_function_solve();
   Id file = file_open(filename, true);
@@ -498,6 +499,7 @@ void write_instance_hdf5(const std::string &filename, int label, const std::stri
 // the format holds none) and counts as solved
 void read_instance_hdf5(const std::string &filename, int label, const std::string &group) {
   Instance &I = get_instance(label);
+  I.need_real("the HDF5 solution format");
   h5();
   Id file = file_open(filename, false);
   Id grp = group_open(file, group);

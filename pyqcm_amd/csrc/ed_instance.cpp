@@ -1004,6 +1004,7 @@ void Instance::green_function_solve() {
 // ---------------------------------------------------------------------------
 void cf_export(Instance &I, int spin, int state, std::vector<std::vector<double>> &a,
                std::vector<std::vector<double>> &b) {
+  I.need_real("continued-fraction export");
   if (!I.gf_solved) I.green_function_solve();
   if (state < 0 || state >= (int)I.states.size()) fail(ED_ERR_INVALID, "cf_export: no such low-energy state");
   const GFRep &G = I.states[state]->gf[I.gf_spin(spin == 1)];
@@ -1039,6 +1040,7 @@ void cf_export(Instance &I, int spin, int state, std::vector<std::vector<double>
 
 void cf_pairs(Instance &I, int spin, const std::vector<std::pair<int, int>> &pairs, int cap,
               std::vector<std::vector<double>> &a, std::vector<std::vector<double>> &b) {
+  I.need_real("continued fractions of orbital pairs");
   if (!I.gs_solved) I.ground_state_solve();
   if (I.states.empty() || !I.states[0]->psi.p) fail(ED_ERR_INVALID, "cf_pairs needs the ground-state vector");
   const int L = I.model->L, ns = I.model->n_sites;
@@ -1093,6 +1095,7 @@ void cf_pairs(Instance &I, int spin, const std::vector<std::pair<int, int>> &pai
 
 void cf_import(Instance &I, double E0, const Sector &sec, int spin, const std::vector<std::vector<double>> &a,
                const std::vector<std::vector<double>> &b) {
+  I.need_real("continued-fraction import");
   const int n = I.gf_dim(), np = n * (n + 1) / 2;
   if ((int)a.size() != 2 * np || (int)b.size() != 2 * np)
     fail(ED_ERR_INVALID, "cf_import: expected " + std::to_string(2 * np) + " fractions");
@@ -1300,10 +1303,11 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
     if (G.kind == 'L') {
       const int M = (int)G.e.size();
       if (!M) continue;
-      DevBuf<double> de, dV, dG((size_t)nz * n * n * 2);
+      DevBuf<double> de, dV, dVi, dG((size_t)nz * n * n * 2);
       de.upload(G.e);
       dV.upload(G.V);
-      vdvh(n, M, de.p, dV.p, nz, dz.p, dG.p, stream());
+      if (!G.Vi.empty()) dVi.upload(G.Vi);
+      vdvh(n, M, de.p, dV.p, nz, dz.p, dG.p, stream(), G.Vi.empty() ? nullptr : dVi.p);
       std::vector<double> h = dG.to_host();
       for (size_t k = 0; k < h.size(); ++k) acc[k] += h[k];
     } else if (G.kind == 'F') {
@@ -1348,6 +1352,20 @@ std::vector<double> Instance::one_body_matrix(bool spin_down) const {
   return h;
 }
 
+std::vector<cplx> Instance::one_body_matrix_c(bool spin_down) const {
+  const int L = model->L, off = spin_down ? L : 0;
+  std::vector<cplx> h((size_t)L * L, 0.0);
+  for (const auto &kv : value) {
+    const Operator &op = model->ops.at(kv.first);
+    if (op.kind != OP_ONE_BODY) continue;
+    for (const auto &x : op.el) {
+      if ((x.r >= L) != spin_down || (x.c >= L) != spin_down) continue;
+      h[(size_t)(x.r - off) * L + (x.c - off)] += kv.second * cplx(x.v, x.vi);
+    }
+  }
+  return h;
+}
+
 // bath part of ED::Potthoff_functional (model_instance::set_hopping_matrix,
 // src_ed/model_instance.hpp:447-473): minus the negative eigenvalues of the
 // bath one-body block, both spins (twice the up block without spin mixing)
@@ -1355,12 +1373,22 @@ double Instance::sef_bath() const {
   const int L = model->L, ns = model->n_sites, nb = model->n_bath;
   if (!nb) return 0.0;
   double s = 0.0;
+  const bool cx = complex_hs();
   for (int spin = 0; spin < ((mixing & 4) ? 2 : 1); ++spin) {
-    const std::vector<double> h = one_body_matrix(spin == 1);
-    std::vector<double> tb((size_t)nb * nb), eb(nb);
-    for (int i = 0; i < nb; ++i)
-      for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
-    symeig(nb, tb.data(), eb.data(), nullptr);
+    std::vector<double> eb(nb);
+    if (cx) {
+      const std::vector<cplx> h = one_body_matrix_c(spin == 1);
+      std::vector<cplx> tb((size_t)nb * nb);
+      for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+      hermeig(nb, tb.data(), eb.data(), nullptr);
+    } else {
+      const std::vector<double> h = one_body_matrix(spin == 1);
+      std::vector<double> tb((size_t)nb * nb);
+      for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+      symeig(nb, tb.data(), eb.data(), nullptr);
+    }
     for (int i = 0; i < nb; ++i)
       if (eb[i] < 0.0) s -= eb[i];
   }
@@ -1372,9 +1400,39 @@ double Instance::sef_bath() const {
 void Instance::hybridization(int nz, const double *z, bool spin_down, double *out) {
   const int L = model->L, ns = model->n_sites, nb = model->n_bath;
   const bool sd = spin_down && (mixing & 4);
-  std::vector<double> h = one_body_matrix(sd);
   std::memset(out, 0, sizeof(double) * 2 * (size_t)nz * ns * ns);
   if (!nb) return;
+  if (complex_hs()) {
+    // Gamma = tcb (z - tb)^-1 tcb^H with tb = S diag(e) S^H (Hermitian):
+    // Gamma_ab = sum_i T_ai conj(T_bi) / (z - e_i), T = tcb S
+    const std::vector<cplx> h = one_body_matrix_c(sd);
+    std::vector<cplx> tb((size_t)nb * nb), S((size_t)nb * nb), T((size_t)ns * nb, 0.0);
+    std::vector<double> eb(nb);
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+    hermeig(nb, tb.data(), eb.data(), S.data());
+    for (int a = 0; a < ns; ++a)
+      for (int i = 0; i < nb; ++i) {
+        cplx t = 0.0;
+        for (int k = 0; k < nb; ++k) t += h[(size_t)a * L + ns + k] * S[k + (size_t)i * nb];
+        T[a + (size_t)i * ns] = t;
+      }
+    for (int iz = 0; iz < nz; ++iz) {
+      const cplx w(z[2 * iz], z[2 * iz + 1]);
+      double *g = out + (size_t)iz * ns * ns * 2;
+      for (int i = 0; i < nb; ++i) {
+        const cplx f = 1.0 / (w - eb[i]);
+        for (int a = 0; a < ns; ++a)
+          for (int b = 0; b < ns; ++b) {
+            const cplx v = T[a + (size_t)i * ns] * std::conj(T[b + (size_t)i * ns]) * f;
+            g[2 * (a + (size_t)b * ns)] += v.real();
+            g[2 * (a + (size_t)b * ns) + 1] += v.imag();
+          }
+      }
+    }
+    return;
+  }
+  std::vector<double> h = one_body_matrix(sd);
   std::vector<double> tb((size_t)nb * nb), eb(nb), Sb((size_t)nb * nb);
   for (int i = 0; i < nb; ++i)
     for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
@@ -1436,7 +1494,7 @@ void Instance::self_energy(int nz, const double *z, bool spin_down, double *out)
   green_function(nz, z, spin_down, G.data());
   hybridization(nz, z, spin_down, Gam.data());
   const bool sd = spin_down && (mixing & 4);
-  std::vector<double> h = one_body_matrix(sd);
+  const std::vector<cplx> h = one_body_matrix_c(sd);
   for (int iz = 0; iz < nz; ++iz) {
     std::vector<cplx> g((size_t)n * n);
     for (size_t k = 0; k < (size_t)n * n; ++k)
@@ -1456,6 +1514,49 @@ void Instance::self_energy(int nz, const double *z, bool spin_down, double *out)
 
 Instance::Hopping Instance::hopping_blocks(bool spin_down, bool diag) const {
   const int L = model->L, ns = model->n_sites, nb = model->n_bath;
+  if (complex_hs()) {
+    // complex blocks; a non-diagonal bath block is diagonalized as a
+    // Hermitian matrix (tcb -> tcb S)
+    const std::vector<cplx> h = one_body_matrix_c(spin_down && (mixing & 4));
+    std::vector<cplx> tc((size_t)ns * ns), tb((size_t)nb * nb), tcb((size_t)ns * nb);
+    for (int a = 0; a < ns; ++a)
+      for (int b = 0; b < ns; ++b) tc[a + (size_t)b * ns] = h[(size_t)a * L + b];
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) tb[i + (size_t)j * nb] = h[(size_t)(ns + i) * L + ns + j];
+    for (int a = 0; a < ns; ++a)
+      for (int i = 0; i < nb; ++i) tcb[a + (size_t)i * ns] = h[(size_t)a * L + ns + i];
+    bool is_diag = true;
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j)
+        if (i != j && tb[i + (size_t)j * nb] != 0.0) is_diag = false;
+    if (diag && nb > 0 && !is_diag) {
+      std::vector<double> eb(nb);
+      std::vector<cplx> S((size_t)nb * nb), t((size_t)ns * nb, 0.0);
+      hermeig(nb, tb.data(), eb.data(), S.data());
+      for (int a = 0; a < ns; ++a)
+        for (int i = 0; i < nb; ++i) {
+          cplx x = 0.0;
+          for (int k = 0; k < nb; ++k) x += tcb[a + (size_t)k * ns] * S[k + (size_t)i * nb];
+          t[a + (size_t)i * ns] = x;
+        }
+      tcb.swap(t);
+      tb.assign((size_t)nb * nb, 0.0);
+      for (int i = 0; i < nb; ++i) tb[i + (size_t)i * nb] = eb[i];
+    }
+    Hopping H;
+    auto split = [](const std::vector<cplx> &m, std::vector<double> &re, std::vector<double> &im) {
+      re.resize(m.size());
+      im.resize(m.size());
+      for (size_t k = 0; k < m.size(); ++k) {
+        re[k] = m[k].real();
+        im[k] = m[k].imag();
+      }
+    };
+    split(tc, H.tc, H.tci);
+    split(tb, H.tb, H.tbi);
+    split(tcb, H.tcb, H.tcbi);
+    return H;
+  }
   const std::vector<double> h = one_body_matrix(spin_down && (mixing & 4));  // row-major L x L
   Hopping H;
   H.tc.assign((size_t)ns * ns, 0.0);
@@ -1489,6 +1590,7 @@ Instance::Hopping Instance::hopping_blocks(bool spin_down, bool diag) const {
 }
 
 void Instance::density_matrix(const std::vector<int> &sites, std::vector<double> &rho, std::vector<uint64_t> &basis) {
+  need_real("the reduced density matrix");
   if (!gs_solved) ground_state_solve();
   if (states.empty() || !states[0]->psi.p)
     fail(ED_ERR_INVALID, "the density matrix needs the ground-state vector (not read from a file)");
@@ -1549,6 +1651,7 @@ static void print_binary_double(std::ostream &o, uint64_t w, int m) {
 }
 
 std::string Instance::print_wavefunction() {
+  need_real("print_wavefunction");
   if (!gs_solved) ground_state_solve();
   const int L = model->L;
   const long max_print = params().geti("max_dim_print");
@@ -1575,6 +1678,8 @@ std::string Instance::print_wavefunction() {
 }
 
 double Instance::fidelity(Instance &other) {
+  need_real("fidelity");
+  other.need_real("fidelity");
   if (!gs_solved) ground_state_solve();
   if (!other.gs_solved) other.ground_state_solve();
   double z = 0.0;
@@ -1626,20 +1731,23 @@ static std::vector<double> hole_b0(const GFRep &R, int n) {
 //    the diagonals of the earlier states as well, as the reference does);
 //  * mcf_set::integrated_Green_function (src_ed/mcf_set.hpp:146-158): the
 //    hole weights' W_h^T W_h.
-std::vector<double> Instance::gf_average(bool spin_down) {
+std::vector<cplx> Instance::gf_average(bool spin_down) {
   if (!gf_solved) green_function_solve();
   const int spin = gf_spin(spin_down);
   const int n = gf_dim();
-  std::vector<double> G((size_t)n * n, 0.0);
-  auto at = [&](int a, int b) -> double & { return G[a + (size_t)b * n]; };
+  std::vector<cplx> G((size_t)n * n, 0.0);
+  auto at = [&](int a, int b) -> cplx & { return G[a + (size_t)b * n]; };
   for (auto &s : states) {
     const GFRep &R = s->gf[spin];
     if (R.kind == 'L') {
+      // G(b, a) += V(a, i) conj(V(b, i)) over the poles below zero
       for (size_t i = 0; i < R.e.size(); ++i) {
         if (R.e[i] >= 0.0) continue;
         const double *v = R.V.data() + i * n;
+        const double *vi = R.Vi.empty() ? nullptr : R.Vi.data() + i * n;
         for (int a = 0; a < n; ++a)
-          for (int b = 0; b < n; ++b) at(b, a) += v[a] * v[b];
+          for (int b = 0; b < n; ++b)
+            at(b, a) += vi ? cplx(v[a], vi[a]) * cplx(v[b], -vi[b]) : cplx(v[a] * v[b], 0.0);
       }
     } else if (R.kind == 'F') {
       const std::vector<double> b0 = hole_b0(R, n);
@@ -1647,7 +1755,7 @@ std::vector<double> Instance::gf_average(bool spin_down) {
       for (int o1 = 0; o1 < n; ++o1) at(o1, o1) += b0[q(o1, o1)];
       for (int o1 = 0; o1 < n; ++o1)
         for (int o2 = 0; o2 < o1; ++o2) {
-          const double t = 0.5 * (b0[q(o1, o2)] - at(o1, o1) - at(o2, o2));
+          const cplx t = 0.5 * (b0[q(o1, o2)] - at(o1, o1) - at(o2, o2));
           at(o1, o2) += t;
           at(o2, o1) += t;
         }
@@ -1672,8 +1780,8 @@ double Instance::gf_density() {
   double nG = 0.0;
   const int n = gf_dim();
   for (int sd = 0; sd < ((mixing & 4) ? 2 : 1); ++sd) {
-    const std::vector<double> G = gf_average(sd == 1);
-    for (int i = 0; i < n; ++i) nG += G[i + (size_t)i * n];
+    const std::vector<cplx> G = gf_average(sd == 1);
+    for (int i = 0; i < n; ++i) nG += G[i + (size_t)i * n].real();
   }
   nG /= model->n_sites;
   return mixing == 0 ? 2.0 * nG : nG;
@@ -1718,6 +1826,7 @@ double Instance::tr_sigma_inf() {
 // the Hamiltonian of that sector from O |Omega>, poles shifted by -E_Omega.
 // O is the operator as defined (its instance value is not applied).
 std::vector<Instance::ChiPart> Instance::susceptibility_parts(const std::string &op) {
+  need_real("susceptibility");
   if (!model->ops.count(op)) fail(ED_ERR_INVALID, "Operator " + op + " is not defined in the model.");
   if (!value.count(op))
     fail(ED_ERR_INVALID, "operator " + op + " is not activated in instance " + std::to_string(label));

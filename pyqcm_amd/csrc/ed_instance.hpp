@@ -97,7 +97,7 @@ struct Instance {
   double sef_bath() const;
   // frequency integral of the Green function of the spin block, n x n
   // column-major (real Hamiltonians: real and symmetric)
-  std::vector<double> gf_average(bool spin_down);
+  std::vector<cplx> gf_average(bool spin_down);
   // electrons per site from the diagonal of gf_average
   double gf_density();
   // trace of the self-energy at infinite frequency, both spins

@@ -1009,7 +1009,10 @@ __global__ void k_cf_eval(int ncf, const int *off, const double *a, const double
   out[2 * ((size_t)iz * ncf + f) + 1] = G.im;
 }
 
-__global__ void k_vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G) {
+// G[b + a L] = sum_i V(a, i) conj(V(b, i)) / (z - e_i); Vi (imaginary parts
+// of V) may be null
+__global__ void k_vdvh(int L, int M, const double *e, const double *V, const double *Vi, int nz, const double *z,
+                       double *G) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int LL = L * L;
   if (t >= nz * LL) return;
@@ -1017,12 +1020,24 @@ __global__ void k_vdvh(int L, int M, const double *e, const double *V, int nz, c
   const int bb = ab % L, aa = ab / L;  // G[b + a L]
   const double zr = z[2 * iz], zi = z[2 * iz + 1];
   double sr = 0.0, si = 0.0;
-  for (int i = 0; i < M; ++i) {
-    const double tr = zr - e[i], ti = zi;
-    const double inv = 1.0 / (tr * tr + ti * ti);
-    const double w = V[aa + (size_t)i * L] * V[bb + (size_t)i * L];
-    sr += w * tr * inv;
-    si -= w * ti * inv;
+  if (!Vi) {
+    for (int i = 0; i < M; ++i) {
+      const double tr = zr - e[i], ti = zi;
+      const double inv = 1.0 / (tr * tr + ti * ti);
+      const double w = V[aa + (size_t)i * L] * V[bb + (size_t)i * L];
+      sr += w * tr * inv;
+      si -= w * ti * inv;
+    }
+  } else {
+    for (int i = 0; i < M; ++i) {
+      const double tr = zr - e[i], ti = zi;
+      const double inv = 1.0 / (tr * tr + ti * ti);
+      const double ar = V[aa + (size_t)i * L], ai = Vi[aa + (size_t)i * L];
+      const double br = V[bb + (size_t)i * L], bi = Vi[bb + (size_t)i * L];
+      const double wr = ar * br + ai * bi, wi = ai * br - ar * bi;  // V(a) conj(V(b))
+      sr += (wr * tr + wi * ti) * inv;
+      si += (wi * tr - wr * ti) * inv;
+    }
   }
   G[2 * ((size_t)iz * LL + ab)] = sr;
   G[2 * ((size_t)iz * LL + ab) + 1] = si;
@@ -1811,10 +1826,11 @@ void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg,
   ED_HIP(hipGetLastError());
 }
 
-void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s) {
+void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s,
+          const double *Vi) {
   const int n = nz * L * L;
   if (!n) return;
-  hipLaunchKernelGGL(k_vdvh, dim3((n + 255) / 256), dim3(256), 0, s, L, M, e, V, nz, z, G);
+  hipLaunchKernelGGL(k_vdvh, dim3((n + 255) / 256), dim3(256), 0, s, L, M, e, V, Vi, nz, z, G);
   ED_HIP(hipGetLastError());
 }
 

@@ -239,8 +239,10 @@ void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, 
 // G + at[g] (doubles) as nz column-major n[g] x n[g] complex matrices
 void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, int ncf, int nz,
                  const double *vals, double *G, hipStream_t s);
-// Lehmann sum per z: G[iz][b + a L] = sum_i V[a,i] V[b,i] / (z - e_i)
-void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s);
+// G[b + a L] (nz blocks) = sum_i V(a, i) conj(V(b, i)) / (z - e_i), V real or
+// with imaginary parts Vi
+void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s,
+          const double *Vi = nullptr);
 // matrix continued fraction per z (p <= 16)
 void mcf_eval(int p, int nfl, const double *A, const double *B, const double *W, int q, int nz, const double *z,
               double *G, hipStream_t s);

@@ -303,18 +303,36 @@ def parametersC(label=0):
 
 
 def new_operator_complex(model, op, type, elements):
-    """qcm.new_operator_complex (src_python/qcm_ED_wrap.hpp:219-261): complex
-    Hilbert spaces are not supported here; elements whose values are all real
-    define the same operator as new_operator, anything else raises EDError
-    (code 8)."""
-    el = [(e[0], e[1], complex(e[2])) for e in elements]
-    if any(v.imag != 0.0 for (_, _, v) in el):
-        raise EDError(8, "operator %s has complex matrix elements: complex operators are not supported" % op)
-    new_operator(model, op, type, [(r, c, v.real) for (r, c, v) in el])
+    """qcm.new_operator_complex (src_python/qcm_ED_wrap.hpp:219-261): elements
+    are 1-based (r, c, v) with r <= c and v complex; the lower triangle holds
+    their conjugates.  An instance with a nonzero value on the operator works
+    in a complex Hilbert space."""
+    rr, cc, vv = [], [], []
+    fac = 0.5 if type == "anomalous" else 1.0
+    for i, e in enumerate(elements):
+        if len(e) != 3:
+            raise EDError(1, "element %d of argument 4 of 'new_operator' should be a 3-tuple" % i)
+        r, c, v = int(e[0]), int(e[1]), complex(e[2])
+        if r > c:
+            raise EDError(1, "the first index of element %d of argument 4 of 'new_operator' cannot be bigger "
+                             "than the second index" % i)
+        if r == 0 or c == 0:
+            raise EDError(1, "indices in matrix elements of operators are labelled starting at 1, not at 0.")
+        rr.append(r - 1)
+        cc.append(c - 1)
+        vv.extend([fac * v.real, fac * v.imag])
+    n = len(rr)
+    ra = (C.c_int * max(n, 1))(*rr)
+    ca = (C.c_int * max(n, 1))(*cc)
+    va = (C.c_double * max(2 * n, 1))(*vv)
+    check(lib().ed_new_operator_complex(model.encode(), op.encode(), type.encode(), n, ra, ca, va))
+    if model in _models and op not in _models[model][2]:
+        _models[model][2].append(op)
 
 
 def ED_complex_HS(label=0):
-    """qcm.ED_complex_HS (src_python/qcm_ED_wrap.hpp:27-30): 0 (real Hilbert space)."""
+    """qcm.ED_complex_HS (src_python/qcm_ED_wrap.hpp:27-30): 1 in a complex
+    Hilbert space, else 0."""
     x = C.c_int()
     check(lib().ed_complex_hs(int(label), C.byref(x)))
     return x.value
@@ -369,7 +387,7 @@ def matrix_elements(model, op):
     tb = C.create_string_buffer(32)
     check(lib().ed_matrix_elements(model.encode(), op.encode(), tb, 32, 0, None, None, None, C.byref(n)))
     m = max(n.value, 1)
-    r, c, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m)
+    r, c, v = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m, np.complex128)
     check(lib().ed_matrix_elements(model.encode(), op.encode(), tb, 32, n.value, r.ctypes.data_as(C.POINTER(C.c_int)),
                                    c.ctypes.data_as(C.POINTER(C.c_int)), v.ctypes.data_as(C.POINTER(C.c_double)),
                                    C.byref(n)))
@@ -485,14 +503,20 @@ def cluster_averages(label=0):
 
 
 def qmatrix(label=0, spin_down=0):
-    """Lehmann representation (poles e[M], residues Q[L, M]) of the first state."""
+    """Lehmann representation (poles e[M], residues Q[L, M]) of the first state
+    (ED::qmatrix, src_ed/qcm_ED.cpp:406-416): G(r, c) = sum_i Q[c, i]
+    conj(Q[r, i]) / (z - e[i]).  Q is real in a real Hilbert space and complex
+    in a complex one."""
     L_, M_ = C.c_int(), C.c_int()
     check(lib().ed_qmatrix(int(label), int(spin_down), 0, None, None, C.byref(L_), C.byref(M_)))
     e = np.zeros(max(M_.value, 1))
-    v = np.zeros(max(M_.value * L_.value, 1))
+    v = np.zeros(max(M_.value * L_.value, 1), dtype=np.complex128)
     check(lib().ed_qmatrix(int(label), int(spin_down), M_.value, e.ctypes.data_as(C.POINTER(C.c_double)),
                            v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(L_), C.byref(M_)))
-    return e[:M_.value], v[:M_.value * L_.value].reshape(M_.value, L_.value).T.copy()
+    Q = v[:M_.value * L_.value].reshape(M_.value, L_.value).T.copy()
+    if not ED_complex_HS(label):
+        Q = Q.real.copy()
+    return e[:M_.value], Q
 
 
 def combined_mcf(label=0, spin_down=0):

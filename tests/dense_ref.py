@@ -36,11 +36,14 @@ def _create(word, a):
 
 def dense_hamiltonian(L, terms, Nup, Ndw):
     """terms: [(type, value, [(r, c, v), ...])] with the reference's element
-    conventions (one-body Hermitian-completed from r <= c; interaction n_r n_c)."""
+    conventions (one-body Hermitian-completed from r <= c, the lower triangle
+    the conjugate of the upper one; interaction n_r n_c).  Complex values give
+    a complex matrix."""
     states = sector_states(L, Nup, Ndw)
     index = {s: i for i, s in enumerate(states)}
     n = len(states)
-    H = np.zeros((n, n))
+    cx = any(isinstance(v, complex) and v.imag != 0.0 for (_, _, els) in terms for (_, _, v) in els)
+    H = np.zeros((n, n), dtype=np.complex128 if cx else np.float64)
     for (typ, val, els) in terms:
         typ = {"one-body": 0, "interaction": 1}.get(typ, typ)
         for (r, c, v) in els:
@@ -49,7 +52,7 @@ def dense_hamiltonian(L, terms, Nup, Ndw):
                     if (s >> r) & 1 and (s >> c) & 1:
                         H[i, i] += val * v
                 continue
-            pairs = [(r, c, v)] if r == c else ([(r, c, v), (c, r, v)] if r < c else [])
+            pairs = [(r, c, v)] if r == c else ([(r, c, v), (c, r, np.conj(v))] if r < c else [])
             for (a, b, w) in pairs:
                 for i, s in enumerate(states):
                     s1, g1 = _annihilate(s, b)

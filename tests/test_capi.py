@@ -206,18 +206,39 @@ def test_hopping_matrix_hybridization_interactions_matrix_elements():
 
 def test_parameters_and_complex_operator_entry():
     """parametersC returns the instance's parameters and model name;
-    new_operator_complex takes real-valued elements and refuses complex ones
-    (code 8: complex Hilbert spaces are not supported)"""
+    new_operator_complex completes the lower triangle with the conjugates
+    (one_body_operator.hpp:67-76), refuses an imaginary diagonal (not
+    Hermitian, code 1) and other operator types (code 8); an instance with a
+    nonzero value on a complex operator works in a complex Hilbert space, and
+    its one-body matrices carry the imaginary parts"""
     qcm.new_model("capi_par", 2, 0)
     qcm.new_operator("capi_par", "t", "one-body", [(1, 2, -1.0), (3, 4, -1.0)])
     qcm.new_operator_complex("capi_par", "h", "one-body", [(1, 1, 1.0 + 0j), (3, 3, -1.0 + 0j)])
     assert qcm.matrix_elements("capi_par", "h") == ("one_body", [(0, 0, 1 + 0j), (2, 2, -1 + 0j)])
+    qcm.new_operator_complex("capi_par", "ti", "one-body", [(1, 2, 0.5 + 1j), (3, 4, 0.5 + 1j)])
+    assert qcm.matrix_elements("capi_par", "ti") == (
+        "one_body", [(0, 1, 0.5 + 1j), (1, 0, 0.5 - 1j), (2, 3, 0.5 + 1j), (3, 2, 0.5 - 1j)])
     with pytest.raises(_lib.EDError) as err:
-        qcm.new_operator_complex("capi_par", "ti", "one-body", [(1, 2, 1j), (3, 4, 1j)])
+        qcm.new_operator_complex("capi_par", "bad", "one-body", [(1, 1, 1j)])
+    assert err.value.code == 1
+    with pytest.raises(_lib.EDError) as err:
+        qcm.new_operator_complex("capi_par", "bad", "interaction", [(1, 3, 1j)])
     assert err.value.code == 8
-    qcm.new_model_instanceC("capi_par", {"t": 1.0, "h": 0.3}, "R0:N2:S0", 71)
-    assert qcm.parametersC(71) == ({"h": 0.3, "t": 1.0}, "capi_par")
-    qcm.erase_model_instance(71)
+    qcm.new_model_instanceC("capi_par", {"t": 1.0, "h": 0.3, "ti": 0.0}, "R0:N2:S0", 71)
+    assert qcm.parametersC(71) == ({"h": 0.3, "t": 1.0, "ti": 0.0}, "capi_par")
+    # a complex operator with a nonzero value makes the Hilbert space complex
+    # (src_ed/qcm_ED.cpp:145-157), whether or not its values have imaginary parts
+    assert qcm.ED_complex_HS(71) == 1
+    qcm.new_model_instanceC("capi_par", {"t": 1.0, "ti": 0.0}, "R0:N2:S0", 72)
+    assert qcm.ED_complex_HS(72) == 0
+    qcm.new_model_instanceC("capi_par", {"t": 1.0, "ti": 2.0}, "R0:N2:S0", 73)
+    assert qcm.ED_complex_HS(73) == 1
+    h = np.array([[0.0, -1.0 + 1.0 + 2j], [-1.0 + 1.0 - 2j, 0.0]])
+    for spin_down in (0, 1):
+        # the binding's flat copy of the column-major matrix is its transpose
+        assert np.abs(qcm.hopping_matrix(spin_down, label=73) - h.T).max() == 0.0
+    for lab in (71, 72, 73):
+        qcm.erase_model_instance(lab)
 
 
 class _LibstdcxxNormal:

@@ -1,0 +1,321 @@
+"""Complex Hilbert space (model_instance<Complex>, src_ed/qcm_ED.cpp:127-158):
+operators defined with complex elements, the ground state on split [Re | Im]
+vectors, the band-Lanczos Green function and its Q-matrix, cluster averages,
+the self-energy, the hybridization and the Potthoff functional, against:
+  * the reference's fixture tests/test_diagonalization/S_format_complex.ref
+    (8-site open chain with the imaginary hopping ti, U = 1e-9): E0 -9.51754
+    and the printed Q-matrix, through G(z) (tests/golden/s_format_complex.json,
+    itself pinned by tests/test_complex_refs.py), and the free-fermion closed
+    forms E0 = 2 sum of the negative levels, G(z) = (z - h)^-1;
+  * a gauge transformation: a complex hopping on an open interacting chain is
+    the real one up to c_j -> e^{i theta_j} c_j (same E0, G -> D G D^*);
+  * a flux through an interacting ring, which no gauge removes: the dense
+    complex Jordan-Wigner Hamiltonian of tests/dense_ref.py (E0, averages,
+    the Lehmann sum of G);
+  * a free cluster with a complex bath: G = [(z - h)^-1] on the cluster, the
+    self-energy zero (so G, t and Gamma share one convention), the Potthoff
+    functional E0 minus twice the negative bath levels.
+The binding returns each G block as the flat copy of the column-major matrix
+(src_python/qcm_nb.hpp:27-41): the array's [i, j] is G(j, i)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from pyqcm_amd import _lib, qcm
+from pyqcm_amd._lib import EDError
+from tests.dense_ref import dense_c, dense_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+_FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "s_format_complex.json")))
+_label = [9700]
+_models = set()
+_Z = np.array([0.3 + 0.7j, -1.1 + 0.05j, 2.5j, 0.9 + 0.2j])
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    _lib.require_device()
+    for k, v in (("Hamiltonian_format", "G"), ("GF_method", "L")):
+        qcm.set_global_parameterC(k, v)
+    yield
+    for k, v in (("Hamiltonian_format", "G"), ("GF_method", "L"), ("lanczos_block", 6), ("band_lanczos_block", 8)):
+        qcm.set_global_parameterC(k, v)
+
+
+def _instance(name, n_sites, n_bath, ops, values, sector):
+    """ops: name -> (type, 0-based elements r <= c, complex?)"""
+    if name not in _models:
+        qcm.new_model(name, n_sites, n_bath)
+        for op, (typ, el, cx) in ops.items():
+            one = [(r + 1, c + 1, v) for (r, c, v) in el]
+            if cx:
+                qcm.new_operator_complex(name, op, typ, one)
+            else:
+                qcm.new_operator(name, op, typ, one)
+        _models.add(name)
+    _label[0] += 1
+    lab = _label[0]
+    vals = dict(values)
+    for op in ops:
+        vals.setdefault(op, 0.0)
+    qcm.new_model_instanceC(name, vals, sector, lab)
+    return lab
+
+
+def _both_spins(L, el):
+    return el + [(r + L, c + L, v) for (r, c, v) in el]
+
+
+def _one_body(L, ops, values):
+    """physical one-body matrix h[r, c], the coefficient of c^+_r c_c (up spin)"""
+    h = np.zeros((L, L), complex)
+    for op, (typ, el, _) in ops.items():
+        if typ != "one-body" or values.get(op, 0.0) == 0.0:
+            continue
+        for (r, c, v) in el:
+            if r >= L or c >= L:
+                continue
+            h[r, c] += values[op] * v
+            if r != c:
+                h[c, r] += values[op] * np.conj(v)
+    return h
+
+
+def _gf(lab, z=_Z):
+    """G(r, c) at each z (undoing the binding's flat copy)"""
+    return np.transpose(qcm.Green_function_grid(z, label=lab), (0, 2, 1))
+
+
+# ---------------------------------------------------------------------------
+# the reference's fixture
+# ---------------------------------------------------------------------------
+def _chain8_ops():
+    L = 8
+    return {
+        "ti": ("one-body", _both_spins(L, [(j, j + 1, 1j) for j in range(L - 1)]), True),
+        "U": ("interaction", [(j, j + L, 1.0) for j in range(L)], False),
+        "mu": ("one-body", [(j, j, -1.0) for j in range(2 * L)], False),
+    }
+
+
+@pytest.mark.parametrize("fmt", ["G", "S", "N"])
+def test_s_format_complex_fixture(fmt):
+    qcm.set_global_parameterC("Hamiltonian_format", fmt)
+    try:
+        ops = _chain8_ops()
+        p = _FIX["parameters"]
+        lab = _instance("cx_chain8", 8, 0, ops, {"ti": p["ti"], "U": p["U"], "mu": p["mu"]}, _FIX["sector"])
+        assert qcm.ED_complex_HS(lab) == 1
+        E0, sec = qcm.ground_state_solve(lab)
+        h = _one_body(8, ops, {"ti": 1.0})
+        eps = np.linalg.eigvalsh(h)
+        assert abs(E0 - _FIX["E0"]) < 5e-6, (fmt, E0)
+        assert abs(E0 - 2 * eps[eps < 0].sum()) < 1e-7, (fmt, E0)
+        e, Q = qcm.qmatrix(lab)
+        assert np.iscomplexobj(Q) and Q.shape == (8, len(e))
+        q = _FIX["qmatrix"]
+        ef = np.array(q["poles"])
+        Qf = np.array([[complex(*x) for x in row] for row in q["residues"]]).T
+        # one pole per level, as printed (the two copies of each level in the
+        # split space are merged by streamlining)
+        assert len(e) == len(ef), (fmt, e)
+        assert np.abs(np.sort(e) - np.sort(ef)).max() < 1e-8
+        # Q Q^H = 1 (Q_matrix::check_norm)
+        assert np.abs(Q @ Q.conj().T - np.eye(8)).max() < 1e-10
+        G = _gf(lab)
+        for k, z in enumerate(_Z):
+            Gf = np.einsum("ci,ri,i->rc", Qf, Qf.conj(), 1.0 / (z - ef))
+            Gq = np.einsum("ci,ri,i->rc", Q, Q.conj(), 1.0 / (z - e))
+            assert np.abs(G[k] - Gf).max() < 1e-8, (fmt, z, np.abs(G[k] - Gf).max())
+            assert np.abs(G[k] - Gq).max() < 1e-12
+            assert np.abs(G[k] - np.linalg.inv(z * np.eye(8) - h)).max() < 1e-7
+        qcm.erase_model_instance(lab)
+    finally:
+        qcm.set_global_parameterC("Hamiltonian_format", "G")
+
+
+# ---------------------------------------------------------------------------
+# gauge transformation of an interacting open chain
+# ---------------------------------------------------------------------------
+def test_open_chain_gauge():
+    L = 6
+    rng = np.random.default_rng(21)
+    phi = rng.uniform(0, 2 * np.pi, L - 1)
+    amp = [-np.exp(1j * f) for f in phi]
+    common = {
+        "U": ("interaction", [(j, j + L, 1.0) for j in range(L)], False),
+        "mu": ("one-body", [(j, j, -1.0) for j in range(2 * L)], False),
+    }
+    ops_c = dict(common, t=("one-body", _both_spins(L, [(j, j + 1, amp[j]) for j in range(L - 1)]), True))
+    ops_r = dict(common, t=("one-body", _both_spins(L, [(j, j + 1, -1.0) for j in range(L - 1)]), False))
+    vals = {"t": 1.0, "U": 4.0, "mu": 2.0}
+    lc = _instance("cx_gauge_c", L, 0, ops_c, vals, "R0:N6:S0")
+    lr = _instance("cx_gauge_r", L, 0, ops_r, vals, "R0:N6:S0")
+    Ec, _ = qcm.ground_state_solve(lc)
+    Er, _ = qcm.ground_state_solve(lr)
+    assert abs(Ec - Er) < 1e-10 * abs(Er), (Ec, Er)
+    # d_j = e^{-i theta_j} c_j with theta_{j+1} = theta_j - phi_j turns the
+    # complex hopping into the real one: G_c = D G_r D^*, D = diag(e^{i theta})
+    theta = np.concatenate([[0.0], -np.cumsum(phi)])
+    D = np.diag(np.exp(1j * theta))
+    Gc, Gr = _gf(lc), _gf(lr)
+    for k in range(len(_Z)):
+        assert np.abs(Gc[k] - D @ Gr[k] @ D.conj()).max() < 1e-8, k
+    # the averages of the gauge-invariant operators agree, and so does that
+    # of the hopping (the transformation maps it onto the real one)
+    ac, ar = qcm.cluster_averages(lc), qcm.cluster_averages(lr)
+    for op in ("t", "U", "mu"):
+        assert abs(ac[op][0] - ar[op][0]) < 1e-9, (op, ac[op], ar[op])
+        assert abs(ac[op][1] - ar[op][1]) < 1e-8, (op, ac[op], ar[op])
+    Ac = qcm.Green_function_average(lc).T
+    Ar = qcm.Green_function_average(lr).T
+    assert np.abs(Ac - D @ Ar @ D.conj()).max() < 1e-9
+    assert abs(qcm.Green_function_density(lc) - qcm.Green_function_density(lr)) < 1e-10
+    for lab in (lc, lr):
+        qcm.erase_model_instance(lab)
+
+
+# ---------------------------------------------------------------------------
+# flux through an interacting ring: dense complex reference
+# ---------------------------------------------------------------------------
+def _ring_ops(L, flux):
+    bonds = [(j, j + 1, -1.0 + 0j) for j in range(L - 1)] + [(0, L - 1, -np.exp(1j * flux))]
+    return {
+        "t": ("one-body", _both_spins(L, bonds), True),
+        "U": ("interaction", [(j, j + L, 1.0) for j in range(L)], False),
+        "mu": ("one-body", [(j, j, -1.0) for j in range(2 * L)], False),
+        "tp": ("one-body", _both_spins(L, [(0, 2, 0.3j), (1, 3, -0.2 + 0.1j)]), True),
+    }
+
+
+def _dense_terms(ops, vals):
+    return [(typ, vals[op], el) for op, (typ, el, _) in ops.items() if vals.get(op, 0.0) != 0.0]
+
+
+def _dense_ring(L, ops, vals, Nup, Ndw):
+    """E0, ground state and G(z) of the dense complex Hamiltonian (Lehmann sums
+    over every state of the N +- 1 sectors), spin up"""
+    H = dense_hamiltonian(L, _dense_terms(ops, vals), Nup, Ndw)
+    w, V = np.linalg.eigh(H)
+    assert w[1] - w[0] > 1e-6  # a single ground state
+    E0, psi = w[0], V[:, 0]
+    G = np.zeros((len(_Z), L, L), complex)
+    He = dense_hamiltonian(L, _dense_terms(ops, vals), Nup + 1, Ndw)
+    Hh = dense_hamiltonian(L, _dense_terms(ops, vals), Nup - 1, Ndw)
+    we, Ve = np.linalg.eigh(He)
+    wh, Vh = np.linalg.eigh(Hh)
+    ae = np.array([Ve.conj().T @ (dense_c(L, a, Nup, Ndw, True) @ psi) for a in range(L)])   # <n|c+_a|0>
+    ah = np.array([Vh.conj().T @ (dense_c(L, a, Nup, Ndw, False) @ psi) for a in range(L)])  # <m|c_a|0>
+    for k, z in enumerate(_Z):
+        ue = 1.0 / (z - (we - E0))
+        uh = 1.0 / (z + (wh - E0))
+        # G_rc = <c_r (z - H + E0)^-1 c+_c> + <c+_c (z + H - E0)^-1 c_r>
+        G[k] = np.einsum("rn,cn,n->rc", ae.conj(), ae, ue) + np.einsum("cm,rm,m->rc", ah.conj(), ah, uh)
+    return E0, psi, G, H
+
+
+@pytest.mark.parametrize("fmt,lanczos_block,band_block", [("G", 6, 8), ("G", 0, 0), ("S", 6, 8), ("N", 3, 0)])
+def test_flux_ring_against_dense(fmt, lanczos_block, band_block):
+    L = 4
+    ops = _ring_ops(L, 0.7)
+    vals = {"t": 1.0, "U": 4.0, "mu": 1.5, "tp": 1.0}
+    qcm.set_global_parameterC("Hamiltonian_format", fmt)
+    qcm.set_global_parameterC("lanczos_block", lanczos_block)
+    qcm.set_global_parameterC("band_lanczos_block", band_block)
+    try:
+        lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
+        E0, _ = qcm.ground_state_solve(lab)
+        E0d, psi, Gd, _ = _dense_ring(L, ops, vals, 2, 2)
+        assert abs(E0 - E0d) < 1e-10 * abs(E0d), (E0, E0d)
+        G = _gf(lab)
+        assert np.abs(G - Gd).max() < 1e-8, np.abs(G - Gd).max()
+        # averages: <psi|O|psi> and its variance on the dense ground state
+        ave = qcm.cluster_averages(lab)
+        for op in ops:
+            O = dense_hamiltonian(L, _dense_terms(ops, {op: 1.0}), 2, 2)
+            a = np.vdot(psi, O @ psi).real
+            v = np.vdot(O @ psi, O @ psi).real - a * a
+            assert abs(ave[op][0] - a) < 1e-9, (op, ave[op], a)
+            assert abs(ave[op][1] - v) < 1e-8, (op, ave[op], v)
+        qcm.erase_model_instance(lab)
+    finally:
+        for k, v in (("Hamiltonian_format", "G"), ("lanczos_block", 6), ("band_lanczos_block", 8)):
+            qcm.set_global_parameterC(k, v)
+
+
+# ---------------------------------------------------------------------------
+# free cluster with a complex bath: self-energy, hybridization, Potthoff
+# ---------------------------------------------------------------------------
+def _bath_ops():
+    ns, nb = 2, 2
+    L = ns + nb
+    return L, {
+        "t": ("one-body", _both_spins(L, [(0, 1, -1.0 + 0.4j)]), True),
+        "eb": ("one-body", _both_spins(L, [(2, 2, -0.8), (3, 3, 0.9), (2, 3, 0.25 - 0.3j)]), True),
+        "v": ("one-body", _both_spins(L, [(0, 2, 0.5 + 0.2j), (1, 3, 0.4j), (0, 3, 0.1)]), True),
+        "U": ("interaction", [(j, j + L, 1.0) for j in range(ns)], False),
+    }
+
+
+@pytest.mark.parametrize("band_block", [0, 8])
+def test_free_cluster_with_complex_bath(band_block):
+    L, ops = _bath_ops()
+    vals = {"t": 1.0, "eb": 1.0, "v": 1.0}
+    h = _one_body(L, ops, vals)
+    eps = np.linalg.eigvalsh(h)
+    nneg = int((eps < 0).sum())
+    qcm.set_global_parameterC("band_lanczos_block", band_block)
+    try:
+        lab = _instance("cx_bath22", 2, 2, ops, vals, "R0:N%d:S0" % (2 * nneg))
+        E0, _ = qcm.ground_state_solve(lab)
+        assert abs(E0 - 2 * eps[:nneg].sum()) < 1e-10, (E0, eps)
+        G = _gf(lab)
+        S = np.array([qcm.self_energyC(z, label=lab).T for z in _Z])
+        Gam = np.array([qcm.hybridization_functionC(z, label=lab).T for z in _Z])
+        tb = h[2:, 2:]
+        tcb = h[:2, 2:]
+        for k, z in enumerate(_Z):
+            assert np.abs(G[k] - np.linalg.inv(z * np.eye(L) - h)[:2, :2]).max() < 1e-9
+            g_ref = tcb @ np.linalg.inv(z * np.eye(2) - tb) @ tcb.conj().T
+            assert np.abs(Gam[k] - g_ref).max() < 1e-12
+            assert np.abs(S[k]).max() < 1e-8, (z, np.abs(S[k]).max())
+        # the bath block, diagonalized (Hermitian), and the cluster-bath block
+        e, T = qcm.hybridization(lab)
+        assert np.abs(e - np.linalg.eigvalsh(tb)).max() < 1e-12
+        T = T.reshape(-1).reshape(2, 2, order="F")  # n_sites x n_bath
+        assert np.abs(T @ T.conj().T - tcb @ tcb.conj().T).max() < 1e-12
+        full = qcm.hopping_matrix(label=lab, full=1).T
+        assert np.abs(full - h).max() < 1e-15
+        omega = qcm.Potthoff_functionalC(lab)
+        eb = np.linalg.eigvalsh(tb)
+        assert abs(omega - (E0 - 2 * eb[eb < 0].sum())) < 1e-10
+        qcm.erase_model_instance(lab)
+    finally:
+        qcm.set_global_parameterC("band_lanczos_block", 8)
+
+
+def test_complex_unsupported_paths():
+    """what a complex Hilbert space does not provide here fails with code 8"""
+    L = 4
+    ops = _ring_ops(L, 0.3)
+    vals = {"t": 1.0, "U": 4.0, "mu": 1.5}
+    lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
+    qcm.ground_state_solve(lab)
+    for fn in (lambda: qcm.susceptibility("U", [0.5j], label=lab), lambda: qcm.fidelity(lab, lab),
+               lambda: qcm.density_matrix([0], label=lab), lambda: qcm.print_wavefunction(lab)):
+        with pytest.raises(EDError) as err:
+            fn()
+        assert err.value.code == 8
+    qcm.erase_model_instance(lab)
+    qcm.set_global_parameterC("GF_method", "F")
+    try:
+        lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
+        with pytest.raises(EDError) as err:
+            qcm.Green_function_grid(_Z, label=lab)
+        assert err.value.code == 8
+        qcm.erase_model_instance(lab)
+    finally:
+        qcm.set_global_parameterC("GF_method", "L")
