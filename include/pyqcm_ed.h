@@ -153,7 +153,8 @@ int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, do
 int ed_hamiltonian_apply_dot_noalloc(int64_t handle, const double *dx, double *dy, double beta, double *d_dot);
 /* Hamiltonian::diag */
 int ed_hamiltonian_diag(int64_t handle, double *d);
-/* strict-lower CSR pattern of the 'S' format (Hamiltonian_CSR.hpp:60-81): Iptr[dim+1], J[nnz] */
+/* strict-lower CSR pattern of the 'S' format (Hamiltonian_CSR.hpp:60-81): Iptr[dim+1], J[nnz]
+   (ED_ERR_UNSUPPORTED for the split plan of a complex Hilbert space) */
 int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v);
 /* time nrep device applications y = H x (HIP events on the library stream) */
 int ed_hamiltonian_time(int64_t handle, int nrep, double *ms_per_apply, int variant);

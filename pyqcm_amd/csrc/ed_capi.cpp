@@ -744,6 +744,8 @@ int ed_hamiltonian_diag(int64_t handle, double *d) {
 int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v) {
   return guard([&] {
     Hamiltonian &H = get_h(handle);
+    if (terms_complex(H.terms))
+      fail(ED_ERR_UNSUPPORTED, "the CSR pattern of a complex Hilbert space's Hamiltonian is not provided");
     std::vector<int32_t> ip, jj;
     std::vector<double> vv, dg;
     csr_lower(H.terms, H.Nup, H.Ndw, ip, jj, vv, dg);
