@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 
 #include "../../include/pyqcm_ed.h"
@@ -20,31 +21,62 @@ using namespace ed;
 
 namespace {
 thread_local std::string g_last_error;
-// One caller at a time: the library's device stream, reduction buffers and
-// registries are shared, so concurrent calls (the reference's OpenMP loop over
-// clusters, src_qcm/lattice_model_instance.cpp:107, :160) queue here.
-// ed_request_cancel and ed_last_error do not take it.
-std::recursive_mutex g_api_mu;
+// Concurrency.  Calls that define or remove models, instances, global
+// parameters, handles or device memory take the library lock exclusively;
+// calls on one instance label take it shared plus that label's own lock, so
+// solves and evaluations of different labels (the reference's OpenMP loop
+// over clusters, src_qcm/lattice_model_instance.cpp:107-113, :160) run
+// concurrently, each queued on its host thread's stream.  Every call returns
+// with its device work complete, so the next call on a label may come from
+// any thread.  ed_request_cancel and ed_last_error take no lock.
+std::shared_mutex g_api_mu;
+std::mutex g_label_mu;
+std::map<int, std::shared_ptr<std::mutex>> g_label_locks;
+
+std::shared_ptr<std::mutex> label_lock(int label) {
+  std::lock_guard<std::mutex> l(g_label_mu);
+  auto &m = g_label_locks[label];
+  if (!m) m = std::make_shared<std::mutex>();
+  return m;
+}
+
+template <typename F>
+int run(F &&f) {
+  int rc = ED_OK;
+  try {
+    f();
+  } catch (const ed::Error &e) {
+    g_last_error = e.what();
+    rc = e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    rc = ED_ERR_OOM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    rc = ED_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    rc = ED_ERR_INTERNAL;
+  }
+  if (stream_started() && hipStreamSynchronize(stream()) != hipSuccess && rc == ED_OK) {
+    g_last_error = std::string("HIP error: ") + hipGetErrorString(hipGetLastError());
+    rc = ED_ERR_HIP;
+  }
+  return rc;
+}
 
 template <typename F>
 int guard(F &&f) {
-  std::lock_guard<std::recursive_mutex> lock(g_api_mu);
-  try {
-    f();
-    return ED_OK;
-  } catch (const ed::Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return ED_ERR_OOM;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return ED_ERR_INTERNAL;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return ED_ERR_INTERNAL;
-  }
+  std::unique_lock<std::shared_mutex> lock(g_api_mu);
+  return run(f);
+}
+
+template <typename F>
+int guard_label(int label, F &&f) {
+  std::shared_lock<std::shared_mutex> lock(g_api_mu);
+  const std::shared_ptr<std::mutex> m = label_lock(label);
+  std::lock_guard<std::mutex> ll(*m);
+  return run(f);
 }
 
 std::mutex g_h_mu;
@@ -183,7 +215,7 @@ int ed_erase_model_instance(int label) {
 }
 
 int ed_ground_state_solve(int label, double *E0, char *sector, int cap) {
-  return guard([&] {
+  return guard_label(label, [&] {
     auto r = get_instance(label).ground_state_solve();
     if (E0) *E0 = r.first;
     if (sector && cap > 0) {
@@ -194,7 +226,7 @@ int ed_ground_state_solve(int label, double *E0, char *sector, int cap) {
 }
 
 int ed_ground_state_energy(int label, double *E0) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     I.ground_state_solve();
     *E0 = I.E0;
@@ -202,15 +234,15 @@ int ed_ground_state_energy(int label, double *E0) {
 }
 
 int ed_green_function_solve(int label) {
-  return guard([&] { get_instance(label).green_function_solve(); });
+  return guard_label(label, [&] { get_instance(label).green_function_solve(); });
 }
 
 int ed_green_function_dimension(int label, int *d) {
-  return guard([&] { *d = get_instance(label).gf_dim(); });
+  return guard_label(label, [&] { *d = get_instance(label).gf_dim(); });
 }
 
 int ed_green_function(int label, int nz, const double *z, int spin_down, int blocks, double *out) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     I.green_function(nz, z, spin_down != 0, out);
     const PointGroup &G = I.model->group;
@@ -253,15 +285,15 @@ int ed_green_function_batch(const int *labels, int n, int nz, const double *z, i
 }
 
 int ed_self_energy(int label, int nz, const double *z, int spin_down, double *out) {
-  return guard([&] { get_instance(label).self_energy(nz, z, spin_down != 0, out); });
+  return guard_label(label, [&] { get_instance(label).self_energy(nz, z, spin_down != 0, out); });
 }
 
 int ed_hybridization_function(int label, int nz, const double *z, int spin_down, double *out) {
-  return guard([&] { get_instance(label).hybridization(nz, z, spin_down != 0, out); });
+  return guard_label(label, [&] { get_instance(label).hybridization(nz, z, spin_down != 0, out); });
 }
 
 int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     const auto &A = I.cluster_averages();
     *n = (int)A.size();
@@ -273,7 +305,7 @@ int ed_cluster_averages(int label, int cap, double *ave, double *var, int *n) {
 }
 
 int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, int *M) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     I.green_function_solve();
     need(!I.states.empty(), "no state");
@@ -292,7 +324,7 @@ int ed_qmatrix(int label, int spin_down, int cap, double *e, double *v, int *L, 
 }
 
 int ed_states(int label, int cap, double *energy, double *weight, int *n) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     I.ground_state_solve();
     *n = (int)I.states.size();
@@ -304,7 +336,7 @@ int ed_states(int label, int cap, double *energy, double *weight, int *n) {
 }
 
 int ed_mixing(int label, int *mixing) {
-  return guard([&] { *mixing = get_instance(label).mixing; });
+  return guard_label(label, [&] { *mixing = get_instance(label).mixing; });
 }
 
 int ed_solve_batch(const int *labels, int n, int flags) {
@@ -349,7 +381,7 @@ static KGrid kgrid_from(const ed_kgrid *K) {
 
 int ed_cpt_projected_gf(int label, int nz, const double *z, int spin_down, const ed_superlattice *V,
                         const ed_kgrid *K, double *out) {
-  return guard([&] {
+  return guard_label(label, [&] {
     if (nz < 0 || (nz > 0 && (!z || !out))) fail(ED_ERR_INVALID, "ed_cpt_projected_gf: bad arguments");
     Instance &I = get_instance(label);
     const Superlattice S = superlattice_from(V);
@@ -367,7 +399,7 @@ int ed_cpt_projected_gf(int label, int nz, const double *z, int spin_down, const
 
 int ed_sef_grid(int label, int nw, const double *w, const double *weights, const ed_superlattice *Vup,
                 const ed_superlattice *Vdn, const ed_kgrid *K, double *integral, double *per_freq) {
-  return guard([&] {
+  return guard_label(label, [&] {
     if (nw < 0 || (nw > 0 && (!w || !weights)) || !integral) fail(ED_ERR_INVALID, "ed_sef_grid: bad arguments");
     Instance &I = get_instance(label);
     if (!I.gf_solved) I.green_function_solve();
@@ -399,7 +431,7 @@ int ed_sef_grid(int label, int nw, const double *w, const double *weights, const
 }
 
 int ed_potthoff_functional(int label, double *omega) {
-  return guard([&] {
+  return guard_label(label, [&] {
     if (!omega) fail(ED_ERR_INVALID, "ed_potthoff_functional: null output");
     Instance &I = get_instance(label);
     I.ground_state_solve();
@@ -409,7 +441,7 @@ int ed_potthoff_functional(int label, double *omega) {
 
 int ed_instance_parameters(int label, int cap, int name_len, char *names, double *values, int *n, char *model,
                            int model_cap) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(n != nullptr && (cap == 0 || (names && values && name_len > 0)), "ed_instance_parameters: bad arguments");
     Instance &I = get_instance(label);
     *n = (int)I.value.size();
@@ -430,14 +462,14 @@ int ed_instance_parameters(int label, int cap, int name_len, char *names, double
 }
 
 int ed_complex_hs(int label, int *is_complex) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(is_complex != nullptr, "ed_complex_hs: null output");
     *is_complex = get_instance(label).complex_hs() ? 1 : 0;
   });
 }
 
 int ed_hopping_matrix(int label, int spin_down, int diag, int full, double *out) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(out != nullptr, "ed_hopping_matrix: null output");
     Instance &I = get_instance(label);
     const int ns = I.model->n_sites, nb = I.model->n_bath;
@@ -473,7 +505,7 @@ int ed_hopping_matrix(int label, int spin_down, int diag, int full, double *out)
 }
 
 int ed_interactions(int label, int cap, int *r, int *c, double *v, int *n) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(n != nullptr && (cap == 0 || (r && c && v)), "ed_interactions: bad output arrays");
     Instance &I = get_instance(label);
     int k = 0;
@@ -494,7 +526,7 @@ int ed_interactions(int label, int cap, int *r, int *c, double *v, int *n) {
 }
 
 int ed_hybridization_lehmann(int label, int spin_down, double *e, double *tcb) {
-  return guard([&] {
+  return guard_label(label, [&] {
     Instance &I = get_instance(label);
     const int ns = I.model->n_sites, nb = I.model->n_bath;
     if (nb == 0) fail(ED_ERR_INVALID, "System has no bath. Hybridization not defined.");
@@ -533,7 +565,7 @@ int ed_matrix_elements(const char *model, const char *op, char *type, int type_c
 }
 
 int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *rho, uint64_t *basis, int *dim_a) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(nsites >= 0 && (nsites == 0 || sites) && dim_a, "ed_density_matrix: bad arguments");
     std::vector<double> r;
     std::vector<uint64_t> b;
@@ -551,7 +583,7 @@ int ed_density_matrix(int label, const int *sites, int nsites, int cap, double *
 }
 
 int ed_print_wavefunction(int label, char *buf, int cap, int *len) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(len != nullptr && (cap == 0 || buf), "ed_print_wavefunction: bad arguments");
     const std::string s = get_instance(label).print_wavefunction();
     *len = (int)s.size();
@@ -589,7 +621,7 @@ int ed_hdf5_version(char *buf, int cap) {
 }
 
 int ed_green_function_average(int label, int spin_down, double *out) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(out != nullptr, "ed_green_function_average: null output");
     Instance &I = get_instance(label);
     const std::vector<cplx> G = I.gf_average(spin_down != 0);
@@ -601,21 +633,21 @@ int ed_green_function_average(int label, int spin_down, double *out) {
 }
 
 int ed_green_function_density(int label, double *density) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(density != nullptr, "ed_green_function_density: null output");
     *density = get_instance(label).gf_density();
   });
 }
 
 int ed_tr_sigma_inf(int label, double *value) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(value != nullptr, "ed_tr_sigma_inf: null output");
     *value = get_instance(label).tr_sigma_inf();
   });
 }
 
 int ed_susceptibility(int label, const char *op, int nw, const double *w, double *chi) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(op != nullptr, "ed_susceptibility: null operator name");
     need(nw >= 0 && (nw == 0 || (w && chi)), "ed_susceptibility: bad frequency arrays");
     const auto parts = get_instance(label).susceptibility_parts(op);
@@ -645,7 +677,7 @@ int ed_susceptibility(int label, const char *op, int nw, const double *w, double
 }
 
 int ed_susceptibility_poles(int label, const char *op, int cap, double *poles, int *n) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(op != nullptr && n != nullptr, "ed_susceptibility_poles: null argument");
     need(cap >= 0 && (cap == 0 || poles), "ed_susceptibility_poles: bad output array");
     const auto parts = get_instance(label).susceptibility_parts(op);
@@ -1000,7 +1032,7 @@ int ed_release_cache(void) {
 
 int ed_continued_fractions(int label, int spin_down, int state, int cap, double *a, double *b, int *len,
                            int *nfrac) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(cap >= 0 && nfrac != nullptr, "ed_continued_fractions: bad arguments");
     Instance &I = get_instance(label);
     std::vector<std::vector<double>> av, bv;
@@ -1019,7 +1051,7 @@ int ed_continued_fractions(int label, int spin_down, int state, int cap, double 
 
 int ed_continued_fractions_pairs(int label, int spin_down, int npairs, const int *o1, const int *o2, int cap,
                                  double *a, double *b, int *len) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(npairs >= 0 && cap >= 1 && (npairs == 0 || (o1 && o2 && a && b && len)),
          "ed_continued_fractions_pairs: bad arguments");
     Instance &I = get_instance(label);
@@ -1053,7 +1085,7 @@ int ed_set_continued_fractions(int label, double E0, const char *sector, int spi
 }
 
 int ed_combined_mcf(int label, int spin_down, int cap, double *W, double *A, double *B, int *p, int *nfl) {
-  return guard([&] {
+  return guard_label(label, [&] {
     need(p && nfl && cap >= 0, "ed_combined_mcf: bad arguments");
     Instance &I = get_instance(label);
     if (I.gf_method != 'M')

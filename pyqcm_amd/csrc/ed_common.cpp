@@ -60,6 +60,7 @@ Params::Params() {
   i["lanczos_block"] = 6;           // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
   i["band_lanczos_block"] = 8;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
   b["lanczos_graph"] = true;        // replay the block as a captured hipGraph
+  i["gf_streams"] = 2;              // band-Lanczos Green function: hole and electron parts on 2 host threads / streams (1: one after the other)
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)
   b["sell16"] = true;               // 16-bit sign-only sliced-ELL entries for single-magnitude hoppings
   d["lanczos_basis_budget_GB"] = -1.0;  // negative: 90 % of the available device memory less 4 GB
@@ -98,11 +99,19 @@ Params &params() {
 // ---------------------------------------------------------------------------
 // caching allocator: exact-size free lists (Krylov vectors of one sector all
 // have the same size, so reuse is near perfect); freed on ed_release_cache
-// or when an allocation fails.
+// or when an allocation fails.  Each block belongs to the stream of the thread
+// that allocated it and is handed out again only to that stream, whose order
+// then covers any work still queued on the block; a block freed from another
+// stream waits for that stream first.
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_alloc_mu;
-std::unordered_multimap<size_t, void *> g_free;
+struct FreeBlock {
+  void *p;
+  hipStream_t s;
+};
+std::unordered_multimap<size_t, FreeBlock> g_free;
+std::unordered_map<void *, hipStream_t> g_owner;  // live blocks
 size_t g_cached_bytes = 0;
 
 size_t round_up(size_t b) {
@@ -113,7 +122,7 @@ size_t round_up(size_t b) {
 
 void dev_release_cache() {
   std::lock_guard<std::mutex> l(g_alloc_mu);
-  for (auto &kv : g_free) (void)hipFree(kv.second);
+  for (auto &kv : g_free) (void)hipFree(kv.second.p);
   g_free.clear();
   g_cached_bytes = 0;
 }
@@ -137,15 +146,18 @@ size_t dev_available_bytes() {
 
 void *dev_alloc(size_t bytes) {
   size_t b = round_up(bytes);
+  const hipStream_t s = stream();
   {
     std::lock_guard<std::mutex> l(g_alloc_mu);
-    auto it = g_free.find(b);
-    if (it != g_free.end()) {
-      void *p = it->second;
-      g_free.erase(it);
-      g_cached_bytes -= b;
-      return p;
-    }
+    auto r = g_free.equal_range(b);
+    for (auto it = r.first; it != r.second; ++it)
+      if (it->second.s == s) {
+        void *p = it->second.p;
+        g_free.erase(it);
+        g_cached_bytes -= b;
+        g_owner[p] = s;
+        return p;
+      }
   }
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, b);
@@ -158,73 +170,126 @@ void *dev_alloc(size_t bytes) {
       fail(ED_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
   }
+  std::lock_guard<std::mutex> l(g_alloc_mu);
+  g_owner[p] = s;
   return p;
 }
 
 void dev_free(void *p, size_t bytes) {
   if (!p) return;
   size_t b = round_up(bytes);
+  hipStream_t owner = nullptr;
+  {
+    std::lock_guard<std::mutex> l(g_alloc_mu);
+    auto it = g_owner.find(p);
+    if (it != g_owner.end()) {
+      owner = it->second;
+      g_owner.erase(it);
+    }
+  }
+  // freed from another thread's stream: its queued work on the block ends
+  // before the owner stream may hand the block out again
+  if (stream_started() && stream() != owner) (void)hipStreamSynchronize(stream());
   std::lock_guard<std::mutex> l(g_alloc_mu);
-  g_free.emplace(b, p);
+  g_free.emplace(b, FreeBlock{p, owner ? owner : stream()});
   g_cached_bytes += b;
 }
+
+// Pinned staging buffers belong to one host thread at a time (each thread
+// copies through its own, with events recorded on its own stream: an event
+// recorded on another thread's stream cannot be waited on while that stream
+// is being captured into a graph).  A thread's set comes from a process-wide
+// pool on first use and goes back, drained, when the thread ends.
+namespace {
+struct Staging {
+  static constexpr int kUp = 16;                          // upload ring slots
+  static constexpr size_t kUpSlot = size_t(512) << 10;
+  static constexpr size_t kDown = size_t(8) << 20;        // download double buffer
+  char *up = nullptr, *down[2] = {nullptr, nullptr};
+  hipEvent_t ev_up[kUp], ev_down[2];
+  bool used[kUp] = {};
+  int next = 0;
+  Staging() {
+    ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&up), kUp * kUpSlot, hipHostMallocDefault));
+    for (int i = 0; i < kUp; ++i) ED_HIP(hipEventCreateWithFlags(&ev_up[i], hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) ED_HIP(hipEventCreateWithFlags(&ev_down[i], hipEventDisableTiming));
+  }
+  char *down_buf(int i) {
+    if (!down[i]) ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&down[i]), kDown, hipHostMallocDefault));
+    return down[i];
+  }
+  // the uploads still in flight end before another thread may own the set
+  void drain() {
+    for (int i = 0; i < kUp; ++i)
+      if (used[i]) (void)hipEventSynchronize(ev_up[i]);
+    for (int i = 0; i < kUp; ++i) used[i] = false;
+  }
+};
+std::mutex g_staging_mu;
+std::vector<Staging *> *g_staging_pool = new std::vector<Staging *>;  // never destroyed
+struct ThreadStaging {
+  Staging *s = nullptr;
+  ~ThreadStaging() {
+    if (!s) return;
+    s->drain();
+    std::lock_guard<std::mutex> l(g_staging_mu);
+    g_staging_pool->push_back(s);
+  }
+};
+thread_local ThreadStaging t_staging;
+Staging &staging() {
+  if (!t_staging.s) {
+    {
+      std::lock_guard<std::mutex> l(g_staging_mu);
+      if (!g_staging_pool->empty()) {
+        t_staging.s = g_staging_pool->back();
+        g_staging_pool->pop_back();
+      }
+    }
+    if (!t_staging.s) t_staging.s = new Staging;
+  }
+  return *t_staging.s;
+}
+}  // namespace
 
 void h2d_staged(void *dst, const void *src, size_t bytes) {
   // pinned staging ring: slot k is reused once the copy out of it has run
   // (its event); the host copy into pinned memory is the only host wait, so
   // the many small table uploads of a sector no longer each pay a pageable
   // transfer's synchronous staging
-  constexpr int kSlots = 16;
-  constexpr size_t kSlot = size_t(512) << 10;
-  static std::mutex mu;
-  static char *pin = nullptr;
-  static hipEvent_t ev[kSlots];
-  static bool used[kSlots] = {};
-  static int next = 0;
   if (!bytes) return;
-  std::lock_guard<std::mutex> lock(mu);
-  if (!pin) {
-    ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&pin), kSlots * kSlot, hipHostMallocDefault));
-    for (int i = 0; i < kSlots; ++i) ED_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
-  }
-  for (size_t off = 0; off < bytes; off += kSlot) {
-    const size_t m = std::min(kSlot, bytes - off);
-    const int k = next;
-    next = (next + 1) % kSlots;
-    if (used[k]) ED_HIP(hipEventSynchronize(ev[k]));
-    char *slot = pin + (size_t)k * kSlot;
+  Staging &R = staging();
+  for (size_t off = 0; off < bytes; off += Staging::kUpSlot) {
+    const size_t m = std::min(Staging::kUpSlot, bytes - off);
+    const int k = R.next;
+    R.next = (R.next + 1) % Staging::kUp;
+    if (R.used[k]) ED_HIP(hipEventSynchronize(R.ev_up[k]));
+    char *slot = R.up + (size_t)k * Staging::kUpSlot;
     std::memcpy(slot, static_cast<const char *>(src) + off, m);
     ED_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, slot, m, hipMemcpyHostToDevice, stream()));
-    ED_HIP(hipEventRecord(ev[k], stream()));
-    used[k] = true;
+    ED_HIP(hipEventRecord(R.ev_up[k], stream()));
+    R.used[k] = true;
   }
 }
 
 void d2h_pageable(void *dst, const void *src, size_t bytes) {
-  constexpr size_t kChunk = size_t(8) << 20;
+  constexpr size_t kChunk = Staging::kDown;
   if (bytes <= kChunk) {
     if (bytes) ED_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
     ED_HIP(hipStreamSynchronize(stream()));
     return;
   }
-  static std::mutex mu;
-  static char *pin[2] = {nullptr, nullptr};
-  static hipEvent_t ev[2];
-  std::lock_guard<std::mutex> lock(mu);
-  if (!pin[0])
-    for (int i = 0; i < 2; ++i) {
-      ED_HIP(hipHostMalloc(reinterpret_cast<void **>(&pin[i]), kChunk, hipHostMallocDefault));
-      ED_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
-    }
+  Staging &R = staging();
+  char *pin[2] = {R.down_buf(0), R.down_buf(1)};
   const size_t nch = (bytes + kChunk - 1) / kChunk;
   auto len = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
   auto issue = [&](size_t c) {
     ED_HIP(hipMemcpyAsync(pin[c % 2], static_cast<const char *>(src) + c * kChunk, len(c), hipMemcpyDeviceToHost,
                           stream()));
-    ED_HIP(hipEventRecord(ev[c % 2], stream()));
+    ED_HIP(hipEventRecord(R.ev_down[c % 2], stream()));
   };
   auto drain = [&](size_t c) {
-    ED_HIP(hipEventSynchronize(ev[c % 2]));
+    ED_HIP(hipEventSynchronize(R.ev_down[c % 2]));
     char *d = static_cast<char *>(dst) + c * kChunk;
     const char *h = pin[c % 2];
     const size_t m = len(c), blk = size_t(1) << 20, nb = (m + blk - 1) / blk;
@@ -239,14 +304,39 @@ void d2h_pageable(void *dst, const void *src, size_t bytes) {
   drain(nch - 1);
 }
 
+// Each host thread queues its work on a stream of its own (non-blocking,
+// default priority), taken from a process-wide pool on the thread's first
+// use and handed back when the thread ends, so threads that come and go
+// reuse streams; work a thread left queued stays ordered before the next
+// owner's.
+namespace {
+std::mutex g_stream_mu;
+std::vector<hipStream_t> *g_stream_pool = new std::vector<hipStream_t>;  // never destroyed
+struct ThreadStream {
+  hipStream_t s = nullptr;
+  ~ThreadStream() {
+    if (!s) return;
+    std::lock_guard<std::mutex> l(g_stream_mu);
+    g_stream_pool->push_back(s);
+  }
+};
+thread_local ThreadStream t_stream;
+}  // namespace
+
 hipStream_t stream() {
-  static hipStream_t s = [] {
-    hipStream_t st;
-    ED_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    return st;
-  }();
-  return s;
+  if (!t_stream.s) {
+    std::lock_guard<std::mutex> l(g_stream_mu);
+    if (!g_stream_pool->empty()) {
+      t_stream.s = g_stream_pool->back();
+      g_stream_pool->pop_back();
+    } else {
+      ED_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
+    }
+  }
+  return t_stream.s;
 }
+
+bool stream_started() { return t_stream.s != nullptr; }
 
 void sync() { ED_HIP(hipStreamSynchronize(stream())); }
 

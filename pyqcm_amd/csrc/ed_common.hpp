@@ -66,7 +66,10 @@ void dev_free(void *p, size_t bytes);
 void dev_release_cache();
 // free device memory plus the blocks this library holds in its cache
 size_t dev_available_bytes();
-hipStream_t stream();  // the library's compute stream (default priority)
+// the calling thread's compute stream (default priority; one per host thread)
+hipStream_t stream();
+// whether the calling thread has a stream already
+bool stream_started();
 
 // host-to-device copy out of pageable memory through a pinned staging ring
 // (asynchronous on the library stream once the data is staged)

@@ -248,12 +248,14 @@ Sector cdag_apply(int L, int pm, int a, const Sector &src, const double *x, doub
 // and keeps the second, so after a vector of odd length the next vector starts
 // with that kept draw (from the previous seed's stream)
 static std::normal_distribution<double> g_normal(0.0, 1.0);
+static std::mutex g_normal_mu;
 
 void random_start(size_t n, long seed, std::vector<double> &x) {
   // random() of src_util/vector_num.cpp:114-119: default_random_engine(seed),
   // N(0,1) draws, scaled by 1 / ||x|| (normalize, src_util/vector_num.hpp:139-145)
   std::default_random_engine gen((unsigned)seed);
   x.resize(n);
+  std::lock_guard<std::mutex> lock(g_normal_mu);
   double s = 0.0;
   for (size_t i = 0; i < n; ++i) {
     x[i] = g_normal(gen);
@@ -865,10 +867,66 @@ static void build_gf(Instance &I, State &st, int spin) {
     streamline(ns, out.first, out.second);
     return out;
   };
-  // the hole part's eigen-decomposition runs on a host thread while the
-  // device works on the electron part
-  std::future<Part> hole_part;
-  for (int pm = -1; pm <= 1; pm += 2) {
+  if (G.kind == 'L') {
+    // band Lanczos of one part (pm = -1 holes, +1 electrons) from the ns
+    // vectors c^{pm}_i |Omega>; an empty run when the target sector is
+    // invalid or empty
+    auto run_part = [&](int pm) -> BandLanczosRun {
+      check_cancel();
+      Sector tgt = shift_sector(st.sec, pm, spin);
+      if (!sector_valid(L, tgt)) return BandLanczosRun{};
+      const size_t dt = sector_dim(L, tgt);
+      if (dt == 0) return BandLanczosRun{};
+      vlog("GF %s: start", tgt.name().c_str());
+      auto H = make_hamiltonian(T, tgt, I.format);
+      std::vector<DevBuf<double>> phi;
+      std::vector<const double *> ph;
+      for (int i = 0; i < ns; ++i) {
+        phi.emplace_back(dt);
+        fill(dt, phi.back().p, 0.0, stream());
+        cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p, 1.0, phi.back().p);
+        ph.push_back(phi.back().p);
+      }
+      vlog("GF %s (pm %+d, spin %d): dim %zu, %d start vectors ready", tgt.name().c_str(), pm, spin, dt, ns);
+      return band_lanczos_run(*H, ph);
+    };
+    Part h, e;
+    if (params().geti("gf_streams") > 1) {
+      // the two parts on two host threads, each queuing on its own stream
+      // (the same runs as one after the other); Omega is complete before the
+      // second stream reads it, and the worker's stream is idle before its
+      // buffers and Omega may be released
+      sync();
+      std::future<Part> fe = std::async(std::launch::async, [&]() {
+        Part p = lehmann(run_part(1), 1);
+        sync();
+        return p;
+      });
+      try {
+        h = lehmann(run_part(-1), -1);
+      } catch (...) {
+        fe.wait();
+        throw;
+      }
+      e = fe.get();
+    } else {
+      // the hole part's eigen-decomposition on a host thread while the
+      // device runs the electron part
+      std::future<Part> fh = std::async(std::launch::async, [lehmann, r = run_part(-1)]() { return lehmann(r, -1); });
+      try {
+        e = lehmann(run_part(1), 1);
+      } catch (...) {
+        fh.wait();
+        throw;
+      }
+      h = fh.get();
+    }
+    eh.swap(h.first);
+    Vh.swap(h.second);
+    ee.swap(e.first);
+    Ve.swap(e.second);
+  }
+  for (int pm = -1; pm <= 1 && G.kind != 'L'; pm += 2) {
     check_cancel();
     Sector tgt = shift_sector(st.sec, pm, spin);
     if (!sector_valid(L, tgt)) continue;
@@ -884,19 +942,7 @@ static void build_gf(Instance &I, State &st, int spin) {
       cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p, 1.0, phi.back().p);
     }
     (void)dim0;
-    if (G.kind == 'L') {
-      std::vector<const double *> ph;
-      for (auto &b : phi) ph.push_back(b.p);
-      vlog("GF %s (pm %+d, spin %d): dim %zu, %d start vectors ready", tgt.name().c_str(), pm, spin, dt, ns);
-      BandLanczosRun run = band_lanczos_run(*H, ph);
-      if (pm == -1) {
-        hole_part = std::async(std::launch::async, [lehmann, r = std::move(run)]() { return lehmann(r, -1); });
-      } else {
-        Part el = lehmann(run, 1);
-        ee.swap(el.first);
-        Ve.swap(el.second);
-      }
-    } else if (G.kind == 'F') {
+    if (G.kind == 'F') {
       // build_cf (src_ed/model_instance.cpp:35-100)
       DevBuf<double> psi(dt);
       int idx = 0;
@@ -968,11 +1014,6 @@ static void build_gf(Instance &I, State &st, int spin) {
     } else {
       fail(ED_ERR_INVALID, std::string("GF_method must be L, F or M (got ") + G.kind + ")");
     }
-  }
-  if (hole_part.valid()) {
-    Part h = hole_part.get();
-    eh.swap(h.first);
-    Vh.swap(h.second);
   }
   if (G.kind == 'L') {
     G.e = eh;

@@ -1259,9 +1259,11 @@ static __device__ __align__(256) double scratch[kPartialsCap + kNoAllocAux];
 
 NoAllocScratch noalloc_scratch() {
   static double *base[kMaxNoAllocDevices] = {};
+  static std::mutex mu;
   int dev = 0;
   ED_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxNoAllocDevices) fail(ED_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
   if (!base[dev]) {
     void *p = nullptr;
     ED_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
@@ -1439,9 +1441,9 @@ int hv_factorized_cfg(const FactArgs &a, const double *x, double *y, double ax, 
   const size_t slabs_per_xcd = (Q.nslab + Q.nxcd - 1) / Q.nxcd;
   const size_t nb = (size_t)Q.nxcd * slabs_per_xcd * Q.nchunk;
   // per-block partials go to a scratch array folded back to <= kPartialsCap
-  // (one array for the process: library calls are serialized), or to the
+  // (one array per host thread, queued on that thread's stream), or to the
   // caller's aux array
-  static DevBuf<double> big;
+  thread_local DevBuf<double> big;
   double *wide = nullptr;
   if (part && nb > (size_t)kPartialsCap) {
     if (aux) {

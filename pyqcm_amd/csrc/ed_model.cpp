@@ -441,18 +441,42 @@ struct PartialArena {
     return (int)((size_t)(p - d) / slot_len);
   }
 };
-// One arena for the process: every C-ABI call holds the library lock, so the
-// slots are never used from two threads at once, and the pinned block is
-// allocated once however many threads call in.
-PartialArena *g_arena = nullptr;
+// One arena per host thread (a thread polls only the reductions it queued on
+// its own stream), taken from a process-wide pool on first use and handed
+// back when the thread ends, so the pinned blocks are allocated once per
+// concurrent thread, not once per thread.
+std::mutex g_arena_mu;
+std::vector<PartialArena *> *g_arena_pool = new std::vector<PartialArena *>;  // never destroyed
+struct ThreadArena {
+  PartialArena *a = nullptr;
+  ~ThreadArena() {
+    if (!a) return;
+    bool dirty = false;
+    for (int s = 0; s < kSlots; ++s) dirty = dirty || a->busy[s];
+    // an arena with reductions abandoned by an error path may still be
+    // written by queued kernels: it stays out of the pool
+    if (dirty) return;
+    std::lock_guard<std::mutex> l(g_arena_mu);
+    g_arena_pool->push_back(a);
+  }
+};
+thread_local ThreadArena t_arena;
 PartialArena &arena() {
-  static PartialArena a;
-  g_arena = &a;
-  return a;
+  if (!t_arena.a) {
+    {
+      std::lock_guard<std::mutex> l(g_arena_mu);
+      if (!g_arena_pool->empty()) {
+        t_arena.a = g_arena_pool->back();
+        g_arena_pool->pop_back();
+      }
+    }
+    if (!t_arena.a) t_arena.a = new PartialArena;
+  }
+  return *t_arena.a;
 }
-// slot of p in the arena, -1 if p is elsewhere (or no arena exists yet: the
-// lookup never allocates one)
-int arena_slot_of(const double *p) { return g_arena ? g_arena->slot_of(p) : -1; }
+// slot of p in the calling thread's arena, -1 if p is elsewhere (or the
+// thread has no arena yet: the lookup never allocates one)
+int arena_slot_of(const double *p) { return t_arena.a ? t_arena.a->slot_of(p) : -1; }
 
 // the device writes these entries while the host polls them: each read is an
 // atomic acquire load, so the compiler re-reads memory on every poll and the
@@ -519,7 +543,7 @@ double sum_partials_rows(const double *dpart, int nb, int m, size_t stride, std:
     }
     return first;
   }
-  PartialArena &A = *g_arena;
+  PartialArena &A = *t_arena.a;
   const double *hp = A.h + (dpart - A.d);
   if (m > 0 && nb > 0) {
     if ((size_t)(dpart - A.d) - (size_t)slot * A.slot_len + (size_t)(m - 1) * stride + nb > A.slot_len)
@@ -980,9 +1004,11 @@ struct HamFactorized : Hamiltonian {
     // small sectors: the fused kernel or an LDS-slab two-pass schedule, timed
     // once per sector shape in the process (the tables' sizes key it)
     static std::map<std::array<uint64_t, 4>, std::pair<int, SlabConfig>> small_cache;
+    static std::mutex small_mu;
     const std::array<uint64_t, 4> key{args.up.dim, args.dn.dim, (uint64_t)up.host.ptr.back(),
                                       (uint64_t)dn.host.ptr.back()};
     if (small && !force) {
+      std::lock_guard<std::mutex> l(small_mu);
       auto it = small_cache.find(key);
       if (it != small_cache.end()) {
         best_variant = it->second.first;
@@ -1020,7 +1046,10 @@ struct HamFactorized : Hamiltonian {
     }
     ED_HIP(hipEventDestroy(e0));
     ED_HIP(hipEventDestroy(e1));
-    if (small) small_cache[key] = {best_variant, best_sc};
+    if (small) {
+      std::lock_guard<std::mutex> l(small_mu);
+      small_cache[key] = {best_variant, best_sc};
+    }
   }
   int apply(const double *x, double *y, double ax, double beta, bool want_dot) override {
     int v = variant;

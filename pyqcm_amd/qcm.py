@@ -51,7 +51,7 @@ _PARAM_TYPES = {
     "n_states": _INT, "max_dim_full": _INT, "max_dim_print": _INT, "max_iter_BL": _INT, "max_iter_CF": _INT,
     "max_iter_lanczos": _INT, "seed": _INT, "GF_lookup_depth": _INT, "hv_variant": _INT,
     "sell_sort_window": _INT, "hv_block": _INT, "hv_row_order": _INT, "vector_block": _INT, "reduction_blocks_per_cu": _INT,
-    "lanczos_block": _INT, "band_lanczos_block": _INT,
+    "lanczos_block": _INT, "band_lanczos_block": _INT, "gf_streams": _INT,
     "Hamiltonian_format": _CHAR, "Ground_state_method": _CHAR, "GF_method": _CHAR,
 }
 
