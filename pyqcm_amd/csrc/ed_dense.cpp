@@ -165,7 +165,7 @@ void band_eig_first_rows(int n, int b, const double *Ain, int p, std::vector<dou
         // annihilate M[i][k] with rows (i-1, i)
         double x = at(i - 1, k), y = at(i, k);
         if (y == 0.0) continue;
-        double r = std::hypot(x, y), c = x / r, s = y / r;
+        double r = fast_hypot(x, y), c = x / r, s = y / r;
         apply_rot(i - 1, c, s);
         at(i, k) = 0.0;
         // chase the bulge created at (i + b, i - 1)
@@ -175,7 +175,7 @@ void band_eig_first_rows(int n, int b, const double *Ain, int p, std::vector<dou
           x = at(row - 1, col);
           y = at(row, col);
           if (y == 0.0) break;
-          r = std::hypot(x, y);
+          r = fast_hypot(x, y);
           c = x / r;
           s = y / r;
           apply_rot(row - 1, c, s);

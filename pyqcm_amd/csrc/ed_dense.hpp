@@ -1,9 +1,20 @@
 // ed_dense.hpp -- host eigen-solvers for the small projected problems.
 #pragma once
 
+#include <cmath>
 #include <vector>
 
 namespace ed {
+
+// sqrt(x^2 + y^2) without std::hypot's scaling when the squares cannot
+// overflow or underflow (the common case here; std::hypot dominated the QL
+// and the band reduction's rotations)
+inline double fast_hypot(double x, double y) {
+  const double ax = std::fabs(x), ay = std::fabs(y);
+  const double m = ax > ay ? ax : ay;
+  if (m < 1e150 && m > 1e-150) return std::sqrt(x * x + y * y);
+  return std::hypot(x, y);
+}
 
 // implicit QL on a symmetric tridiagonal matrix (d diagonal, e[i] couples i and
 // i+1); the rotations are accumulated into the nz rows of Z (row-major nz x n).

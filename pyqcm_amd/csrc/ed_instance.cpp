@@ -152,6 +152,18 @@ TermList Instance::terms() const {
 
 bool Instance::complex_hs() const { return terms_complex(terms()); }
 
+std::unique_ptr<Hamiltonian> Instance::take_gf_hamiltonian(const Sector &tgt) {
+  std::future<std::unique_ptr<Hamiltonian>> f;
+  {
+    std::lock_guard<std::mutex> lock(gf_hams_mu);
+    auto it = gf_hams.find(tgt);
+    if (it == gf_hams.end()) return nullptr;
+    f = std::move(it->second);
+    gf_hams.erase(it);
+  }
+  return f.get();
+}
+
 void Instance::need_real(const char *what) const {
   if (complex_hs())
     fail(ED_ERR_UNSUPPORTED, std::string(what) + " in a complex Hilbert space is not supported (instance " +
@@ -287,6 +299,24 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     const size_t dim = sector_dim(model->L, s) * (cx ? 2 : 1);
     if (dim == 0) continue;
     auto H = make_hamiltonian(T, s, format, cx);
+    // the band-Lanczos Green function of this sector's ground state needs the
+    // N +- 1 sectors (up spin): their tables are built and uploaded on host
+    // threads, each with its own stream, while the device runs the Lanczos
+    if (gf_method == 'L' && targets.size() == 1 && !cx && model->group.g == 1 && P.geti("gf_streams") > 1) {
+      std::lock_guard<std::mutex> lock(gf_hams_mu);
+      gf_hams.clear();
+      for (int pm = -1; pm <= 1; pm += 2) {
+        const Sector tgt = shift_sector(s, pm, 0);
+        if (!sector_valid(model->L, tgt) || sector_dim(model->L, tgt) == 0) continue;
+        const char fmt = format;
+        gf_hams[tgt] = std::async(std::launch::async, [T, tgt, fmt]() {
+          auto Ht = make_hamiltonian(T, tgt, fmt);
+          Ht->prepare();
+          sync();
+          return Ht;
+        });
+      }
+    }
     long ns = std::max<long>(1, std::min<long>(n_states, (long)dim));
     char meth = (method == 'L' && ns > 1) ? 'D' : method;
     // each level of the split form appears twice (psi and i psi)
@@ -878,7 +908,8 @@ static void build_gf(Instance &I, State &st, int spin) {
       const size_t dt = sector_dim(L, tgt);
       if (dt == 0) return BandLanczosRun{};
       vlog("GF %s: start", tgt.name().c_str());
-      auto H = make_hamiltonian(T, tgt, I.format);
+      std::unique_ptr<Hamiltonian> H = spin == 0 ? I.take_gf_hamiltonian(tgt) : nullptr;
+      if (!H) H = make_hamiltonian(T, tgt, I.format);
       std::vector<DevBuf<double>> phi;
       std::vector<const double *> ph;
       for (int i = 0; i < ns; ++i) {

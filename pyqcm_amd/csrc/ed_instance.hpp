@@ -2,8 +2,10 @@
 // representations (model_instance<T>, src_ed/model_instance.hpp).
 #pragma once
 
+#include <future>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <tuple>
@@ -65,6 +67,12 @@ struct Instance {
   std::vector<std::unique_ptr<State>> states;
   std::vector<std::tuple<std::string, double, double>> averages;
   bool averages_done = false;
+  // Hamiltonians of the Green function's target sectors, built on host
+  // threads while the ground state is solved (band-Lanczos method, one target
+  // sector); taken by the Green-function build
+  std::map<Sector, std::future<std::unique_ptr<Hamiltonian>>> gf_hams;
+  std::mutex gf_hams_mu;
+  std::unique_ptr<Hamiltonian> take_gf_hamiltonian(const Sector &tgt);
 
   TermList terms() const;
   // complex Hilbert space: a term defined with complex elements has a nonzero

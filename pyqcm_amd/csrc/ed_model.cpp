@@ -1437,15 +1437,6 @@ static std::unique_ptr<Hamiltonian> make_plan(const TermList &T, const Sector &s
 // implicit QL on a symmetric tridiagonal matrix; rotations accumulated into the
 // rows of Z given by zrows (n_z rows of length n, row-major) -- the full matrix
 // (zrows = n, identity start) or only the last row (for the Ritz residual).
-// sqrt(x^2 + y^2) without std::hypot's scaling when the squares cannot
-// overflow or underflow (the common case here; std::hypot dominated the QL)
-static inline double fast_hypot(double x, double y) {
-  const double ax = std::fabs(x), ay = std::fabs(y);
-  const double m = ax > ay ? ax : ay;
-  if (m < 1e150 && m > 1e-150) return std::sqrt(x * x + y * y);
-  return std::hypot(x, y);
-}
-
 void ql_rows(int n, double *d, double *e, double *Z, int nz) {
   if (n <= 0) return;
   e[n - 1] = 0.0;
