@@ -1676,58 +1676,59 @@ __device__ __forceinline__ double lz_fold(const double *part, int nb, double *sh
   __syncthreads();
   return sh[kLzBlock / kWave];
 }
-// a[k] = <W, C> from the product's partials; W -= a[k] C + b[k-1] P; norm partials
-__global__ __launch_bounds__(kLzBlock) void k_lz_update(size_t n, double *W, const double *C, const double *P,
-                                                        double *coef, int cap, const int *base, int j,
-                                                        const double *dpart, int dnb, double *npart) {
+// One device-driven step on unnormalized vectors (v_k = U_k / n_k with
+// n_k = |U_k| = b[k-1], n_0 = 1): Y = H U_k and its partials of <Y, U_k> come
+// from the product; then a[k] = <Y, U_k> / n_k^2, b[k-1] = n_k (folded from the
+// previous step's norm partials), and over Y
+//   U_{k+1} = (Y - a[k] U_k) / n_k - (n_k / n_{k-1}) U_{k-1},
+// with v_k = U_k / n_k copied to slots[k] when that entry exists and the norm
+// partials of U_{k+1} written; the two norm-partial arrays alternate with the
+// parity of k, so a step never overwrites the partials it folds.  n_k = 0 (an
+// exhausted Krylov space) leaves non-finite values, never read: b[k-1] = 0
+// ends the recurrence on the host.
+__global__ __launch_bounds__(kLzBlock) void k_lz_step(size_t n, double *Y, const double *U, const double *Up,
+                                                      double *coef, int cap, const int *base, int j,
+                                                      const double *dpart, int dnb, double *npart, int nnb,
+                                                      double *const *slots, int nslots) {
   __shared__ double red[kLzBlock / kWave + 1];
   const int k = *base + j;
-  const double a = lz_fold(dpart, dnb, red);
-  if (blockIdx.x == 0 && threadIdx.x == 0) coef[k] = a;
-  const double bp = k > 0 ? coef[cap + k - 1] : 0.0;
+  const double d = lz_fold(dpart, dnb, red);
+  double nk = 1.0, nkm = 1.0;
+  if (k > 0) {
+    nk = sqrt(lz_fold(npart + (size_t)(k & 1) * kPartialsCap, nnb, red));
+    if (k > 1) nkm = coef[cap + k - 2];
+  }
+  const double inv = 1.0 / nk, a = d * inv * inv, cp = k > 0 ? nk / nkm : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    coef[k] = a;
+    if (k > 0) coef[cap + k - 1] = nk;
+  }
+  double *dst = (slots && k > 0 && k < nslots) ? slots[k] : nullptr;
   double s = 0.0;
   const size_t stride = (size_t)gridDim.x * kLzBlock;
-  if (k > 0) {
-    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
-      const double w = W[i] - a * C[i] - bp * P[i];
-      W[i] = w;
-      s += w * w;
-    }
-  } else {
-    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
-      const double w = W[i] - a * C[i];
-      W[i] = w;
-      s += w * w;
-    }
+  // (the first step has no previous vector: its buffer is not read)
+  for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
+    const double u = U[i];
+    double w = (Y[i] - a * u) * inv;
+    if (k > 0) w -= cp * Up[i];
+    Y[i] = w;
+    s += w * w;
+    if (dst) dst[i] = u * inv;
   }
   s = block_sum<kLzBlock>(s, red);
-  if (threadIdx.x == 0) npart[blockIdx.x] = s;
+  if (threadIdx.x == 0) npart[(size_t)((k + 1) & 1) * kPartialsCap + blockIdx.x] = s;
 }
-// b[k] = |W| from the update's partials; W /= b[k] (and copied to slots[k + 1]).
-// b[k] = 0 (an exhausted Krylov space) leaves non-finite values, never read:
-// the host's stop rule ends the recurrence at that step
-__global__ __launch_bounds__(kLzBlock) void k_lz_advance(size_t n, double *W, double *coef, int cap,
-                                                         const int *base, int j, const double *npart, int nnb,
-                                                         double *const *slots, int nslots) {
+// end of a block of steps: b[k] = |U_{k+1}| for its last step k (the next
+// step would fold the same value), then *base moves on
+__global__ __launch_bounds__(kLzBlock) void k_lz_next(int *base, int steps, const double *npart, int nnb,
+                                                      double *coef, int cap) {
   __shared__ double red[kLzBlock / kWave + 1];
-  const int k = *base + j;
-  const double b = sqrt(lz_fold(npart, nnb, red));
-  if (blockIdx.x == 0 && threadIdx.x == 0) coef[cap + k] = b;
-  const double inv = 1.0 / b;
-  double *dst = (slots && k + 1 < nslots) ? slots[k + 1] : nullptr;
-  const size_t stride = (size_t)gridDim.x * kLzBlock;
-  if (dst) {
-    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) {
-      const double w = W[i] * inv;
-      W[i] = w;
-      dst[i] = w;
-    }
-  } else {
-    for (size_t i = (size_t)blockIdx.x * kLzBlock + threadIdx.x; i < n; i += stride) W[i] *= inv;
+  const int k = *base + steps - 1;
+  const double nk1 = sqrt(lz_fold(npart + (size_t)((k + 1) & 1) * kPartialsCap, nnb, red));
+  if (threadIdx.x == 0) {
+    coef[cap + k] = nk1;
+    *base += steps;
   }
-}
-__global__ void k_lz_next(int *base, int steps) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *base += steps;
 }
 
 __global__ __launch_bounds__(256) void k_group_project(ProjArgs A, const double *x, double *y) {
@@ -1749,19 +1750,15 @@ static int lz_grid(size_t n) {
   const size_t g = (n + kLzBlock - 1) / kLzBlock;
   return (int)std::max<size_t>(1, std::min<size_t>(g, (size_t)num_cu()));
 }
-void lz_update(const LanczosDev &L, double *W, const double *C, const double *P, int j, int dnb, hipStream_t s) {
-  hipLaunchKernelGGL(k_lz_update, dim3(lz_grid(L.n)), dim3(kLzBlock), 0, s, L.n, W, C, P, L.coef, L.cap,
-                     (const int *)L.base, j, (const double *)L.part, dnb, L.part + kPartialsCap);
-  ED_HIP(hipGetLastError());
-}
-void lz_advance(const LanczosDev &L, double *W, int j, hipStream_t s) {
-  hipLaunchKernelGGL(k_lz_advance, dim3(lz_grid(L.n)), dim3(kLzBlock), 0, s, L.n, W, L.coef, L.cap,
-                     (const int *)L.base, j, (const double *)(L.part + kPartialsCap), lz_grid(L.n), L.slots,
-                     L.nslots);
+void lz_step(const LanczosDev &L, double *W, const double *C, const double *P, int j, int dnb, hipStream_t s) {
+  const int g = lz_grid(L.n);
+  hipLaunchKernelGGL(k_lz_step, dim3(g), dim3(kLzBlock), 0, s, L.n, W, C, P, L.coef, L.cap, (const int *)L.base, j,
+                     (const double *)L.part, dnb, L.part + kPartialsCap, g, L.slots, L.nslots);
   ED_HIP(hipGetLastError());
 }
 void lz_next(const LanczosDev &L, int steps, hipStream_t s) {
-  hipLaunchKernelGGL(k_lz_next, dim3(1), dim3(64), 0, s, L.base, steps);
+  hipLaunchKernelGGL(k_lz_next, dim3(1), dim3(kLzBlock), 0, s, L.base, steps,
+                     (const double *)(L.part + kPartialsCap), lz_grid(L.n), L.coef, L.cap);
   ED_HIP(hipGetLastError());
 }
 

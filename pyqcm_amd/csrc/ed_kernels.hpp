@@ -194,24 +194,22 @@ void cdag_gather(const CdagArgs &a, const double *x, double *y, hipStream_t s);
 // The recurrence's scalars stay on the device: a[k] at coef[k], b[k] at
 // coef[cap + k], with k = *base + j for step j of a block (j is a launch
 // constant, so a captured block replays unchanged; lz_next moves *base on).
-// Vectors are normalized; step k is
-//   W = H C (dot partials in part),  W -= a[k] C + b[k-1] P (a[k] folded from
-//   those partials; norm partials in part + partials_cap()),  W /= b[k] (b[k]
-//   folded from them; W also copied to slots[k + 1] when that entry of the
-//   device slot table is not null).
+// Vectors are kept unnormalized (v_k = U_k / n_k, n_k = |U_k| = b[k-1]); step k
+// is W = H C (C = U_k, dot partials in part), then one pass (lz_step) that
+// folds a[k] and n_k, forms U_{k+1} over W from W, C and P = U_{k-1}, copies
+// v_k to slots[k] when that entry of the device slot table is not null, and
+// leaves the norm partials of U_{k+1} in part + (1 + (k+1) % 2) partials_cap();
+// lz_next folds the last step's b[k] and moves *base on.
 struct LanczosDev {
   size_t n = 0;
   int cap = 0;
   double *coef = nullptr;   // 2 cap
   int *base = nullptr;
-  double *part = nullptr;   // 2 partials_cap()
+  double *part = nullptr;   // 3 partials_cap()
   double *const *slots = nullptr;  // device table of nslots vector pointers (or null)
   int nslots = 0;
 };
-// W -= a[k] C + b[k-1] P with a[k] folded from the product's dnb partials in part
-void lz_update(const LanczosDev &L, double *W, const double *C, const double *P, int j, int dnb, hipStream_t s);
-// W /= b[k], b[k] folded from the update's partials
-void lz_advance(const LanczosDev &L, double *W, int j, hipStream_t s);
+void lz_step(const LanczosDev &L, double *W, const double *C, const double *P, int j, int dnb, hipStream_t s);
 void lz_next(const LanczosDev &L, int steps, hipStream_t s);
 
 // projection on an irreducible representation of the cluster's point group, in

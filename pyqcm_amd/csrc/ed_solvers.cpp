@@ -73,17 +73,19 @@ double comb(size_t n, double *t, double st, const std::vector<const double *> &b
 }
 
 // ---------------------------------------------------------------------------
-// Device-driven Lanczos recurrence over normalized vectors (kernels lz_*): the
-// scalars of a step are folded, used and stored on the device, so a block of
-// steps runs without a host wait; the block is queued once under stream
-// capture and replayed as a hipGraph.  The host reads the new coefficients
-// once per launch and applies the caller's stop rule to them in step order, so
-// the recurrence it keeps is the one a per-step check would keep (the steps
-// queued past the stop are discarded).  Steps rotate over three buffers (step
-// k: C = R[k % 3], P = R[(k + 2) % 3], W = R[(k + 1) % 3]), hence blocks of a
-// multiple of 3 steps.  With slots, step k also copies v_{k+1} into a vector
-// of its own (the Krylov basis kept in HBM), allocated by the host before the
-// block that needs it, within a byte budget.
+// Device-driven Lanczos recurrence (kernels lz_*) over unnormalized vectors
+// U_k = b[k-1] v_k: the scalars of a step are folded, used and stored on the
+// device, and the normalization of U_{k+1} is folded into the next step's
+// pass, so a step is one product and one vector pass and a block of steps
+// runs without a host wait; the block is queued once under stream capture and
+// replayed as a hipGraph.  The host reads the new coefficients once per launch
+// and applies the caller's stop rule to them in step order, so the recurrence
+// it keeps is the one a per-step check would keep (the steps queued past the
+// stop are discarded).  Steps rotate over three buffers (step k: C = R[k % 3],
+// P = R[(k + 2) % 3], W = R[(k + 1) % 3]), hence blocks of a multiple of 3
+// steps.  With slots, step k also copies v_k into a vector of its own (the
+// Krylov basis kept in HBM), allocated by the host before the block that
+// needs it, within a byte budget.
 // ---------------------------------------------------------------------------
 class LanczosDeviceRun {
  public:
@@ -95,7 +97,7 @@ class LanczosDeviceRun {
     cap = ((max_steps + K - 1) / K + 1) * K;
     for (auto &r : R) r.alloc(n);
     coef.alloc(2 * (size_t)cap);
-    part.alloc(2 * (size_t)partials_cap());
+    part.alloc(3 * (size_t)partials_cap());
     base.alloc(1);
     d2d(R[0].p, d_start, n);
     const int zero = 0;
@@ -160,8 +162,7 @@ class LanczosDeviceRun {
     for (int j = 0; j < K; ++j) {
       double *C = R[j % 3].p, *Pv = R[(j + 2) % 3].p, *W = R[(j + 1) % 3].p;
       const int nb = H.apply_fixed(C, W, 1.0, 0.0, part.p, nullptr, 0);
-      lz_update(D, W, C, Pv, j, nb, stream());
-      lz_advance(D, W, j, stream());
+      lz_step(D, W, C, Pv, j, nb, stream());
     }
     lz_next(D, K, stream());
   }
@@ -180,7 +181,7 @@ class LanczosDeviceRun {
     (void)hipGraphDestroy(g);
     ED_HIP(e);
   }
-  // vectors for slots 1 .. upto (step k writes slot k + 1) within the budget
+  // vectors for slots 1 .. upto (step k writes slot k) within the budget
   void grow_basis(int upto) {
     if (basis.empty() || basis_full) return;
     const size_t first = basis.size();
