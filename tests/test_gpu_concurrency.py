@@ -135,3 +135,43 @@ def test_one_label_from_several_threads_in_turn():
     step("ave", lambda: qcm.cluster_averages(lab))
     _same((res["gs"][0], res["gs"][1], res["gf"], res["ave"]), want)
     qcm.erase_model_instance(lab)
+
+
+def test_many_threads_mixed_calls():
+    """eight threads, each creating, solving, evaluating and erasing its own
+    instances in a loop (with threads ending and new ones starting, so
+    streams, partial-sum arenas and staging buffers are handed on), while
+    the main thread defines a new model: every result equals the serial one"""
+    cases = _cases()
+    serial = []
+    for wl, v in cases:
+        lab = _new(wl, v)
+        serial.append(_solve(lab))
+        qcm.erase_model_instance(lab)
+    lock = threading.Lock()
+    errs = []
+
+    def work(t):
+        try:
+            for r in range(3):
+                i = (t + r) % len(cases)
+                wl, v = cases[i]
+                with lock:
+                    lab = _new(wl, v)
+                got = _solve(lab)
+                qcm.erase_model_instance(lab)
+                _same(got, serial[i])
+        except Exception as e:  # reported below
+            errs.append((t, repr(e)))
+
+    for wave in range(2):
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+        for t in ths:
+            t.start()
+        if wave == 0:
+            qcm.new_model("conc_extra", 2, 0)
+            qcm.new_operator("conc_extra", "t", "one-body", [(1, 2, -1.0), (3, 4, -1.0)])
+        for t in ths:
+            t.join(600)
+        assert all(not t.is_alive() for t in ths)
+    assert not errs, errs[:3]
