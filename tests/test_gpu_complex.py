@@ -101,7 +101,7 @@ def _chain8_ops():
     }
 
 
-@pytest.mark.parametrize("fmt", ["G", "S", "N"])
+@pytest.mark.parametrize("fmt", ["G", "S", "E", "O", "N"])
 def test_s_format_complex_fixture(fmt):
     qcm.set_global_parameterC("Hamiltonian_format", fmt)
     try:
