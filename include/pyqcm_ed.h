@@ -138,7 +138,7 @@ int ed_hamiltonian_destroy(int64_t handle);
 int ed_hamiltonian_dim(int64_t handle, int64_t *dim);
 /* y += H x on host arrays (Hamiltonian::mult_add) */
 int ed_hamiltonian_mult_add(int64_t handle, const double *x, double *y);
-/* y = H x + beta y on device arrays, on the library stream (y not read when beta == 0) */
+/* y = H x + beta y on device arrays, on the calling thread's stream (y not read when beta == 0); complete on return */
 int ed_hamiltonian_apply_device(int64_t handle, const double *dx, double *dy, double beta);
 /* Allocation-free variant of ed_hamiltonian_apply_device (no counterpart in the
    reference): y = H x + beta y on device arrays, and *d_dot = <x, y_new> written
@@ -156,7 +156,7 @@ int ed_hamiltonian_diag(int64_t handle, double *d);
 /* strict-lower CSR pattern of the 'S' format (Hamiltonian_CSR.hpp:60-81): Iptr[dim+1], J[nnz]
    (ED_ERR_UNSUPPORTED for the split plan of a complex Hilbert space) */
 int ed_hamiltonian_csr_pattern(int64_t handle, int64_t *nnz, int32_t *Iptr, int32_t *J, double *v);
-/* time nrep device applications y = H x (HIP events on the library stream) */
+/* time nrep device applications y = H x (HIP events on the calling thread's stream) */
 int ed_hamiltonian_time(int64_t handle, int nrep, double *ms_per_apply, int variant);
 /* kernel selection knob for the factorized format: 0 auto (autotuned), 1 fused
    row kernel, 2 two-pass (row up-part + column-slab down-part) */

@@ -72,11 +72,11 @@ hipStream_t stream();
 bool stream_started();
 
 // host-to-device copy out of pageable memory through a pinned staging ring
-// (asynchronous on the library stream once the data is staged)
+// (asynchronous on the calling thread's stream once the data is staged)
 void h2d_staged(void *dst, const void *src, size_t bytes);
 // device-to-host copy into pageable memory through pinned staging buffers
 // (transfers of one chunk overlap the host copy of the previous one);
-// synchronizes the library stream
+// synchronizes the calling thread's stream
 void d2h_pageable(void *dst, const void *src, size_t bytes);
 
 template <typename T>

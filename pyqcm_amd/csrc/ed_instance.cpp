@@ -1086,7 +1086,7 @@ void cf_export(Instance &I, int spin, int state, std::vector<std::vector<double>
   b.assign(2 * np, {});
   if (G.store) {
     const CfStore &st = *G.store;
-    sync();  // the store is written on the library stream
+    sync();  // the store is written on this thread's stream
     std::vector<int> off(st.nfrac + 1);
     ED_HIP(hipMemcpy(off.data(), st.off.p, off.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int f = 0; f < 2 * np; ++f) {

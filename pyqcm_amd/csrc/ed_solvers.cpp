@@ -1,6 +1,6 @@
 // ed_solvers.cpp -- Krylov drivers: host control flow, device vectors.
 //
-// Every vector operation runs on the library stream; the host only sees the
+// Every vector operation runs on the calling thread's stream; the host only sees the
 // scalars of the recurrences (alpha, beta, Gram coefficients), obtained from
 // deterministic per-block partial sums.  All recurrences are FP64.
 #include "ed_solvers.hpp"
