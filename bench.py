@@ -334,6 +334,75 @@ def ed_solve_gpu(nz=1024, beta=50.0, reps=5, windows=False):
     return wl, E0, float(np.median(times))
 
 
+def ed_solve_concurrent(nclusters=4, nz=1024, beta=50.0, reps=3):
+    """The reference's OpenMP loop over clusters (src_qcm/lattice_model_instance.cpp:107-113):
+    nclusters instances of the plaquette + 8 bath model (each at its own U)
+    solved (ground state + band-Lanczos G on the grid) one after the other,
+    then from nclusters host threads at once; median of reps after a warm-up."""
+    import threading
+    import numpy as np
+    from pyqcm_amd import qcm, workloads as W
+    wl = W.plaquette_bath(seed=2025)
+    z = W.matsubara(nz, beta)
+    name = "bench_conc"
+    W.setup(wl, label=7300, model_name=name)
+    qcm.erase_model_instance(7300)
+    lab = [7300]
+
+    def make():
+        labs = []
+        for c in range(nclusters):
+            lab[0] += 1
+            v = dict(wl.values)
+            v["U"] = wl.values["U"] + 0.25 * c
+            qcm.new_model_instanceC(name, v, wl.sector, lab[0])
+            labs.append(lab[0])
+        return labs
+
+    def solve(l, out):
+        qcm.ground_state_solve(l)
+        out[l] = qcm.Green_function_grid(z, label=l)
+
+    def serial():
+        labs, out = make(), {}
+        t0 = time.perf_counter()
+        for l in labs:
+            solve(l, out)
+        t = time.perf_counter() - t0
+        for l in labs:
+            qcm.erase_model_instance(l)
+        return t, [out[l] for l in labs]
+
+    def threaded():
+        labs, out = make(), {}
+        ths = [threading.Thread(target=solve, args=(l, out)) for l in labs]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        t = time.perf_counter() - t0
+        for l in labs:
+            qcm.erase_model_instance(l)
+        return t, [out[l] for l in labs]
+
+    serial()
+    threaded()
+    ts, tt, dG = [], [], 0.0
+    for _ in range(reps):
+        a, Ga = serial()
+        b, Gb = threaded()
+        ts.append(a)
+        tt.append(b)
+        dG = max(dG, max(float(np.abs(x - y).max()) for x, y in zip(Ga, Gb)))
+    s_, t_ = float(np.median(ts)), float(np.median(tt))
+    return {"workload": "%d instances of the plaquette + 8 bath model (U = 8, 8.25, ...), ground state + "
+                        "band-Lanczos G on 1024 Matsubara points each" % nclusters,
+            "serial_s": round(s_, 4), "threads_s": round(t_, 4), "threads": nclusters,
+            "speedup": round(s_ / t_, 2), "max_dG_threads_vs_serial": dG,
+            "timing": "median of %d after a warm-up of each" % reps}
+
+
 def ed_solve_cpu(wl, nz=1024, beta=50.0, reps=5):
     """Oracle pipeline on the CPU: Lanczos GS, band Lanczos per (pm), VDVH;
     one warm-up run, then the median of reps."""
@@ -580,6 +649,7 @@ def _solve_leg(args):
              "gpu_solves_per_s": round(1.0 / tg, 3), "gpu_s_per_solve": round(tg, 4), "E0_gpu": E0g,
              "timing": "median of 5 GPU solves after one warm-up; CPU: median of 5 after one warm-up, "
                        "measured in this run"}
+    solve["concurrent_clusters"] = ed_solve_concurrent()
     if not args.no_cpu:
         E0c, tc = ed_solve_cpu(wl2)
         import oracle.oracle as orc
