@@ -793,10 +793,90 @@ static double qnorm_error_c(int L, const std::vector<cplx> &V) {
 // twice the complex residue, hence the factor 1/sqrt(2), and streamlining
 // merges the copies.  As there, the electron part stores conj(<phi_a|n>) and
 // the hole part <chi_a|m>, so that G(r, c) = sum_i V(c, i) conj(V(r, i)) / (z - e_i).
+// Continued fractions of a state of a complex Hilbert space (build_cf,
+// src_ed/model_instance.cpp:101-177): for o1 >= o2 the fraction of
+// c_o1 (+ c_o2) |Omega>, and for o1 > o2 that of c_o2 -+ i c_o1 |Omega> (- for
+// electrons, + for holes), each a LanczosGreen on the split plan from the
+// split vector (multiplication by i is [x_r | x_i] -> [-x_i | x_r]).  On split
+// vectors <w, (z - M)^-1 w> sums both copies of every level: it equals the
+// complex <psi|(z - H)^-1|psi>, so each fraction is the reference's.
+static void build_cf_complex(Instance &I, State &st, int spin) {
+  const int L = I.model->L, ns = I.model->n_sites;
+  const TermList T = I.terms();
+  const size_t ds = sector_dim(L, st.sec);
+  GFRep &G = st.gf[spin];
+  G = GFRep();
+  G.kind = 'F';
+  G.n = ns;
+  const int np = ns * (ns + 1) / 2, ni = ns * (ns - 1) / 2;
+  G.ea.assign(np, {});
+  G.eb.assign(np, {});
+  G.ha.assign(np, {});
+  G.hb.assign(np, {});
+  G.eai.assign(ni, {});
+  G.ebi.assign(ni, {});
+  G.hai.assign(ni, {});
+  G.hbi.assign(ni, {});
+  for (int pm = -1; pm <= 1; pm += 2) {
+    check_cancel();
+    Sector tgt = shift_sector(st.sec, pm, spin);
+    if (!sector_valid(L, tgt)) continue;
+    const size_t dt = sector_dim(L, tgt);
+    if (dt == 0) continue;
+    auto H = make_hamiltonian(T, tgt, I.format, true);
+    std::vector<DevBuf<double>> phi;
+    for (int i = 0; i < ns; ++i) {
+      phi.emplace_back(2 * dt);
+      fill(2 * dt, phi.back().p, 0.0, stream());
+      cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p, 1.0, phi.back().p);
+      cdag_apply(L, pm, i + spin * L, st.sec, st.psi.p + ds, 1.0, phi.back().p + dt);
+    }
+    DevBuf<double> psi(2 * dt);
+    // the fraction of the normalized psi, transformed as the reference's
+    // continued_fraction constructor does (src_ed/continued_fraction.cpp:33-42)
+    auto fraction = [&](std::vector<double> &a, std::vector<double> &b) {
+      const double nrm = ddot(2 * dt, psi.p, psi.p);
+      if (!(nrm > 1e-14)) return;
+      scale(2 * dt, psi.p, 1.0 / std::sqrt(nrm), stream());
+      lanczos_green(*H, psi.p, a, b);
+      for (auto &x : b) x *= x;
+      b[0] = nrm * st.weight;
+      if (pm == 1) for (auto &x : a) x -= st.energy;
+      else for (auto &x : a) x = st.energy - x;
+    };
+    int q = 0;
+    for (int o1 = 0; o1 < ns; ++o1)
+      for (int o2 = 0; o2 <= o1; ++o2, ++q) {
+        std::vector<const double *> src{phi[o1].p};
+        std::vector<double> cf{1.0};
+        if (o2 != o1) {
+          src.push_back(phi[o2].p);
+          cf.push_back(1.0);
+        }
+        combine(2 * dt, psi.p, 0.0, (int)src.size(), src.data(), cf.data(), nullptr, nullptr, nullptr, stream());
+        fraction(pm == 1 ? G.ea[q] : G.ha[q], pm == 1 ? G.eb[q] : G.hb[q]);
+        if (o2 == o1) continue;
+        // c_o2 -+ i c_o1: [b_r + s a_i | b_i - s a_r] with s = +1 (electrons), -1 (holes)
+        const double sg = pm == 1 ? 1.0 : -1.0;
+        const double *pa = phi[o1].p, *pb = phi[o2].p;
+        std::vector<const double *> s1{pb, pa + dt}, s2{pb + dt, pa};
+        std::vector<double> c1{1.0, sg}, c2{1.0, -sg};
+        combine(dt, psi.p, 0.0, 2, s1.data(), c1.data(), nullptr, nullptr, nullptr, stream());
+        combine(dt, psi.p + dt, 0.0, 2, s2.data(), c2.data(), nullptr, nullptr, nullptr, stream());
+        const int qi = o1 * (o1 - 1) / 2 + o2;
+        fraction(pm == 1 ? G.eai[qi] : G.hai[qi], pm == 1 ? G.ebi[qi] : G.hbi[qi]);
+      }
+  }
+}
+
 static void build_gf_complex(Instance &I, State &st, int spin) {
+  if (I.gf_method == 'F') {
+    build_cf_complex(I, st, spin);
+    return;
+  }
   if (I.gf_method != 'L')
     fail(ED_ERR_UNSUPPORTED, std::string("GF_method ") + I.gf_method +
-                                 " in a complex Hilbert space is not supported (use L)");
+                                 " in a complex Hilbert space is not supported (use L or F)");
   const int L = I.model->L, ns = I.model->n_sites;
   const TermList T = I.terms();
   const size_t ds = sector_dim(L, st.sec);
@@ -1218,22 +1298,30 @@ void cf_import(Instance &I, double E0, const Sector &sec, int spin, const std::v
 // the fractions of one representation, appended to a packed list
 static void pack_cf(const GFRep &G, int n, std::vector<int> &off, std::vector<double> &A, std::vector<double> &B) {
   const int np = n * (n + 1) / 2;
-  for (int q = 0; q < np; ++q)
-    for (int eh = 0; eh < 2; ++eh) {
-      const auto &a = eh ? G.ha[q] : G.ea[q];
-      const auto &b = eh ? G.hb[q] : G.eb[q];
-      A.insert(A.end(), a.begin(), a.end());
-      B.insert(B.end(), b.begin(), b.begin() + a.size());
-      off.push_back((int)A.size());
-    }
+  auto put = [&](const std::vector<double> &a, const std::vector<double> &b) {
+    A.insert(A.end(), a.begin(), a.end());
+    B.insert(B.end(), b.begin(), b.begin() + a.size());
+    off.push_back((int)A.size());
+  };
+  for (int q = 0; q < np; ++q) {
+    put(G.ea[q], G.eb[q]);
+    put(G.ha[q], G.hb[q]);
+  }
+  // complex Hilbert space: the pairs' imaginary combinations follow
+  for (size_t q = 0; q < G.eai.size(); ++q) {
+    put(G.eai[q], G.ebi[q]);
+    put(G.hai[q], G.hbi[q]);
+  }
 }
 
 // continued-fraction representations summed into one G block of n x n per
-// frequency, at offset `at` (doubles) of the output
+// frequency, at offset `at` (doubles) of the output; cx: the representations
+// hold the imaginary combinations too (complex Hilbert space)
 struct CfGroup {
   std::vector<const GFRep *> reps;
   int n = 0;
   size_t at = 0;
+  int cx = 0;
 };
 
 // the groups' fractions evaluated in one launch and assembled on the device
@@ -1250,7 +1338,7 @@ static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const dou
   }
   for (auto &kv : by_store) {
     const CfStore *st = kv.first;
-    std::vector<int> nn, pbeg{0}, frac0;
+    std::vector<int> nn, pbeg{0}, frac0, cxg;
     std::vector<uint64_t> at;
     DevBuf<int> doff;
     DevBuf<double> dA, dB;
@@ -1270,6 +1358,7 @@ static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const dou
         pbeg.push_back((int)frac0.size());
         nn.push_back(g->n);
         at.push_back(g->at);
+        cxg.push_back(0);
       }
       ncf = fe - fb;
       off = st->off.p + fb;
@@ -1279,10 +1368,16 @@ static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const dou
       size_t na = 0, nf = 0;
       for (const CfGroup *g : kv.second)
         for (const GFRep *r : g->reps)
+        {
           for (size_t q = 0; q < r->ea.size(); ++q) {
             na += r->ea[q].size() + r->ha[q].size();
             nf += 2;
           }
+          for (size_t q = 0; q < r->eai.size(); ++q) {
+            na += r->eai[q].size() + r->hai[q].size();
+            nf += 2;
+          }
+        }
       std::vector<int> hoff;
       std::vector<double> hA, hB;
       hoff.reserve(nf + 1);
@@ -1297,6 +1392,7 @@ static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const dou
         pbeg.push_back((int)frac0.size());
         nn.push_back(g->n);
         at.push_back(g->at);
+        cxg.push_back(g->cx);
       }
       ncf = (int)hoff.size() - 1;
       doff.upload(hoff);
@@ -1306,15 +1402,16 @@ static void eval_cf_groups(const std::vector<CfGroup> &groups, int nz, const dou
       A = dA.p;
       B = dB.p;
     }
-    DevBuf<int> dn, dpb, dfr;
+    DevBuf<int> dn, dpb, dfr, dcx;
     DevBuf<uint64_t> dat;
     DevBuf<double> dV((size_t)nz * ncf * 2);
     dn.upload(nn);
     dpb.upload(pbeg);
     dfr.upload(frac0);
     dat.upload(at);
+    dcx.upload(cxg);
     cf_eval(ncf, off, A, B, nz, dz, dV.p, stream());
-    cf_assemble((int)kv.second.size(), dn.p, dat.p, dpb.p, dfr.p, ncf, nz, dV.p, dG, stream());
+    cf_assemble((int)kv.second.size(), dn.p, dat.p, dpb.p, dfr.p, dcx.p, ncf, nz, dV.p, dG, stream());
   }
 }
 
@@ -1346,6 +1443,7 @@ void green_function_batch(const std::vector<Instance *> &insts, int nz, const do
     bool all_cf = true;
     for (auto &s : I.states) {
       all_cf = all_cf && s->gf[spin].kind == 'F';
+      if (!s->gf[spin].eai.empty()) g.cx = 1;
       g.reps.push_back(&s->gf[spin]);
     }
     if (all_cf) groups.push_back(std::move(g));
@@ -1385,6 +1483,7 @@ void Instance::green_function(int nz, const double *z, bool spin_down, double *o
     } else if (G.kind == 'F') {
       CfGroup g;
       g.n = n;
+      g.cx = G.eai.empty() ? 0 : 1;
       g.reps.push_back(&G);
       DevBuf<double> dG((size_t)nz * n * n * 2);
       eval_cf_groups({g}, nz, dz.p, dG.p);

@@ -35,6 +35,10 @@ struct GFRep {
   // continued fractions per ordered pair (o1 >= o2) for electrons / holes; a, b
   // already transformed (continued_fraction ctor, src_ed/continued_fraction.cpp:33-42)
   std::vector<std::vector<double>> ea, eb, ha, hb;
+  // complex Hilbert space: for each pair a > b (index a (a - 1) / 2 + b) the
+  // fractions of c_b - i c_a (electrons, c^dagger) and c_b + i c_a (holes),
+  // as the reference's build_cf for o1 < o2; empty otherwise
+  std::vector<std::vector<double>> eai, ebi, hai, hbi;
   // or, when `store` is set, fractions frac0 + 2 q (electron) and frac0 + 2 q + 1
   // (hole) of that device store, with ea .. hb empty
   std::shared_ptr<const CfStore> store;

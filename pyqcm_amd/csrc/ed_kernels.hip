@@ -1778,9 +1778,11 @@ void cdag_gather(const CdagArgs &A, const double *x, double *y, hipStream_t s) {
 
 // one thread per (group, frequency, matrix element)
 __global__ void k_cf_assemble(int ngroups, const int *nn, const uint64_t *at, const int *pbeg, const int *frac0,
-                              int ncf, int nz, const double *vals, double *G) {
+                              const int *cx, int ncf, int nz, const double *vals, double *G) {
   for (int g = blockIdx.y; g < ngroups; g += gridDim.y) {
     const int n = nn[g];
+    const bool cplx = cx && cx[g];
+    const int np = n * (n + 1) / 2;
     const size_t nsq = (size_t)n * n, m = nsq * nz;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += (size_t)gridDim.x * blockDim.x) {
       const int iz = (int)(t / nsq), e = (int)(t - (size_t)iz * nsq);
@@ -1795,11 +1797,28 @@ __global__ void k_cf_assemble(int ngroups, const int *nn, const uint64_t *at, co
         if (a == b) {
           gr += ar;
           gi += ai;
-        } else {
+        } else if (!cplx) {
           const double br = f[4 * qbb] + f[4 * qbb + 2], bi = f[4 * qbb + 1] + f[4 * qbb + 3];
           const double cr = f[4 * qab] + f[4 * qab + 2], ci = f[4 * qab + 1] + f[4 * qab + 3];
           gr += 0.5 * (cr - ar - br);
           gi += 0.5 * (ci - ai - bi);
+        } else {
+          // G(b, a) = (t1 + t2 - (1 + i) s) / 2, G(a, b) = (t1 - t2 - (1 - i) s) / 2
+          // with t1 the (a, b) pair's fraction, t2 = i times its imaginary
+          // combination's, s = G_aa + G_bb; here (o1, o2) is (row, column)
+          const int qi = np + (a * (a - 1) / 2 + b);
+          const double br = f[4 * qbb] + f[4 * qbb + 2], bi = f[4 * qbb + 1] + f[4 * qbb + 3];
+          const double t1r = f[4 * qab] + f[4 * qab + 2], t1i = f[4 * qab + 1] + f[4 * qab + 3];
+          const double vr = f[4 * qi] + f[4 * qi + 2], vi = f[4 * qi + 1] + f[4 * qi + 3];
+          const double t2r = -vi, t2i = vr;
+          const double sr = ar + br, si = ai + bi;
+          if (o1 < o2) {  // row b, column a
+            gr += 0.5 * (t1r + t2r - (sr - si));
+            gi += 0.5 * (t1i + t2i - (si + sr));
+          } else {        // row a, column b
+            gr += 0.5 * (t1r - t2r - (sr + si));
+            gi += 0.5 * (t1i - t2i - (si - sr));
+          }
         }
       }
       G[at[g] + 2 * t] = gr;
@@ -1817,11 +1836,11 @@ void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, 
   ED_HIP(hipGetLastError());
 }
 
-void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, int ncf, int nz,
-                 const double *vals, double *G, hipStream_t s) {
+void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, const int *cx,
+                 int ncf, int nz, const double *vals, double *G, hipStream_t s) {
   if (ngroups <= 0 || nz <= 0) return;
   const dim3 grid(64, (unsigned)(ngroups < 65535 ? ngroups : 65535));
-  hipLaunchKernelGGL(k_cf_assemble, grid, dim3(256), 0, s, ngroups, n, at, pbeg, frac0, ncf, nz, vals, G);
+  hipLaunchKernelGGL(k_cf_assemble, grid, dim3(256), 0, s, ngroups, n, at, pbeg, frac0, cx, ncf, nz, vals, G);
   ED_HIP(hipGetLastError());
 }
 

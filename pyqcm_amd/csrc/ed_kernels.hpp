@@ -235,8 +235,12 @@ void cf_eval(int ncf, const int *off, const double *a, const double *b, int nz, 
 // (o1 >= o2), electron then hole.  G_aa = F_aa, G_ab = (F_ab - F_aa - F_bb) / 2
 // with F = electron + hole, summed over the group's states, written at
 // G + at[g] (doubles) as nz column-major n[g] x n[g] complex matrices
-void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, int ncf, int nz,
-                 const double *vals, double *G, hipStream_t s);
+// cx[g] != 0: group g's representations also hold, after their n (n + 1) / 2
+// pairs, the fractions of the pairs' imaginary combinations (a > b, index
+// a (a - 1) / 2 + b), assembled as continued_fraction_set::Green_function does
+// for a complex Hilbert space
+void cf_assemble(int ngroups, const int *n, const uint64_t *at, const int *pbeg, const int *frac0, const int *cx,
+                 int ncf, int nz, const double *vals, double *G, hipStream_t s);
 // G[b + a L] (nz blocks) = sum_i V(a, i) conj(V(b, i)) / (z - e_i), V real or
 // with imaginary parts Vi
 void vdvh(int L, int M, const double *e, const double *V, int nz, const double *z, double *G, hipStream_t s,

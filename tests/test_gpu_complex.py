@@ -246,6 +246,45 @@ def test_flux_ring_against_dense(fmt, lanczos_block, band_block):
             qcm.set_global_parameterC(k, v)
 
 
+@pytest.mark.parametrize("fmt", ["G", "N"])
+def test_continued_fractions_in_a_complex_space(fmt):
+    """GF_method 'F' (build_cf with HilbertField = Complex,
+    src_ed/model_instance.cpp:101-177; assembly of continued_fraction_set::Green_function,
+    src_ed/continued_fraction_set.cpp:63-98): the fractions of c_o1 (+ c_o2) and
+    of c_o2 -+ i c_o1, run to the exhaustion of their Krylov spaces
+    (accur_continued_fraction 1e-14), give the dense G of the flux ring and,
+    on the reference's fixture, the printed Q-matrix's G"""
+    L = 4
+    ops = _ring_ops(L, 0.7)
+    vals = {"t": 1.0, "U": 4.0, "mu": 1.5, "tp": 1.0}
+    qcm.set_global_parameterC("Hamiltonian_format", fmt)
+    qcm.set_global_parameterC("GF_method", "F")
+    qcm.set_global_parameterC("accur_continued_fraction", 1e-14)
+    try:
+        lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
+        E0, _ = qcm.ground_state_solve(lab)
+        E0d, psi, Gd, _ = _dense_ring(L, ops, vals, 2, 2)
+        assert abs(E0 - E0d) < 1e-10 * abs(E0d)
+        G = _gf(lab)
+        assert np.abs(G - Gd).max() < 1e-8, np.abs(G - Gd).max()
+        qcm.erase_model_instance(lab)
+        p = _FIX["parameters"]
+        lab = _instance("cx_chain8", 8, 0, _chain8_ops(), {"ti": p["ti"], "U": p["U"], "mu": p["mu"]}, _FIX["sector"])
+        qcm.ground_state_solve(lab)
+        q = _FIX["qmatrix"]
+        ef = np.array(q["poles"])
+        Qf = np.array([[complex(*x) for x in row] for row in q["residues"]]).T
+        G = _gf(lab)
+        for k, z in enumerate(_Z):
+            Gf = np.einsum("ci,ri,i->rc", Qf, Qf.conj(), 1.0 / (z - ef))
+            assert np.abs(G[k] - Gf).max() < 1e-8, (z, np.abs(G[k] - Gf).max())
+        qcm.erase_model_instance(lab)
+    finally:
+        qcm.set_global_parameterC("Hamiltonian_format", "G")
+        qcm.set_global_parameterC("GF_method", "L")
+        qcm.set_global_parameterC("accur_continued_fraction", 0.01)
+
+
 # ---------------------------------------------------------------------------
 # free cluster with a complex bath: self-energy, hybridization, Potthoff
 # ---------------------------------------------------------------------------
@@ -310,7 +349,7 @@ def test_complex_unsupported_paths():
             fn()
         assert err.value.code == 8
     qcm.erase_model_instance(lab)
-    qcm.set_global_parameterC("GF_method", "F")
+    qcm.set_global_parameterC("GF_method", "M")
     try:
         lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
         with pytest.raises(EDError) as err:
