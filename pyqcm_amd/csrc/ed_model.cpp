@@ -947,7 +947,12 @@ struct HamFactorized : Hamiltonian {
     BD = half_basis(L, Ndw);
     dim = (size_t)BU->dim * BD->dim;
     format = 'F';
-    HalfHost hu = build_half(T, 0, *BU), hd = build_half(T, 1, *BD);
+    // equal spin populations and no term that tells the spins apart: the
+    // down tables are the up ones
+    bool same = Nup == Ndw;
+    for (const auto &tc : T.terms) same = same && !(tc.second != 0.0 && tc.first->spin_asym);
+    HalfHost hu = build_half(T, 0, *BU);
+    HalfHost hd = same ? hu : build_half(T, 1, *BD);
     up.upload(hu, *BU);
     dn.upload(hd, *BD);
     args.up = up.view;
