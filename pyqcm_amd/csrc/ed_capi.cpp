@@ -211,7 +211,11 @@ int ed_new_model_instance(const char *model, int n_values, const char *const *na
 }
 
 int ed_erase_model_instance(int label) {
-  return guard([&] { erase_instance(label); });
+  return guard([&] {
+    erase_instance(label);
+    std::lock_guard<std::mutex> l(g_label_mu);  // no call holds the label's lock now
+    g_label_locks.erase(label);
+  });
 }
 
 int ed_ground_state_solve(int label, double *E0, char *sector, int cap) {

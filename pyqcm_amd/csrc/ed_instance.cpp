@@ -307,7 +307,11 @@ std::pair<double, std::string> Instance::ground_state_solve() {
       gf_hams.clear();
       for (int pm = -1; pm <= 1; pm += 2) {
         const Sector tgt = shift_sector(s, pm, 0);
-        if (!sector_valid(model->L, tgt) || sector_dim(model->L, tgt) == 0) continue;
+        // small sectors only: a large one's tables take longer than nothing
+        // here, and its schedule timing would run beside the Lanczos
+        if (!sector_valid(model->L, tgt) || sector_dim(model->L, tgt) == 0 ||
+            sector_dim(model->L, tgt) > ((size_t)1 << 24))
+          continue;
         const char fmt = format;
         gf_hams[tgt] = std::async(std::launch::async, [T, tgt, fmt]() {
           auto Ht = make_hamiltonian(T, tgt, fmt);
