@@ -78,10 +78,10 @@ int ed_new_operator(const char *model, const char *name, const char *type, int n
    complex values as (re, im) pairs, elements r <= c, the lower triangle their conjugates
    (an imaginary diagonal element is ED_ERR_INVALID: not Hermitian).  An instance with a
    nonzero value on such an operator works in a complex Hilbert space (qcm_ED.cpp:145-157):
-   ground state, cluster averages, Green function by band Lanczos (GF_method 'L') or continued
-   fractions ('F'), Q-matrix, self-energy, hybridization, Potthoff functional; Davidson
-   (n_states > 1), point groups, GF_method 'M', susceptibilities, fidelity, density matrices,
-   wavefunction printing,
+   ground state (Lanczos, or Davidson for n_states > 1), cluster averages, Green function by
+   band Lanczos (GF_method 'L') or continued fractions ('F'), Q-matrix, self-energy,
+   hybridization, Potthoff functional; point groups, GF_method 'M', susceptibilities,
+   fidelity, density matrices, wavefunction printing,
    continued-fraction export / import, batched solves and the HDF5 format return
    ED_ERR_UNSUPPORTED there.  Hamiltonian handles of such an instance act on split vectors
    [Re | Im] of twice the sector dimension. */

@@ -277,6 +277,35 @@ void random_start(size_t n, long seed, std::vector<double> &x) {
   for (auto &v : x) v *= f;
 }
 
+// Distinct complex states among eigenpairs of the split plan [[A, -B], [B, A]]
+// (ascending; split vectors [x_r | x_i] of 2 n doubles, unit norm): each
+// vector is orthogonalized, in the complex inner product
+// <u|w> = u_r.w_r + u_i.w_i + i (u_r.w_i - u_i.w_r), against the states kept
+// so far; i psi, the partner of a kept psi, leaves nothing and is dropped, a
+// new state leaves a vector of norm near 1 and is kept (normalized), until k
+// states are kept.
+static void complex_distinct(size_t n, const std::vector<double> &ev2, std::vector<DevBuf<double>> &v2, int k,
+                             std::vector<double> &ev, std::vector<DevBuf<double>> &vecs) {
+  for (size_t i = 0; i < ev2.size() && (int)vecs.size() < k; ++i) {
+    double *w = v2[i].p;
+    for (const auto &u : vecs) {
+      const double cr = ddot(n, u.p, w) + ddot(n, u.p + n, w + n);
+      const double ci = ddot(n, u.p, w + n) - ddot(n, u.p + n, w);
+      // w -= (cr + i ci) u
+      const double *s1[2] = {u.p, u.p + n}, *s2[2] = {u.p + n, u.p};
+      const double c1[2] = {-cr, ci}, c2[2] = {-cr, -ci};
+      combine(n, w, 1.0, 2, s1, c1, nullptr, nullptr, nullptr, stream());
+      combine(n, w + n, 1.0, 2, s2, c2, nullptr, nullptr, nullptr, stream());
+    }
+    const double nn = ddot(2 * n, w, w);
+    if (!(nn > 0.5)) continue;
+    scale(2 * n, w, 1.0 / std::sqrt(nn), stream());
+    ev.push_back(ev2[i]);
+    vecs.push_back(std::move(v2[i]));
+  }
+  if ((int)vecs.size() < k) fail(ED_ERR_NOT_CONVERGED, "Davidson: fewer distinct complex states than requested");
+}
+
 std::pair<double, std::string> Instance::ground_state_solve() {
   if (gs_solved) return {GS_energy, gs_string()};
   Params &P = params();
@@ -323,9 +352,6 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     }
     long ns = std::max<long>(1, std::min<long>(n_states, (long)dim));
     char meth = (method == 'L' && ns > 1) ? 'D' : method;
-    // each level of the split form appears twice (psi and i psi)
-    if (cx && meth == 'D')
-      fail(ED_ERR_UNSUPPORTED, "several states per sector (Davidson) in a complex Hilbert space are not supported");
     // a point group: the sector's states are those of the full (N, S) sector in
     // representation s.irrep
     std::unique_ptr<GroupProjector> proj;
@@ -337,7 +363,14 @@ std::pair<double, std::string> Instance::ground_state_solve() {
     std::vector<double> ev;
     std::vector<DevBuf<double>> vecs;
     vlog("ground state: sector %s dim %zu, Hamiltonian ready", s.name().c_str(), dim);
-    if (meth == 'D') {
+    if (meth == 'D' && cx) {
+      // each level of the split form appears twice (psi and i psi): twice
+      // the states, then the distinct complex ones
+      std::vector<double> ev2;
+      std::vector<DevBuf<double>> v2;
+      davidson(*H, 2 * (int)ns, acc_dav, ev2, v2);
+      complex_distinct(dim / 2, ev2, v2, (int)ns, ev, vecs);
+    } else if (meth == 'D') {
       davidson(*H, (int)ns, acc_dav, ev, vecs);
       if (verbose()) {
         std::ostringstream os;

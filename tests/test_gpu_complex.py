@@ -285,6 +285,52 @@ def test_continued_fractions_in_a_complex_space(fmt):
         qcm.set_global_parameterC("accur_continued_fraction", 0.01)
 
 
+def test_several_states_at_finite_temperature():
+    """n_states 3 (Davidson on the split plan for twice the states, then the
+    distinct complex ones) at temperature 1.5: the kept states' energies and
+    Boltzmann weights, and the thermal G = sum_s w_s G_s, against the dense
+    complex Hamiltonian of the flux ring.  The restarted Davidson iteration
+    (src_ed/Hamiltonian/Davidson.hpp:38-230) stalls near a residual of 1e-7
+    in double precision, on real Hamiltonians too (the preconditioned
+    correction tends to the Ritz vector itself): at accur_Davidson 1e-7 the
+    energies are exact to second order, the states and G to about that."""
+    L = 4
+    ops = _ring_ops(L, 0.7)
+    vals = {"t": 1.0, "U": 4.0, "mu": 1.5, "tp": 1.0}
+    for k, v in (("n_states", 3), ("temperature", 1.5), ("accur_Davidson", 1e-7)):
+        qcm.set_global_parameterC(k, v)
+    try:
+        lab = _instance("cx_ring4", L, 0, ops, vals, "R0:N4:S0")
+        qcm.ground_state_solve(lab)
+        e, w = qcm.states(lab)
+        H = dense_hamiltonian(L, _dense_terms(ops, vals), 2, 2)
+        ed, Vd = np.linalg.eigh(H)
+        assert len(e) == 3, e
+        assert np.abs(np.sort(e) - ed[:3]).max() < 1e-9, (e, ed[:3])
+        bw = np.exp(-(ed[:3] - ed[0]) / 1.5)
+        assert np.abs(np.sort(w)[::-1] - bw / bw.sum()).max() < 1e-9, (w, bw / bw.sum())
+        # thermal G from the dense states (each state's Lehmann sums)
+        He = dense_hamiltonian(L, _dense_terms(ops, vals), 3, 2)
+        Hh = dense_hamiltonian(L, _dense_terms(ops, vals), 1, 2)
+        we, Ve = np.linalg.eigh(He)
+        wh, Vh = np.linalg.eigh(Hh)
+        Gd = np.zeros((len(_Z), L, L), complex)
+        for s_ in range(3):
+            psi = Vd[:, s_]
+            ae = np.array([Ve.conj().T @ (dense_c(L, a, 2, 2, True) @ psi) for a in range(L)])
+            ah = np.array([Vh.conj().T @ (dense_c(L, a, 2, 2, False) @ psi) for a in range(L)])
+            for kz, z in enumerate(_Z):
+                Gs = np.einsum("rn,cn,n->rc", ae.conj(), ae, 1.0 / (z - (we - ed[s_]))) + \
+                    np.einsum("cm,rm,m->rc", ah.conj(), ah, 1.0 / (z + (wh - ed[s_])))
+                Gd[kz] += bw[s_] / bw.sum() * Gs
+        G = _gf(lab)
+        assert np.abs(G - Gd).max() < 1e-5, np.abs(G - Gd).max()
+        qcm.erase_model_instance(lab)
+    finally:
+        for k, v in (("n_states", 1), ("temperature", 0.0), ("accur_Davidson", 1e-5)):
+            qcm.set_global_parameterC(k, v)
+
+
 # ---------------------------------------------------------------------------
 # free cluster with a complex bath: self-energy, hybridization, Potthoff
 # ---------------------------------------------------------------------------
