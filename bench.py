@@ -527,6 +527,8 @@ def main():
     ap.add_argument("--no-lattice", action="store_true", help="skip the CPT / Potthoff-functional leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 4x4 ground-state solve")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="skip the concurrent-clusters part of the solve leg (kernel-trace profiles of one solve)")
     ap.add_argument("--solve-windows", action="store_true",
                     help="print each ED solve's host-clock window (and its phases) to stderr, for cutting a trace")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -649,7 +651,8 @@ def _solve_leg(args):
              "gpu_solves_per_s": round(1.0 / tg, 3), "gpu_s_per_solve": round(tg, 4), "E0_gpu": E0g,
              "timing": "median of 5 GPU solves after one warm-up; CPU: median of 5 after one warm-up, "
                        "measured in this run"}
-    solve["concurrent_clusters"] = ed_solve_concurrent()
+    if not args.no_concurrent:
+        solve["concurrent_clusters"] = ed_solve_concurrent()
     if not args.no_cpu:
         E0c, tc = ed_solve_cpu(wl2)
         import oracle.oracle as orc

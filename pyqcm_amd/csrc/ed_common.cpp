@@ -57,8 +57,8 @@ Params::Params() {
   b["store_lanczos_basis"] = true;  // keep Krylov vectors in HBM instead of replaying the recurrence
   b["poll_reductions"] = true;      // wait for reduction results by polling pinned memory (else stream sync)
   b["host_random_start"] = false;   // Lanczos start vector from the reference's host generator instead of the device one
-  i["lanczos_block"] = 6;           // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
-  i["band_lanczos_block"] = 8;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
+  i["lanczos_block"] = 12;          // Lanczos steps queued per block without a host wait (multiple of 3; 0: a wait per step)
+  i["band_lanczos_block"] = 6;      // band-Lanczos steps queued per block with their scalars computed on the device (< 2: a host wait per step)
   b["lanczos_graph"] = true;        // replay the block as a captured hipGraph
   i["gf_streams"] = 2;              // band-Lanczos Green function: hole and electron parts on 2 host threads / streams (1: one after the other)
   b["lanczos_reorth"] = false;      // full FP64 reorthogonalisation of the ground-state Lanczos (basis in HBM)

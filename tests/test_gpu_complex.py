@@ -41,7 +41,7 @@ def _device():
     for k, v in (("Hamiltonian_format", "G"), ("GF_method", "L")):
         qcm.set_global_parameterC(k, v)
     yield
-    for k, v in (("Hamiltonian_format", "G"), ("GF_method", "L"), ("lanczos_block", 6), ("band_lanczos_block", 8)):
+    for k, v in (("Hamiltonian_format", "G"), ("GF_method", "L"), ("lanczos_block", 12), ("band_lanczos_block", 6)):
         qcm.set_global_parameterC(k, v)
 
 
@@ -242,7 +242,7 @@ def test_flux_ring_against_dense(fmt, lanczos_block, band_block):
             assert abs(ave[op][1] - v) < 1e-8, (op, ave[op], v)
         qcm.erase_model_instance(lab)
     finally:
-        for k, v in (("Hamiltonian_format", "G"), ("lanczos_block", 6), ("band_lanczos_block", 8)):
+        for k, v in (("Hamiltonian_format", "G"), ("lanczos_block", 12), ("band_lanczos_block", 6)):
             qcm.set_global_parameterC(k, v)
 
 
@@ -379,7 +379,7 @@ def test_free_cluster_with_complex_bath(band_block):
         assert abs(omega - (E0 - 2 * eb[eb < 0].sum())) < 1e-10
         qcm.erase_model_instance(lab)
     finally:
-        qcm.set_global_parameterC("band_lanczos_block", 8)
+        qcm.set_global_parameterC("band_lanczos_block", 6)
 
 
 def test_complex_unsupported_paths():

@@ -23,8 +23,8 @@ def _device():
     qcm.set_global_parameterC("Hamiltonian_format", "G")
     qcm.set_global_parameterC("GF_method", "L")
     yield
-    qcm.set_global_parameterC("band_lanczos_block", 8)
-    qcm.set_global_parameterC("lanczos_block", 6)
+    qcm.set_global_parameterC("band_lanczos_block", 6)
+    qcm.set_global_parameterC("lanczos_block", 12)
     qcm.set_global_parameterC("lanczos_graph", True)
     qcm.set_global_parameterC("store_lanczos_basis", True)
 
@@ -164,4 +164,4 @@ def test_band_lanczos_device_steps_same_as_host(case):
             # remaining poles still move by that much)
             assert np.abs(_lehmann(e, Q, z) - Gh).max() < 1e-7, blk
     finally:
-        qcm.set_global_parameterC("band_lanczos_block", 8)
+        qcm.set_global_parameterC("band_lanczos_block", 6)
